@@ -1,0 +1,198 @@
+"""Gravity force-evaluation benchmark on one MI355X.
+
+One step = one pass of the hot path over the resident workload: GPU
+multipole build, zeroing of the accelerations, particle-cell (hexadecapole)
+kernel, particle-particle kernel and Ewald correction, on interaction lists
+already in HBM. Workload: the 128^3 periodic Zel'dovich box (2,097,152
+particles, Ewald nReplicas = 1, fEwCut = 2.6, |h|^2 <= 10, theta = 0.7,
+bucket 16). The metric is G interactions/s, an interaction being one (active
+target, source cell / particle / Ewald term) pair as ChaNGa counts them.
+
+Prints one JSON line. --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32 = 157.3e12  # MI355X FP32 vector peak (FLOP/s), packed issue
+PEAK_FP32_NONPACKED = 78.6e12
+FLOP = dict(pc=183, soft=23, pp=23, ewald_real=333, ewald_k=58)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--form", choices=["precise", "plain"], default="precise")
+    ap.add_argument("--ng", type=int, default=128, help="lattice size of the periodic box")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy legs")
+    ap.add_argument("--cpu-frac", type=float, default=0.02)
+    ap.add_argument("--samples", type=int, default=256)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design, not a measured path)"}))
+        sys.exit(2)
+    import torch
+
+    from changa_amd import configs, engine
+
+    t0 = time.time()
+    w = configs.zeldovich(ng=args.ng, rms=0.02, cfg=2)
+    ev = engine.Evaluation(w, precise=(args.form == "precise"))
+    setup_s = time.time() - t0
+    counts = ev.n_interactions()
+    n_int = sum(counts.values())
+    flops = sum(FLOP[k] * v for k, v in counts.items())
+
+    for _ in range(args.warmup):
+        ev.step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(args.steps):
+        ev.step()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    value = n_int / (ms * 1e-3) / 1e9
+
+    # per-kernel durations (HIP events on the launch stream), separate pass
+    kern = {}
+    for name, fn in [("moments", ev.build_moments), ("zero", ev.zero), ("pc", ev.run_pc),
+                     ("pp", ev.run_pp), ("ewald", ev.run_ewald)]:
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
+        for i in range(args.steps):
+            evs[2 * i].record(stream)
+            fn()
+            evs[2 * i + 1].record(stream)
+        torch.cuda.synchronize()
+        kern[name] = float(np.mean([evs[2 * i].elapsed_time(evs[2 * i + 1])
+                                    for i in range(args.steps)]))
+    pc_flop = FLOP["pc"] * counts["pc"] + FLOP["soft"] * counts["soft"]
+    pc_achieved = pc_flop / (kern["pc"] * 1e-3) / 1e12
+
+    out = {
+        "metric": "G interactions/s (PC+PP+Ewald) & % FP32 VALU peak, 1 GPU, at <=1e-4 force err",
+        "value": round(value, 2),
+        "unit": "G interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Zel'dovich lattice, numpy PCG64 seed 1002)",
+        "config": {
+            "workload": f"periodic Zel'dovich box {args.ng}^3 ({ev.pos.shape[0]} particles), "
+                        "P(k)~k^-2.5, rms 0.02, Ewald nReplicas=1 fEwCut=2.6 |h|^2<=10, "
+                        "theta=0.7, bucket 16, hexadecapole",
+            "form": args.form,
+            "step": "GPU multipole build + zero + PC + PP + Ewald on resident lists",
+            "interactions_per_step": counts,
+            "n_buckets": int(ev.tree["n_buckets"]),
+            "n_nodes": int(ev.tree["n_nodes"]),
+            "host_setup_s": round(setup_s, 2),
+        },
+        "fp32_valu_frac": round(flops / (ms * 1e-3) / PEAK_FP32, 4),
+        "algorithmic_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+        "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
+        "roofline": {
+            "bound": "mfma",
+            "pipe": "FP32 VALU (no matrix work; packed FP32 disabled, so the non-packed "
+                    "ceiling is 78.6 TFLOP/s)",
+            "kernel": "pc_kernel",
+            "achieved": round(pc_achieved, 3),
+            "peak": PEAK_FP32 / 1e12,
+            "unit": "TFLOP/s",
+            "frac": round(pc_achieved * 1e12 / PEAK_FP32, 4),
+            "traffic": None,
+        },
+    }
+
+    if not args.no_accuracy:
+        out["accuracy"] = accuracy(ev, args.samples)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ev, args.cpu_frac)
+    print(json.dumps(out))
+
+
+def accuracy(ev, n_samples):
+    """Force error of this GPU step against (a) the FP64 oracle on the same
+    lists and (b) an FP64 converged Ewald sum, on seeded samples."""
+    import torch
+
+    from oracle import oracle as O
+    from tests.helpers import oracle_forces, rel_err
+
+    ev.step()
+    torch.cuda.synchronize()
+    acc, pot, _ = ev.results()
+    rng = np.random.default_rng(7)
+    tr = ev.tree
+    bsel = np.sort(rng.choice(tr["n_buckets"], size=max(1, n_samples // 8), replace=False))
+    oa, op, _ = oracle_forces(ev, buckets=bsel)
+    idx = np.concatenate([np.arange(tr["bucket_start"][b], tr["bucket_start"][b] + tr["bucket_size"][b])
+                          for b in bsel])
+    e_or = rel_err(acc[idx], oa[idx])
+    samp = np.sort(rng.choice(ev.pos.shape[0], size=n_samples, replace=False)).astype(np.int32)
+    t = time.time()
+    xa, xp = O.ewald_exact(ev.pos, ev.mass, ev.soft, samp, L=ev.L, rcut=0.45 * ev.L)
+    e_ex = rel_err(acc[samp], xa)
+    return {
+        "vs_oracle_same_lists": {"particles": int(len(idx)), "median": float(np.median(e_or)),
+                                 "p99": float(np.percentile(e_or, 99)), "max": float(e_or.max())},
+        "vs_fp64_ewald": {"samples": int(n_samples), "median": float(np.median(e_ex)),
+                          "p99": float(np.percentile(e_ex, 99)), "max": float(e_ex.max()),
+                          "oracle_s": round(time.time() - t, 1)},
+    }
+
+
+def cpu_baseline(ev, frac):
+    """The FP64 oracle (nodeBucketForce/partBucketForce/BucketEwald
+    restatement) on a bounded sample of the same buckets and lists, on the
+    host cores."""
+    from oracle import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    rng = np.random.default_rng(11)
+    tr = ev.tree
+    nb = max(1, int(frac * tr["n_buckets"]))
+    bsel = np.sort(rng.choice(tr["n_buckets"], size=nb, replace=False))
+    geo, mom = O.build_moments(tr, ev.pos, ev.mass, ev.soft)
+    t = time.time()
+    _, _, _, c = O.forces_lists(tr, geo, mom, ev.pos, ev.mass, ev.soft, ev.lists,
+                                period=ev.L, n_threads=threads, buckets=bsel)
+    targets = np.concatenate([np.arange(tr["bucket_start"][b], tr["bucket_start"][b] + tr["bucket_size"][b])
+                              for b in bsel])
+    ewt = O.ewald_table(mom[0], geo[0, 3], ev.L, ev.w["ewh_cut"])
+    _, _, ce = O.bucket_ewald(targets, ev.pos, geo[0, 0:3], geo[0, 5], mom[0], geo[0, 3], ewt,
+                              ev.L, ev.w["n_replicas"], ev.w["ew_cut"], n_threads=threads)
+    dt = time.time() - t
+    n = int(c.sum() + ce.sum())
+    return {"value": round(n / dt / 1e9, 4), "unit": "G interactions/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{nb} of {tr['n_buckets']} buckets ({len(targets)} particles), "
+                      f"{n} interactions, {dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

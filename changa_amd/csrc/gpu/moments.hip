@@ -1,0 +1,327 @@
+// GPU multipole build: bucket P2M and level-synchronous M2M, in FP64.
+//
+// CPU semantics followed: GenericTreeNode::makeBucket (GenericTreeNode.h:
+// 219-257) with MultipoleMoments::operator+=(particle) (MultipoleMoments.h:
+// 276-352) for buckets, LocalTreeBuilder::doneChildren (Compute.cpp:
+// 2534-2580) with operator+=(MultipoleMoments) (MultipoleMoments.h:207-272)
+// for internal nodes, and calculateRadiusFarthestParticle /
+// FarthestCorner (MultipoleMoments.h:460-505) for the expansion radius. The
+// arithmetic kernels of the reference are momMakeFmomr, momShiftFmomr,
+// momScaledAddFmomr and momRescaleFmomr (moments.c:202-268, 772-846,
+// 1002-1058).
+//
+// Formulation. Moments are kept as "complete" symmetric components
+// C(a,b,c) (a x's, b y's, c z's; orders 0 and 2..4, 35 slots with order 1
+// unused). A bucket accumulates the raw moments sum_i m_i dx^a dy^b dz^c
+// about its centre of mass and projects them once onto the symmetric
+// trace-free (STF) subspace; the incremental per-particle shifting of the CPU
+// build is the same sum. An internal node shifts each child's STF moments by
+// the binomial rule for raw moments, R(a,b,c) = sum C(a,a1) C(b,b1) C(c,c1)
+// Q(a1,b1,c1) d^(a-a1, b-b1, c-c1) (the dipole about a centre of mass is
+// zero), adds the two children and projects onto the STF subspace again
+// (trace parts of lower orders vanish under the projection). Stored moments
+// are divided by radius^order (radius-scaled reduced moments).
+#include "common.h"
+
+namespace cgpu {
+
+// slot of the complete component (a,b,c)
+__host__ __device__ constexpr int cidx(int a, int b, int c) {
+  return (a + b + c == 0 ? 0 : a + b + c == 1 ? 1 : a + b + c == 2 ? 4 : a + b + c == 3 ? 10 : 20) +
+         (b + c) * (b + c + 1) / 2 + c;
+}
+__host__ __device__ constexpr int binom(int n, int k) {
+  return k < 0 || k > n ? 0 : (k == 0 || k == n) ? 1 : binom(n - 1, k - 1) + binom(n - 1, k);
+}
+__host__ __device__ constexpr int pairs(int a) { return a * (a - 1) / 2; }
+
+// Project raw symmetric components of orders 2..4 onto the STF subspace.
+__device__ __forceinline__ void stf_project(double* T) {
+  // order 2: T - delta tr/3
+  double tr = T[cidx(2, 0, 0)] + T[cidx(0, 2, 0)] + T[cidx(0, 0, 2)];
+  T[cidx(2, 0, 0)] -= tr / 3.0;
+  T[cidx(0, 2, 0)] -= tr / 3.0;
+  T[cidx(0, 0, 2)] -= tr / 3.0;
+  // order 3: T - (1/5) sum_pairs delta V, V_i = T_ikk
+  double V[3] = {T[cidx(3, 0, 0)] + T[cidx(1, 2, 0)] + T[cidx(1, 0, 2)],
+                 T[cidx(2, 1, 0)] + T[cidx(0, 3, 0)] + T[cidx(0, 1, 2)],
+                 T[cidx(2, 0, 1)] + T[cidx(0, 2, 1)] + T[cidx(0, 0, 3)]};
+#pragma unroll
+  for (int a = 0; a <= 3; ++a)
+#pragma unroll
+    for (int b = 0; b <= 3 - a; ++b) {
+      const int c = 3 - a - b;
+      double s = 0.0;
+      if (a >= 2) s += pairs(a) * V[(a - 2) == 1 ? 0 : (b == 1 ? 1 : 2)];
+      if (b >= 2) s += pairs(b) * V[a == 1 ? 0 : ((b - 2) == 1 ? 1 : 2)];
+      if (c >= 2) s += pairs(c) * V[a == 1 ? 0 : (b == 1 ? 1 : 2)];
+      T[cidx(a, b, c)] -= s / 5.0;
+    }
+  // order 4: T - (1/7) sum_pairs delta W + (1/35) w (delta delta), W_ij = T_ijkk
+  double W[6];  // xx xy xz yy yz zz
+  W[0] = T[cidx(4, 0, 0)] + T[cidx(2, 2, 0)] + T[cidx(2, 0, 2)];
+  W[1] = T[cidx(3, 1, 0)] + T[cidx(1, 3, 0)] + T[cidx(1, 1, 2)];
+  W[2] = T[cidx(3, 0, 1)] + T[cidx(1, 2, 1)] + T[cidx(1, 0, 3)];
+  W[3] = T[cidx(2, 2, 0)] + T[cidx(0, 4, 0)] + T[cidx(0, 2, 2)];
+  W[4] = T[cidx(2, 1, 1)] + T[cidx(0, 3, 1)] + T[cidx(0, 1, 3)];
+  W[5] = T[cidx(2, 0, 2)] + T[cidx(0, 2, 2)] + T[cidx(0, 0, 4)];
+  const double w = W[0] + W[3] + W[5];
+#pragma unroll
+  for (int a = 0; a <= 4; ++a)
+#pragma unroll
+    for (int b = 0; b <= 4 - a; ++b) {
+      const int c = 4 - a - b;
+      // W of the multiset left after removing an equal-index pair
+      auto Wof = [&](int p, int q, int r) -> double {
+        return p == 2 ? W[0] : q == 2 ? W[3] : r == 2 ? W[5] : (p == 1 && q == 1) ? W[1]
+                                                              : (p == 1) ? W[2] : W[4];
+      };
+      double s = 0.0;
+      if (a >= 2) s += pairs(a) * Wof(a - 2, b, c);
+      if (b >= 2) s += pairs(b) * Wof(a, b - 2, c);
+      if (c >= 2) s += pairs(c) * Wof(a, b, c - 2);
+      int dd = (a == 4 || b == 4 || c == 4) ? 3 : ((a == 2) + (b == 2) + (c == 2) == 2 ? 1 : 0);
+      T[cidx(a, b, c)] += -s / 7.0 + w * dd / 35.0;
+    }
+}
+
+// reduced (FMOMR order) <-> complete
+__device__ __forceinline__ void complete_from_reduced(const double* q, double* C) {
+  C[0] = q[0];
+  C[1] = C[2] = C[3] = 0.0;
+  C[cidx(2, 0, 0)] = q[1]; C[cidx(0, 2, 0)] = q[2]; C[cidx(1, 1, 0)] = q[3];
+  C[cidx(1, 0, 1)] = q[4]; C[cidx(0, 1, 1)] = q[5];
+  C[cidx(3, 0, 0)] = q[6]; C[cidx(1, 2, 0)] = q[7]; C[cidx(2, 1, 0)] = q[8];
+  C[cidx(0, 3, 0)] = q[9]; C[cidx(2, 0, 1)] = q[10]; C[cidx(0, 2, 1)] = q[11];
+  C[cidx(1, 1, 1)] = q[12];
+  C[cidx(4, 0, 0)] = q[13]; C[cidx(1, 3, 0)] = q[14]; C[cidx(3, 1, 0)] = q[15];
+  C[cidx(0, 4, 0)] = q[16]; C[cidx(3, 0, 1)] = q[17]; C[cidx(0, 3, 1)] = q[18];
+  C[cidx(2, 2, 0)] = q[19]; C[cidx(2, 1, 1)] = q[20]; C[cidx(1, 2, 1)] = q[21];
+  // z-heavy components from tracelessness, in increasing c
+#pragma unroll
+  for (int n = 2; n <= 4; ++n)
+#pragma unroll
+    for (int c = 2; c <= n; ++c)
+#pragma unroll
+      for (int a = 0; a <= n - c; ++a) {
+        const int b = n - c - a;
+        C[cidx(a, b, c)] = -(C[cidx(a + 2, b, c - 2)] + C[cidx(a, b + 2, c - 2)]);
+      }
+}
+
+__device__ __forceinline__ void reduced_from_complete(const double* C, double* q) {
+  q[0] = C[0];
+  q[1] = C[cidx(2, 0, 0)]; q[2] = C[cidx(0, 2, 0)]; q[3] = C[cidx(1, 1, 0)];
+  q[4] = C[cidx(1, 0, 1)]; q[5] = C[cidx(0, 1, 1)];
+  q[6] = C[cidx(3, 0, 0)]; q[7] = C[cidx(1, 2, 0)]; q[8] = C[cidx(2, 1, 0)];
+  q[9] = C[cidx(0, 3, 0)]; q[10] = C[cidx(2, 0, 1)]; q[11] = C[cidx(0, 2, 1)];
+  q[12] = C[cidx(1, 1, 1)];
+  q[13] = C[cidx(4, 0, 0)]; q[14] = C[cidx(1, 3, 0)]; q[15] = C[cidx(3, 1, 0)];
+  q[16] = C[cidx(0, 4, 0)]; q[17] = C[cidx(3, 0, 1)]; q[18] = C[cidx(0, 3, 1)];
+  q[19] = C[cidx(2, 2, 0)]; q[20] = C[cidx(2, 1, 1)]; q[21] = C[cidx(1, 2, 1)];
+}
+
+// R += shift of STF moments C (unscaled, about a centre displaced by d from
+// the new centre) to the new centre, raw (not yet projected).
+__device__ __forceinline__ void shift_add_raw(const double* C, const double d[3], double* R) {
+  double px[5], py[5], pz[5];
+  px[0] = py[0] = pz[0] = 1.0;
+#pragma unroll
+  for (int k = 1; k < 5; ++k) {
+    px[k] = px[k - 1] * d[0];
+    py[k] = py[k - 1] * d[1];
+    pz[k] = pz[k - 1] * d[2];
+  }
+  R[0] += C[0];
+#pragma unroll
+  for (int n = 2; n <= 4; ++n)
+#pragma unroll
+    for (int a = 0; a <= n; ++a)
+#pragma unroll
+      for (int b = 0; b <= n - a; ++b) {
+        const int c = n - a - b;
+        double s = 0.0;
+#pragma unroll
+        for (int a1 = 0; a1 <= a; ++a1)
+#pragma unroll
+          for (int b1 = 0; b1 <= b; ++b1)
+#pragma unroll
+            for (int c1 = 0; c1 <= c; ++c1) {
+              const int k = a1 + b1 + c1;
+              if (k == 1) continue;
+              s += (double)(binom(a, a1) * binom(b, b1) * binom(c, c1)) * C[cidx(a1, b1, c1)] *
+                   px[a - a1] * py[b - b1] * pz[c - c1];
+            }
+        R[cidx(a, b, c)] += s;
+      }
+}
+
+// write node outputs: node64 record, wire moments, cm64
+__device__ __forceinline__ void emit(int nd, const double* g, const double* C, double u,
+                                     CudaMultipoleMoments* out, double* node64, double* cm64) {
+  double q[22];
+  reduced_from_complete(C, q);
+  const double iu = 1.0 / u;
+  const double s2 = iu * iu, s3 = s2 * iu, s4 = s3 * iu;
+#pragma unroll
+  for (int k = 1; k < 22; ++k) q[k] *= (k < 6 ? s2 : k < 13 ? s3 : s4);
+  double* rec = node64 + (size_t)nd * 34;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) rec[k] = g[k];
+  rec[3] = u;
+#pragma unroll
+  for (int k = 0; k < 22; ++k) rec[12 + k] = q[k];
+  if (cm64) {
+    cm64[3 * (size_t)nd] = g[0];
+    cm64[3 * (size_t)nd + 1] = g[1];
+    cm64[3 * (size_t)nd + 2] = g[2];
+  }
+  CudaMultipoleMoments m;
+  m.radius = (float)u;
+  m.soft = (float)g[4];
+  m.totalMass = (float)g[5];
+  m.cm.x = (float)g[0];
+  m.cm.y = (float)g[1];
+  m.cm.z = (float)g[2];
+  m.xx = (float)q[1]; m.xy = (float)q[3]; m.xz = (float)q[4]; m.yy = (float)q[2]; m.yz = (float)q[5];
+  m.xxx = (float)q[6]; m.xyy = (float)q[7]; m.xxy = (float)q[8]; m.yyy = (float)q[9];
+  m.xxz = (float)q[10]; m.yyz = (float)q[11]; m.xyz = (float)q[12];
+  m.xxxx = (float)q[13]; m.xyyy = (float)q[14]; m.xxxy = (float)q[15]; m.yyyy = (float)q[16];
+  m.xxxz = (float)q[17]; m.yyyz = (float)q[18]; m.xxyy = (float)q[19]; m.xxyz = (float)q[20];
+  m.xyyz = (float)q[21];
+  out[nd] = m;
+}
+
+__global__ __launch_bounds__(128) void bucket_moments_kernel(
+    int nNodes, const int* first, const int* count, const int* child, const double* boxRadius,
+    const double* pos, const double* mass, const double* soft, CudaMultipoleMoments* out,
+    double* node64, double* cm64) {
+  int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= nNodes || child[2 * nd] >= 0) return;
+  const int f = first[nd], cnt = count[nd];
+  double g[12];
+  double M = 0, sx = 0, sy = 0, sz = 0, ss = 0;
+  g[6] = g[7] = g[8] = HUGE_VAL;
+  g[9] = g[10] = g[11] = -HUGE_VAL;
+  for (int i = f; i < f + cnt; ++i) {
+    double m = mass[i], x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    M += m;
+    sx += m * x;
+    sy += m * y;
+    sz += m * z;
+    ss += m * soft[i];
+    g[6] = fmin(g[6], x); g[7] = fmin(g[7], y); g[8] = fmin(g[8], z);
+    g[9] = fmax(g[9], x); g[10] = fmax(g[10], y); g[11] = fmax(g[11], z);
+  }
+  g[0] = sx / M;
+  g[1] = sy / M;
+  g[2] = sz / M;
+  g[4] = ss / M;
+  g[5] = M;
+  double T[35];
+#pragma unroll
+  for (int k = 0; k < 35; ++k) T[k] = 0.0;
+  double r2max = 0.0;
+  for (int i = f; i < f + cnt; ++i) {
+    double m = mass[i];
+    double dx = pos[3 * i] - g[0], dy = pos[3 * i + 1] - g[1], dz = pos[3 * i + 2] - g[2];
+    r2max = fmax(r2max, dx * dx + dy * dy + dz * dz);
+    double px[5], py[5], pz[5];
+    px[0] = py[0] = pz[0] = m;
+    py[0] = pz[0] = 1.0;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) {
+      px[k] = px[k - 1] * dx;
+      py[k] = py[k - 1] * dy;
+      pz[k] = pz[k - 1] * dz;
+    }
+    T[0] += m;
+#pragma unroll
+    for (int n = 2; n <= 4; ++n)
+#pragma unroll
+      for (int a = 0; a <= n; ++a)
+#pragma unroll
+        for (int b = 0; b <= n - a; ++b) {
+          const int c = n - a - b;
+          T[cidx(a, b, c)] += px[a] * py[b] * pz[c];
+        }
+  }
+  stf_project(T);
+  const double u = (cnt > 1 && r2max > 0.0) ? sqrt(r2max) : boxRadius[nd];
+  g[3] = u;
+  emit(nd, g, T, u, out, node64, cm64);
+}
+
+__global__ __launch_bounds__(128) void internal_moments_kernel(int lo, int hi, const int* child,
+                                                               CudaMultipoleMoments* out,
+                                                               double* node64, double* cm64) {
+  int nd = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= hi) return;
+  const int c0 = child[2 * nd], c1 = child[2 * nd + 1];
+  if (c0 < 0) return;
+  const double* r0 = node64 + (size_t)c0 * 34;
+  const double* r1 = node64 + (size_t)c1 * 34;
+  double g[12];
+  const double M0 = r0[5], M1 = r1[5], M = M0 + M1;
+  g[5] = M;
+  g[4] = (M0 * r0[4] + M1 * r1[4]) / M;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) g[k] = (M0 * r0[k] + M1 * r1[k]) / M;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    g[6 + k] = fmin(r0[6 + k], r1[6 + k]);
+    g[9 + k] = fmax(r0[9 + k], r1[9 + k]);
+  }
+  double d2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double e1 = g[k] - g[6 + k], e2 = g[9 + k] - g[k];
+    double e = e1 > e2 ? e1 : e2;
+    d2 += e * e;
+  }
+  const double u = sqrt(d2);
+  g[3] = u;
+  double R[35];
+#pragma unroll
+  for (int k = 0; k < 35; ++k) R[k] = 0.0;
+#pragma unroll
+  for (int ch = 0; ch < 2; ++ch) {
+    const double* r = ch ? r1 : r0;
+    double q[22], C[35];
+    const double uc = r[3], s2 = uc * uc, s3 = s2 * uc, s4 = s3 * uc;
+#pragma unroll
+    for (int k = 0; k < 22; ++k) q[k] = r[12 + k] * (k == 0 ? 1.0 : k < 6 ? s2 : k < 13 ? s3 : s4);
+    complete_from_reduced(q, C);
+    double d[3] = {r[0] - g[0], r[1] - g[1], r[2] - g[2]};
+    shift_add_raw(C, d, R);
+  }
+  stf_project(R);
+  emit(nd, g, R, u, out, node64, cm64);
+}
+
+}  // namespace cgpu
+
+using namespace cgpu;
+
+extern "C" int changaBuildMoments(int nNodes, int nDepths, const int* levelOffsets,
+                                  const int* first, const int* count, const int* child,
+                                  const double* boxRadius, const double* pos, const double* mass,
+                                  const double* soft, CudaMultipoleMoments* moments,
+                                  double* node64, double* cm64, void* stream) {
+  if (nNodes <= 0) return 0;
+  if (!levelOffsets || !first || !count || !child || !boxRadius || !pos || !mass || !soft ||
+      !moments || !node64)
+    return fail("changaBuildMoments: missing argument", hipErrorInvalidValue, __FILE__, __LINE__);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(bucket_moments_kernel, dim3((nNodes + 127) / 128), dim3(128), 0, s, nNodes,
+                     first, count, child, boxRadius, pos, mass, soft, moments, node64, cm64);
+  CGPU_CHECK(hipGetLastError());
+  for (int d = nDepths - 1; d >= 0; --d) {
+    int lo = levelOffsets[d], hi = levelOffsets[d + 1];
+    if (hi <= lo) continue;
+    hipLaunchKernelGGL(internal_moments_kernel, dim3((hi - lo + 127) / 128), dim3(128), 0, s, lo,
+                       hi, child, moments, node64, cm64);
+    CGPU_CHECK(hipGetLastError());
+  }
+  return 0;
+}

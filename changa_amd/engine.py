@@ -1,0 +1,259 @@
+"""One gravity force evaluation on cuda:0 through the C-ABI.
+
+The sequence mirrors ChaNGa's GPU path (TreePiece::startGravity ->
+DataManager::transferLocalToGPU -> TreePiece{Cell,Part}ListDataTransferLocal
+-> EwaldHost -> TransferParticleVarsBack; SURVEY call stack 3.1/3.3): the host
+builds the tree, node geometry and interaction lists (libchanga_host.so); the
+GPU builds the multipole moments, then evaluates PC, PP and Ewald into the
+VariablePartData array (libchanga_gpu.so). PyTorch only provides device memory
+and the stream.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import gpu, host
+
+
+def _dp(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def compact_parts(pos, mass, soft):
+    """CompactPartData[N] {mass, soft, x, y, z} (cuda_typedef.h:240-248)."""
+    a = np.empty((pos.shape[0], 5), dtype=np.float32)
+    a[:, 0] = mass
+    a[:, 1] = soft
+    a[:, 2:] = pos
+    return a
+
+
+def expand_parts(lists):
+    """ILPart runs -> one ILCell per source particle, as ChaNGa's
+    GenericList<ILPart>::serialize sends them (Compute.cpp:1118-1218)."""
+    p = lists["parts"].reshape(-1, 3)
+    num = p[:, 2].astype(np.int64)
+    total = int(num.sum())
+    run = np.repeat(np.arange(len(p)), num)
+    start = np.concatenate([[0], np.cumsum(num)[:-1]])
+    k = np.arange(total) - start[run]
+    cells = np.empty((total, 2), dtype=np.int32)
+    cells[:, 0] = p[run, 0] + k
+    cells[:, 1] = p[run, 1]
+    cs = np.concatenate([[0], np.cumsum(num)])
+    marks = cs[lists["part_marks"]]
+    return cells.reshape(-1), marks
+
+
+def ewald_real_counts(pos, cm, L, n_reps, ew_cut, chunk=1 << 18):
+    """Evaluated real-space Ewald terms per particle (BucketEwald's test,
+    Ewald.cpp:126-138): |d + nL|^2 <= (fEwCut L)^2 or n in the hole."""
+    R = max(int(math.ceil(ew_cut)), n_reps)
+    g = np.arange(-R, R + 1)
+    nv = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    hole = np.all(np.abs(nv) <= n_reps, axis=1)
+    cut2 = (ew_cut * L) ** 2
+    out = np.zeros(pos.shape[0], dtype=np.int64)
+    for i in range(0, pos.shape[0], chunk):
+        d = pos[i:i + chunk] - cm
+        r2 = ((d[:, None, :] + nv[None, :, :] * L) ** 2).sum(axis=2)
+        out[i:i + chunk] = ((r2 <= cut2) | hole[None, :]).sum(axis=1)
+    return out
+
+
+class Evaluation:
+    """Tree, lists and device state for one workload (see configs.py).
+
+    precise: FP64 bucket-centre subtraction form; pp_compact: ILPart lists
+    expanded on the GPU (else ChaNGa's per-particle ILCell lists)."""
+
+    def __init__(self, w, precise=False, pp_compact=True, n_threads=0, device="cuda:0"):
+        self.w = w
+        self.precise = precise
+        self.pp_compact = pp_compact
+        self.device = torch.device(device)
+        periodic = bool(w["periodic"])
+        self.L = float(w["L"]) if periodic else 0.0
+        if periodic:
+            tr = host.build_tree(w["pos"], w["bucket"], box_lo=np.zeros(3), box_size=self.L)
+        else:
+            tr = host.build_tree(w["pos"], w["bucket"])
+        self.tree = tr
+        perm = tr["perm"]
+        self.pos = np.ascontiguousarray(w["pos"][perm])
+        self.mass = np.ascontiguousarray(w["mass"][perm])
+        self.soft = np.ascontiguousarray(w["soft"][perm])
+        self.active = None if w.get("active") is None else np.ascontiguousarray(w["active"][perm])
+        self.geo = host.node_geometry(tr, self.pos, self.mass, self.soft)
+        self.lists = host.walk(tr, self.geo, w["theta"], w["n_replicas"],
+                               self.L if periodic else 1.0, n_threads)
+        self.counts = host.interaction_counts(tr, self.lists, self.active)
+        self._upload()
+
+    # ------------------------------------------------------------------ device
+    def _t(self, a, dtype=None):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+
+    def _upload(self):
+        tr, L = self.tree, self.lists
+        n = self.pos.shape[0]
+        self.d_parts = self._t(compact_parts(self.pos, self.mass, self.soft))
+        self.d_vars = torch.zeros((n, 5), dtype=torch.float32, device=self.device)
+        self.d_pos64 = self._t(self.pos)
+        self.d_mass64 = self._t(self.mass)
+        self.d_soft64 = self._t(self.soft)
+        self.d_first = self._t(tr["node_first"])
+        self.d_count = self._t(tr["node_count"])
+        self.d_child = self._t(tr["node_child"])
+        self.d_boxr = self._t(tr["node_box_radius"])
+        nn = tr["n_nodes"]
+        self.d_moments = torch.zeros((nn, 27), dtype=torch.float32, device=self.device)
+        self.d_node64 = torch.zeros((nn, 34), dtype=torch.float64, device=self.device)
+        self.d_cm64 = torch.zeros((nn, 3), dtype=torch.float64, device=self.device)
+        if L["cell_marks"][-1] >= 2 ** 31:
+            raise ValueError("cell list exceeds the int32 marker range of one request")
+        self.d_cells = self._t(L["cells"])
+        self.d_cmarks = self._t(L["cell_marks"].astype(np.int32))
+        self.d_starts = self._t(tr["bucket_start"])
+        self.d_sizes = self._t(tr["bucket_size"])
+        if self.pp_compact:
+            self.d_plist = self._t(L["parts"])
+            self.d_pmarks = self._t(L["part_marks"].astype(np.int32))
+        else:
+            cells, marks = expand_parts(L)
+            if marks[-1] >= 2 ** 31:
+                raise ValueError("expanded particle list exceeds the int32 marker range")
+            self.d_plist = self._t(cells)
+            self.d_pmarks = self._t(marks.astype(np.int32))
+        self.stream = torch.cuda.current_stream(self.device)
+        self.build_moments()
+        torch.cuda.synchronize(self.device)
+        if self.w["periodic"]:
+            self._setup_ewald()
+
+    def _sp(self):
+        return ctypes.c_void_p(self.stream.cuda_stream)
+
+    def build_moments(self):
+        tr = self.tree
+        lo = np.ascontiguousarray(tr["level_offsets"], dtype=np.int32)
+        rc = gpu.lib().changaBuildMoments(
+            tr["n_nodes"], tr["n_depths"], lo.ctypes.data_as(ctypes.c_void_p), _dp(self.d_first),
+            _dp(self.d_count), _dp(self.d_child), _dp(self.d_boxr), _dp(self.d_pos64),
+            _dp(self.d_mass64), _dp(self.d_soft64), _dp(self.d_moments), _dp(self.d_node64),
+            _dp(self.d_cm64), self._sp())
+        gpu.check(rc, "changaBuildMoments")
+
+    def _setup_ewald(self):
+        w = self.w
+        root = self.d_node64[0].cpu().numpy()
+        cm, u, M, q = root[0:3], root[3], root[5], root[12:34]
+        self.root_cm64 = np.ascontiguousarray(cm)
+        self.ewt64 = host.ewald_table(q, u, self.L, w["ewh_cut"])
+        mc = host.root_momc(q, u)
+        ro = gpu.EwaldReadOnlyData()
+        ro.mm.totalMass = M
+        ro.mm.cmx, ro.mm.cmy, ro.mm.cmz = cm
+        for f, v in zip(gpu.MOMC_FIELDS, mc):
+            setattr(ro.momcRoot, f, float(v))
+        L = self.L
+        alpha = 2.0 / L
+        if self.active is None:
+            self.ew_markers = None
+            ro.n = self.pos.shape[0]
+        else:
+            idx = np.nonzero(self.active)[0].astype(np.int32)
+            self.ew_markers = self._t(idx)
+            ro.n = len(idx)
+        ro.nReps = w["n_replicas"]
+        ro.nEwReps = int(math.ceil(w["ew_cut"]))
+        ro.nEwhLoop = len(self.ewt64)
+        ro.L = L
+        ro.fEwCut = w["ew_cut"]
+        ro.alpha = alpha
+        ro.alpha2 = alpha * alpha
+        ro.k1 = math.pi / (alpha * alpha * L ** 3)
+        ro.ka = 2.0 * alpha / math.sqrt(math.pi)
+        ro.fEwCut2 = w["ew_cut"] ** 2 * L * L
+        ro.fInner2 = 1.1e-2 * L * L
+        self.ewald_ro = ro
+        self.d_ewt = self._t(self.ewt64.astype(np.float32))
+        self.ewald_count_real = None
+
+    def ewald_counts(self):
+        if not self.w["periodic"]:
+            return 0, 0
+        if self.ewald_count_real is None:
+            sel = slice(None) if self.active is None else self.active
+            c = ewald_real_counts(self.pos[sel], self.root_cm64, self.L, self.w["n_replicas"],
+                                  self.w["ew_cut"])
+            self.ewald_count_real = int(c.sum())
+        nact = self.pos.shape[0] if self.active is None else int(self.active.sum())
+        return self.ewald_count_real, nact * len(self.ewt64)
+
+    # ------------------------------------------------------------------ launches
+    def _launch(self, pc):
+        a = gpu.ChangaGravityLaunch()
+        a.targets = self.d_parts.data_ptr()
+        a.vars = self.d_vars.data_ptr()
+        a.moments = self.d_moments.data_ptr()
+        a.sources = self.d_parts.data_ptr()
+        a.list = (self.d_cells if pc else self.d_plist).data_ptr()
+        a.marks = (self.d_cmarks if pc else self.d_pmarks).data_ptr()
+        a.starts = self.d_starts.data_ptr()
+        a.sizes = self.d_sizes.data_ptr()
+        a.numBuckets = self.tree["n_buckets"]
+        a.listKind = 0 if pc else (1 if self.pp_compact else 0)
+        a.fperiod = self.L
+        if self.precise:
+            a.tpos64 = self.d_pos64.data_ptr()
+            a.spos64 = self.d_pos64.data_ptr()
+            a.cm64 = self.d_cm64.data_ptr()
+        return a
+
+    def zero(self):
+        gpu.check(gpu.lib().changaZeroVars(_dp(self.d_vars), self.pos.shape[0], self._sp()),
+                  "changaZeroVars")
+
+    def run_pc(self):
+        a = self._launch(True)
+        gpu.check(gpu.lib().changaNodeGravity(ctypes.byref(a), self._sp()), "changaNodeGravity")
+
+    def run_pp(self):
+        a = self._launch(False)
+        gpu.check(gpu.lib().changaPartGravity(ctypes.byref(a), self._sp()), "changaPartGravity")
+
+    def run_ewald(self):
+        if not self.w["periodic"]:
+            return
+        pos64 = _dp(self.d_pos64) if self.precise else None
+        cm = self.root_cm64.ctypes.data_as(ctypes.c_void_p) if self.precise else None
+        rc = gpu.lib().changaEwald(_dp(self.d_parts), _dp(self.d_vars),
+                                   _dp(self.ew_markers) if self.ew_markers is not None else None,
+                                   0, self.ewald_ro.n, ctypes.byref(self.ewald_ro), _dp(self.d_ewt),
+                                   pos64, cm, self._sp())
+        gpu.check(rc, "changaEwald")
+
+    def step(self, moments=True):
+        """One force evaluation: multipole build, zero, PC, PP, Ewald."""
+        if moments:
+            self.build_moments()
+        self.zero()
+        self.run_pc()
+        self.run_pp()
+        self.run_ewald()
+
+    def results(self):
+        """(acc[N,3], pot[N], dtgrav[N]) in tree order, FP32 as the wire."""
+        v = self.d_vars.cpu().numpy()
+        return v[:, 0:3].astype(np.float64), v[:, 3].astype(np.float64), v[:, 4].astype(np.float64)
+
+    def node64(self):
+        return self.d_node64.cpu().numpy()
+
+    def n_interactions(self):
+        pc, soft, pp = self.counts
+        er, ek = self.ewald_counts()
+        return dict(pc=pc, soft=soft, pp=pp, ewald_real=er, ewald_k=ek)

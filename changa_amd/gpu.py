@@ -1,0 +1,138 @@
+"""ctypes view of libchanga_gpu.so (include/changa_gpu.h).
+
+Loading fails loudly when the library is not built: there is no CPU fallback
+for the GPU path.
+"""
+import ctypes
+import os
+
+from . import build as _build
+
+c_int = ctypes.c_int
+c_float = ctypes.c_float
+c_void_p = ctypes.c_void_p
+c_size_t = ctypes.c_size_t
+
+
+class CudaVector3D(ctypes.Structure):
+    _fields_ = [("x", c_float), ("y", c_float), ("z", c_float)]
+
+
+class MultipoleMomentsData(ctypes.Structure):
+    _fields_ = [("totalMass", c_float), ("cmx", c_float), ("cmy", c_float), ("cmz", c_float)]
+
+
+MOMC_FIELDS = ["m", "xx", "yy", "xy", "xz", "yz", "xxx", "xyy", "xxy", "yyy", "xxz", "yyz", "xyz",
+               "xxxx", "xyyy", "xxxy", "yyyy", "xxxz", "yyyz", "xxyy", "xxyz", "xyyz", "zz", "xzz",
+               "yzz", "zzz", "xxzz", "xyzz", "xzzz", "yyzz", "yzzz", "zzzz"]
+
+
+class MomcData(ctypes.Structure):
+    _fields_ = [(f, c_float) for f in MOMC_FIELDS]
+
+
+class EwaldReadOnlyData(ctypes.Structure):
+    _fields_ = [("mm", MultipoleMomentsData), ("momcRoot", MomcData), ("n", c_int),
+                ("nReps", c_int), ("nEwReps", c_int), ("nEwhLoop", c_int), ("L", c_float),
+                ("fEwCut", c_float), ("alpha", c_float), ("alpha2", c_float), ("k1", c_float),
+                ("ka", c_float), ("fEwCut2", c_float), ("fInner2", c_float)]
+
+
+class EwtData(ctypes.Structure):
+    _fields_ = [("hx", c_float), ("hy", c_float), ("hz", c_float), ("hCfac", c_float),
+                ("hSfac", c_float)]
+
+
+class EwaldData(ctypes.Structure):
+    _fields_ = [("EwaldRange", c_int * 2), ("EwaldMarkers", ctypes.POINTER(c_int)),
+                ("ewt", ctypes.POINTER(EwtData)),
+                ("cachedData", ctypes.POINTER(EwaldReadOnlyData))]
+
+
+class CudaRequest(ctypes.Structure):
+    _fields_ = [("stream", c_void_p), ("d_localMoments", c_void_p), ("d_remoteMoments", c_void_p),
+                ("d_localParts", c_void_p), ("d_remoteParts", c_void_p), ("d_localVars", c_void_p),
+                ("sMoments", c_size_t), ("sCompactParts", c_size_t), ("sVarParts", c_size_t),
+                ("list", c_void_p), ("bucketMarkers", c_void_p), ("bucketStarts", c_void_p),
+                ("bucketSizes", c_void_p), ("numInteractions", c_int),
+                ("numBucketsPlusOne", c_int), ("tp", c_void_p), ("missedNodes", c_void_p),
+                ("missedParts", c_void_p), ("sMissed", c_size_t), ("affectedBuckets", c_void_p),
+                ("cb", c_void_p), ("state", c_void_p), ("fperiod", c_float),
+                ("node", ctypes.c_bool), ("remote", ctypes.c_bool)]
+
+
+class ChangaGravityLaunch(ctypes.Structure):
+    _fields_ = [("targets", c_void_p), ("vars", c_void_p), ("moments", c_void_p),
+                ("sources", c_void_p), ("list", c_void_p), ("marks", c_void_p),
+                ("starts", c_void_p), ("sizes", c_void_p), ("numBuckets", c_int),
+                ("listKind", c_int), ("fperiod", c_float), ("reserved", c_int),
+                ("tpos64", c_void_p), ("spos64", c_void_p), ("cm64", c_void_p)]
+
+
+CALLBACK = ctypes.CFUNCTYPE(None, c_void_p)
+
+# every extern "C" symbol include/changa_gpu.h declares
+EXPORTS = [
+    "allocatePinnedHostMemory", "freePinnedHostMemory", "DataManagerTransferLocalTree",
+    "DataManagerTransferRemoteChunk", "TransferParticleVarsBack",
+    "TreePieceCellListDataTransferLocal", "TreePieceCellListDataTransferRemote",
+    "TreePieceCellListDataTransferRemoteResume", "TreePiecePartListDataTransferLocal",
+    "TreePiecePartListDataTransferLocalSmallPhase", "TreePiecePartListDataTransferRemote",
+    "TreePiecePartListDataTransferRemoteResume", "EwaldHostMemorySetup", "EwaldHostMemoryFree",
+    "EwaldHost", "changaGpuSetCallbackHandler", "changaGpuSetAbortOnError", "changaGpuLastError",
+    "changaGpuReleasePools", "TreePiecePartListDataTransferLocalCompact", "changaNodeGravity",
+    "changaPartGravity", "changaZeroVars", "changaEwald", "changaBuildMoments",
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_build.GPU_LIB):
+        raise RuntimeError("libchanga_gpu.so is not built: run `python -m changa_amd.build`")
+    L = ctypes.CDLL(_build.GPU_LIB)
+    P = c_void_p
+    L.allocatePinnedHostMemory.argtypes = [ctypes.POINTER(c_void_p), c_size_t]
+    L.freePinnedHostMemory.argtypes = [P]
+    L.DataManagerTransferLocalTree.argtypes = [P, c_size_t, P, c_size_t, P, c_size_t,
+                                               ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                               ctypes.POINTER(c_void_p), P, c_int, P]
+    L.DataManagerTransferRemoteChunk.argtypes = [P, c_size_t, P, c_size_t,
+                                                 ctypes.POINTER(c_void_p),
+                                                 ctypes.POINTER(c_void_p), P, P]
+    L.TransferParticleVarsBack.argtypes = [P, c_size_t, P, P, P]
+    for name in ["TreePieceCellListDataTransferLocal", "TreePieceCellListDataTransferRemote",
+                 "TreePieceCellListDataTransferRemoteResume", "TreePiecePartListDataTransferLocal",
+                 "TreePiecePartListDataTransferRemote", "TreePiecePartListDataTransferRemoteResume",
+                 "TreePiecePartListDataTransferLocalCompact"]:
+        getattr(L, name).argtypes = [ctypes.POINTER(CudaRequest)]
+    L.TreePiecePartListDataTransferLocalSmallPhase.argtypes = [ctypes.POINTER(CudaRequest), P, c_int]
+    L.EwaldHostMemorySetup.argtypes = [ctypes.POINTER(EwaldData), c_int, c_int, c_int]
+    L.EwaldHostMemoryFree.argtypes = [ctypes.POINTER(EwaldData), c_int]
+    L.EwaldHost.argtypes = [P, P, ctypes.POINTER(EwaldData), P, P, c_int, c_int]
+    L.changaGpuSetCallbackHandler.argtypes = [CALLBACK]
+    L.changaGpuSetAbortOnError.argtypes = [c_int]
+    L.changaGpuLastError.argtypes = [ctypes.POINTER(ctypes.c_char_p)]
+    L.changaGpuLastError.restype = c_int
+    L.changaNodeGravity.argtypes = [ctypes.POINTER(ChangaGravityLaunch), P]
+    L.changaNodeGravity.restype = c_int
+    L.changaPartGravity.argtypes = [ctypes.POINTER(ChangaGravityLaunch), P]
+    L.changaPartGravity.restype = c_int
+    L.changaZeroVars.argtypes = [P, c_int, P]
+    L.changaZeroVars.restype = c_int
+    L.changaEwald.argtypes = [P, P, P, c_int, c_int, ctypes.POINTER(EwaldReadOnlyData), P, P, P, P]
+    L.changaEwald.restype = c_int
+    L.changaBuildMoments.argtypes = [c_int, c_int, P, P, P, P, P, P, P, P, P, P, P, P]
+    L.changaBuildMoments.restype = c_int
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = ctypes.c_char_p()
+        lib().changaGpuLastError(ctypes.byref(msg))
+        raise RuntimeError(f"{what}: {msg.value.decode() if msg.value else rc}")

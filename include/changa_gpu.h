@@ -1,0 +1,170 @@
+/*
+ * changa_gpu.h -- C-ABI of the MI355X gravity force engine (libchanga_gpu.so).
+ *
+ * Part 1 mirrors ChaNGa's GPU work-request surface, HostCUDA.h:99-126 and
+ * EwaldCUDA.h:59-64, with the same names, argument meaning and wire layouts
+ * (changa_gpu_types.h). Differences a caller sees:
+ *   - every function has C linkage; streams are hipStream_t passed as void*;
+ *   - completion: instead of hapiAddCallback(stream, cb), each call enqueues
+ *     one host function on its stream after all of its device work; that
+ *     function calls the handler installed with changaGpuSetCallbackHandler
+ *     with the request's `cb` pointer (nothing happens when either is NULL);
+ *   - per-request device copies live in persistent per-stream pools: nothing
+ *     is freed while a kernel may still read it;
+ *   - cells flagged CHANGA_ILCELL_SOFTENED are evaluated on the GPU as a
+ *     spline-softened monopole;
+ *   - the Ewald h-loop table has no 80-entry limit;
+ *   - errors abort the process with a message (the reference's cudaChk)
+ *     unless changaGpuSetAbortOnError(0) was called; then the call returns,
+ *     changaGpuLastError() reports the failure and the callback is not run.
+ *
+ * Part 2 is the device-resident interface the mirror is built on: the same
+ * kernels on lists already in device memory, the GPU multipole build, and the
+ * precision and batched forms. It has no counterpart in the reference.
+ */
+#ifndef CHANGA_GPU_H
+#define CHANGA_GPU_H
+
+#include "changa_gpu_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================
+ * Part 1: HostCUDA.h / EwaldCUDA.h mirror
+ * ====================================================================== */
+
+/* HostCUDA.cu:66-83 (hapiMallocHost): pinned host memory. */
+void allocatePinnedHostMemory(void **ptr, size_t size);
+/* HostCUDA.cu:85-101 (hapiFreeHost). */
+void freePinnedHostMemory(void *ptr);
+
+/* HostCUDA.cu:115-149: allocate the device tree (moments, particles,
+   accelerations), copy it host->device on `stream` and zero the first
+   numParticles accelerations (ZeroVars, HostCUDA.cu:2195-2205). The three
+   device pointers are returned through the out-parameters; the caller
+   releases them with hipFree after its last request completed. */
+void DataManagerTransferLocalTree(void *moments, size_t sMoments, void *compactParts,
+                                  size_t sCompactParts, void *varParts, size_t sVarParts,
+                                  void **d_localMoments, void **d_compactParts,
+                                  void **d_varParts, void *stream, int numParticles,
+                                  void *callback);
+
+/* HostCUDA.cu:159-183: remote moments and particles to the device. */
+void DataManagerTransferRemoteChunk(void *moments, size_t sMoments, void *compactParts,
+                                    size_t sCompactParts, void **d_remoteMoments,
+                                    void **d_remoteParts, void *stream, void *callback);
+
+/* HostCUDA.cu:623-630: accelerations device->host. */
+void TransferParticleVarsBack(VariablePartData *hostBuffer, size_t size, void *d_varParts,
+                              void *stream, void *cb);
+
+/* HostCUDA.cu:190-248 / 252-292 / 296-341: particle-cell (hexadecapole)
+   forces. data->list is ILCell[numInteractions] (host memory);
+   bucketMarkers[numBucketsPlusOne], bucketStarts/bucketSizes[numBuckets]
+   index d_localParts; results are added into d_localVars (+= for a, phi;
+   max for dtGrav). Sources: d_localMoments, d_remoteMoments, or
+   data->missedNodes (CudaMultipoleMoments, data->sMissed bytes). */
+void TreePieceCellListDataTransferLocal(CudaRequest *data);
+void TreePieceCellListDataTransferRemote(CudaRequest *data);
+void TreePieceCellListDataTransferRemoteResume(CudaRequest *data);
+
+/* HostCUDA.cu:412-460 / 345-408 / 464-513 / 517-572: particle-particle
+   forces. As ChaNGa's GenericList<ILPart>::serialize (Compute.cpp:1118-1218)
+   produces it, data->list is ILCell[numInteractions], one entry per source
+   particle {particle index, offsetID}. Sources: d_localParts, the `parts`
+   array (small phase), d_remoteParts, or data->missedParts. */
+void TreePiecePartListDataTransferLocal(CudaRequest *data);
+void TreePiecePartListDataTransferLocalSmallPhase(CudaRequest *data, CompactPartData *parts,
+                                                  int len);
+void TreePiecePartListDataTransferRemote(CudaRequest *data);
+void TreePiecePartListDataTransferRemoteResume(CudaRequest *data);
+
+/* HostCUDA.cu:1887-1901: pinned Ewald buffers (markers only in large phase). */
+void EwaldHostMemorySetup(EwaldData *h_idata, int size, int nEwhLoop, int largephase);
+void EwaldHostMemoryFree(EwaldData *h_idata, int largephase);
+/* HostCUDA.cu:1917-1956: Ewald correction (real-space replicas + k-space
+   h-loop) for the active particles: markers[0..n) in large phase, the range
+   EwaldRange otherwise. Results are added into d_localVars. */
+void EwaldHost(CompactPartData *d_localParts, VariablePartData *d_localVars,
+               EwaldData *h_idata, void *stream, void *cb, int myIndex, int largephase);
+
+/* ======================================================================
+ * Part 1b: runtime controls (no reference counterpart)
+ * ====================================================================== */
+typedef void (*changa_gpu_callback_fn)(void *cb);
+void changaGpuSetCallbackHandler(changa_gpu_callback_fn fn);
+void changaGpuSetAbortOnError(int abort_on_error);
+/* 0 when no error happened since the last call; the message is kept in a
+   static buffer. */
+int changaGpuLastError(const char **message);
+/* Releases the per-stream pools (after the caller synchronized). */
+void changaGpuReleasePools(void);
+
+/* ILPart-list variant of TreePiecePartListDataTransferLocal: data->list is
+   ILPart[numInteractions] (one entry per source bucket, 12 B instead of 8 B
+   per source particle) and is expanded on the GPU. */
+void TreePiecePartListDataTransferLocalCompact(CudaRequest *data);
+
+/* ======================================================================
+ * Part 2: device-resident interface
+ * ====================================================================== */
+
+/* One particle-cell or particle-particle launch over lists in device memory.
+   Precision form: when pos64 is non-NULL the kernels take target and source
+   positions (and cell centres, cm64) in FP64, subtract the bucket centre
+   (the first target particle's position) in FP64 and evaluate the
+   interaction in FP32 on the bucket-centred coordinates. */
+typedef struct {
+  const CompactPartData *targets;        /* particles indexed by starts[] */
+  VariablePartData *vars;                /* results (+=, max) */
+  const CudaMultipoleMoments *moments;   /* PC sources */
+  const CompactPartData *sources;        /* PP sources */
+  const void *list;                      /* ILCell* (PC, PP) or ILPart* (PP compact) */
+  const int *marks;                      /* numBuckets + 1 */
+  const int *starts;                     /* numBuckets */
+  const int *sizes;                      /* numBuckets */
+  int numBuckets;
+  int listKind;                          /* 0 ILCell per source, 1 ILPart runs */
+  float fperiod;
+  int reserved;
+  const double *tpos64;                  /* 3 per target particle, or NULL */
+  const double *spos64;                  /* 3 per source particle, or NULL */
+  const double *cm64;                    /* 3 per cell, or NULL */
+} ChangaGravityLaunch;
+
+int changaNodeGravity(const ChangaGravityLaunch *a, void *stream);
+int changaPartGravity(const ChangaGravityLaunch *a, void *stream);
+int changaZeroVars(VariablePartData *vars, int n, void *stream);
+
+/* Ewald on device-resident data. markers: n active particle indices into
+   parts (device memory; NULL: particles first..first+n-1). ewt: the
+   ro->nEwhLoop table entries in device memory. ro: host struct (its
+   momcRoot, mm, L, fEwCut, nReps and nEwReps are used). Precision form:
+   pos64 (device, 3 per particle) and rootCm64 (host, 3 doubles) or NULL. */
+int changaEwald(const CompactPartData *parts, VariablePartData *vars, const int *markers,
+                int first, int n, const EwaldReadOnlyData *ro, const EwtData *ewt,
+                const double *pos64, const double *rootCm64, void *stream);
+
+/* GPU multipole build (the CPU path is GenericTreeNode::makeBucket,
+   GenericTreeNode.h:219-257, and LocalTreeBuilder::doneChildren,
+   Compute.cpp:2534-2580). Node topology (breadth-first numbering, children
+   after parents, depth d occupying nodes [levelOffsets[d],
+   levelOffsets[d+1])): first/count (particle range), child (2 per node, -1
+   for a bucket), boxRadius (initial expansion radius of a bucket: half the
+   diagonal of its spatial cell). Particles: pos (FP64 x,y,z), mass, soft, in
+   tree order. Outputs: moments (wire, FP32), node64 (34 doubles per node:
+   cm.xyz, radius, soft, mass, lo.xyz, hi.xyz, 22 radius-scaled reduced
+   moments) and cm64 (3 doubles per node, may be NULL). All pointers are
+   device pointers. */
+int changaBuildMoments(int nNodes, int nDepths, const int *levelOffsets /* host */,
+                       const int *first, const int *count, const int *child,
+                       const double *boxRadius, const double *pos, const double *mass,
+                       const double *soft, CudaMultipoleMoments *moments, double *node64,
+                       double *cm64, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,880 @@
+/*
+ * oracle.c -- FP64 CPU restatement of ChaNGa's gravity force-evaluation path.
+ *
+ * TEST INFRASTRUCTURE ONLY: linked by nothing in the product. Used by tests/,
+ * by __graft_entry__.smoke() as the checker and by bench.py's cpu_baseline.
+ *
+ * Formulation. Radius-scaled reduced moments (the FMOMR of moments.h:64-68)
+ * are the order-2..4 symmetric trace-free (STF) tensors of the mass
+ * distribution about its centre of mass. This file manipulates them through
+ * their "complete" components C(a,b,c) (a x's, b y's, c z's), obtained from
+ * the stored ones by tracelessness, and through full 3^n tensors where a
+ * projection onto the STF subspace is needed (P2M, M2M). The potential of an
+ * STF multipole is phi = -sum_n (2n-1)!!/n! (Q_n . r^n) / r^(2n+1); the Ewald
+ * real-space kernels use the g-recursion g_(n+1) = -(1/r) d g_n / dr. The
+ * reference's hand-scheduled arithmetic (moments.c, Ewald.cpp) is followed in
+ * meaning, not in operation order; the pinning tests compare against
+ * moments.c compiled from the reference.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* -------------------------------------------------------------------------
+ * Index bookkeeping
+ * ------------------------------------------------------------------------- */
+
+/* stored component k -> index counts (a,b,c); order of moments.h:64-68 */
+static const int RED[ORC_NMOM][3] = {
+    {0, 0, 0},
+    {2, 0, 0}, {0, 2, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1},
+    {3, 0, 0}, {1, 2, 0}, {2, 1, 0}, {0, 3, 0}, {2, 0, 1}, {0, 2, 1}, {1, 1, 1},
+    {4, 0, 0}, {1, 3, 0}, {3, 1, 0}, {0, 4, 0}, {3, 0, 1}, {0, 3, 1}, {2, 2, 0},
+    {2, 1, 1}, {1, 2, 1}};
+
+/* complete components are kept in C[a][b][c] (a+b+c <= 4) */
+typedef struct {
+    double v[5][5][5];
+} cmom;
+
+static int red_index(int a, int b, int c) {
+    for (int k = 0; k < ORC_NMOM; ++k)
+        if (RED[k][0] == a && RED[k][1] == b && RED[k][2] == c) return k;
+    return -1;
+}
+
+/* complete STF component from stored ones: tracelessness in z,
+   T(a,b,c) = -T(a+2,b,c-2) - T(a,b+2,c-2) */
+static double stf_comp(const double *q, int a, int b, int c) {
+    if (c >= 2) return -(stf_comp(q, a + 2, b, c - 2) + stf_comp(q, a, b + 2, c - 2));
+    int k = red_index(a, b, c);
+    return k < 0 ? 0.0 : q[k];
+}
+
+/* complete components, order n multiplied by scale^n */
+static void to_complete(const double *q, double scale, cmom *C) {
+    memset(C, 0, sizeof(*C));
+    for (int n = 0; n <= 4; ++n) {
+        if (n == 1) continue;
+        double s = pow(scale, n);
+        for (int a = n; a >= 0; --a)
+            for (int b = n - a; b >= 0; --b) {
+                int c = n - a - b;
+                C->v[a][b][c] = s * stf_comp(q, a, b, c);
+            }
+    }
+}
+
+static double fact(int n) {
+    double f = 1.0;
+    for (int i = 2; i <= n; ++i) f *= i;
+    return f;
+}
+static double multinom(int a, int b, int c) { return fact(a + b + c) / (fact(a) * fact(b) * fact(c)); }
+static double ipow(double x, int n) {
+    double r = 1.0;
+    for (int i = 0; i < n; ++i) r *= x;
+    return r;
+}
+
+/* P_n = Q_n . r^n and V_n = Q_n . r^(n-1) (a vector) */
+static void contract(const cmom *C, int n, const double r[3], double *P, double V[3]) {
+    double p = 0.0;
+    for (int a = n; a >= 0; --a)
+        for (int b = n - a; b >= 0; --b) {
+            int c = n - a - b;
+            p += multinom(a, b, c) * C->v[a][b][c] * ipow(r[0], a) * ipow(r[1], b) * ipow(r[2], c);
+        }
+    *P = p;
+    V[0] = V[1] = V[2] = 0.0;
+    if (n == 0) return;
+    int m = n - 1;
+    for (int a = m; a >= 0; --a)
+        for (int b = m - a; b >= 0; --b) {
+            int c = m - a - b;
+            double w = multinom(a, b, c) * ipow(r[0], a) * ipow(r[1], b) * ipow(r[2], c);
+            V[0] += w * C->v[a + 1][b][c];
+            V[1] += w * C->v[a][b + 1][c];
+            V[2] += w * C->v[a][b][c + 1];
+        }
+}
+
+/* -------------------------------------------------------------------------
+ * Full rank-n tensors (n <= 4) and the STF projection
+ * ------------------------------------------------------------------------- */
+static int p3(int n) { return n == 0 ? 1 : 3 * p3(n - 1); }
+
+static void digits(int idx, int n, int *d) {
+    for (int p = n - 1; p >= 0; --p) {
+        d[p] = idx % 3;
+        idx /= 3;
+    }
+}
+static int flat(const int *d, int n) {
+    int idx = 0;
+    for (int p = 0; p < n; ++p) idx = idx * 3 + d[p];
+    return idx;
+}
+static double kd(int i, int j) { return i == j ? 1.0 : 0.0; }
+
+/* symmetric trace-free part of a symmetric rank-n tensor, n in {2,3,4} */
+static void stf_project(const double *T, int n, double *S) {
+    int d[4], e[4];
+    if (n == 2) {
+        double tr = T[0] + T[4] + T[8];
+        for (int i = 0; i < 9; ++i) {
+            digits(i, 2, d);
+            S[i] = T[i] - kd(d[0], d[1]) * tr / 3.0;
+        }
+    } else if (n == 3) {
+        double V[3];
+        for (int i = 0; i < 3; ++i) {
+            V[i] = 0;
+            for (int k = 0; k < 3; ++k) {
+                e[0] = i; e[1] = k; e[2] = k;
+                V[i] += T[flat(e, 3)];
+            }
+        }
+        for (int i = 0; i < 27; ++i) {
+            digits(i, 3, d);
+            S[i] = T[i] - (kd(d[0], d[1]) * V[d[2]] + kd(d[0], d[2]) * V[d[1]] +
+                           kd(d[1], d[2]) * V[d[0]]) / 5.0;
+        }
+    } else {
+        double W[9], w = 0.0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0;
+                for (int k = 0; k < 3; ++k) {
+                    e[0] = i; e[1] = j; e[2] = k; e[3] = k;
+                    s += T[flat(e, 4)];
+                }
+                W[i * 3 + j] = s;
+            }
+        w = W[0] + W[4] + W[8];
+        for (int i = 0; i < 81; ++i) {
+            digits(i, 4, d);
+            int a = d[0], b = d[1], c = d[2], f = d[3];
+            double t2 = kd(a, b) * W[c * 3 + f] + kd(a, c) * W[b * 3 + f] + kd(a, f) * W[b * 3 + c] +
+                        kd(b, c) * W[a * 3 + f] + kd(b, f) * W[a * 3 + c] + kd(c, f) * W[a * 3 + b];
+            double t4 = kd(a, b) * kd(c, f) + kd(a, c) * kd(b, f) + kd(a, f) * kd(b, c);
+            S[i] = T[i] - t2 / 7.0 + w * t4 / 35.0;
+        }
+    }
+}
+
+/* read stored components of order n out of a full STF tensor */
+static void store_order(const double *S, int n, double *q) {
+    for (int k = 0; k < ORC_NMOM; ++k) {
+        int a = RED[k][0], b = RED[k][1], c = RED[k][2];
+        if (a + b + c != n) continue;
+        int d[4], p = 0;
+        for (int i = 0; i < a; ++i) d[p++] = 0;
+        for (int i = 0; i < b; ++i) d[p++] = 1;
+        for (int i = 0; i < c; ++i) d[p++] = 2;
+        q[k] = S[flat(d, n)];
+    }
+}
+
+/* full tensor of stored order n (scaled units) */
+static void full_from_stored(const double *q, int n, double *T) {
+    int d[4];
+    for (int i = 0; i < p3(n); ++i) {
+        digits(i, n, d);
+        int cnt[3] = {0, 0, 0};
+        for (int p = 0; p < n; ++p) cnt[d[p]]++;
+        T[i] = stf_comp(q, cnt[0], cnt[1], cnt[2]);
+    }
+}
+
+/* -------------------------------------------------------------------------
+ * Spline softening (gravity.h:147-182; Hernquist & Katz 1989, ApJS 70, 419,
+ * eqs. A1-A2). a is the potential kernel, b = -(1/r) da/dr.
+ * ------------------------------------------------------------------------- */
+void orc_spline(double r2, double twoh, double *a, double *b) {
+    double r = sqrt(r2);
+    if (r < twoh) {
+        double dih = 2.0 / twoh; /* 1/h */
+        double u = r * dih;
+        double u2 = u * u;
+        if (u < 1.0) {
+            *a = dih * (7.0 / 5.0 + u2 * (-2.0 / 3.0 + u2 * (3.0 / 10.0 - u / 10.0)));
+            *b = dih * dih * dih * (4.0 / 3.0 + u2 * (-6.0 / 5.0 + u / 2.0));
+        } else {
+            double ir = 1.0 / r;
+            *a = -ir / 15.0 +
+                 dih * (8.0 / 5.0 + u2 * (-4.0 / 3.0 + u * (1.0 + u * (-3.0 / 10.0 + u / 30.0))));
+            *b = -ir * ir * ir / 15.0 +
+                 dih * dih * dih * (8.0 / 3.0 + u * (-3.0 + u * (6.0 / 5.0 - u / 6.0)));
+        }
+    } else {
+        *a = 1.0 / r;
+        *b = (*a) * (*a) * (*a);
+    }
+}
+
+/* -------------------------------------------------------------------------
+ * Multipole algebra
+ * ------------------------------------------------------------------------- */
+double orc_mom_make(double q[ORC_NMOM], double m, double u, double x, double y, double z) {
+    double r[3] = {x / u, y / u, z / u};
+    double T[81], S[81];
+    int d[4];
+    memset(q, 0, sizeof(double) * ORC_NMOM);
+    q[0] = m;
+    for (int n = 2; n <= 4; ++n) {
+        for (int i = 0; i < p3(n); ++i) {
+            digits(i, n, d);
+            double t = m;
+            for (int p = 0; p < n; ++p) t *= r[d[p]];
+            T[i] = t;
+        }
+        stf_project(T, n, S);
+        store_order(S, n, q);
+    }
+    return r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+}
+
+void orc_mom_shift(double q[ORC_NMOM], double u, double dx, double dy, double dz) {
+    double d[3] = {dx / u, dy / u, dz / u};
+    double Q2[9], Q3[27], T[81], S[81];
+    double m = q[0];
+    double out[ORC_NMOM];
+    int id[4], sub[4];
+    full_from_stored(q, 2, Q2);
+    full_from_stored(q, 3, Q3);
+    memcpy(out, q, sizeof(out));
+    /* Q'_n = STF( sum over subsets P of the index positions, |P| = k, of
+       Q_k(indices in P) * prod_{p not in P} d ), k in {0, 2, ..., n}:
+       the binomial shift of raw moments; STF parts of lower orders suffice
+       because traces vanish under the outer projection. Q_1 = 0 about the
+       old centre of mass. */
+    for (int n = 2; n <= 4; ++n) {
+        for (int i = 0; i < p3(n); ++i) {
+            digits(i, n, id);
+            double acc = 0.0;
+            for (int mask = 0; mask < (1 << n); ++mask) {
+                int k = __builtin_popcount(mask);
+                if (k == 1 || k == n) continue; /* dipole zero; k==n is Q_n itself */
+                double t;
+                if (k == 0) {
+                    t = m;
+                } else {
+                    int np = 0;
+                    for (int p = 0; p < n; ++p)
+                        if (mask & (1 << p)) sub[np++] = id[p];
+                    t = (k == 2) ? Q2[flat(sub, 2)] : Q3[flat(sub, 3)];
+                }
+                for (int p = 0; p < n; ++p)
+                    if (!(mask & (1 << p))) t *= d[id[p]];
+                acc += t;
+            }
+            T[i] = acc;
+        }
+        stf_project(T, n, S);
+        double add[ORC_NMOM];
+        memset(add, 0, sizeof(add));
+        store_order(S, n, add);
+        for (int k = 0; k < ORC_NMOM; ++k) {
+            int o = RED[k][0] + RED[k][1] + RED[k][2];
+            if (o == n) out[k] = q[k] + add[k];
+        }
+    }
+    memcpy(q, out, sizeof(out));
+}
+
+void orc_mom_rescale(double q[ORC_NMOM], double unew, double uold) {
+    double f = uold / unew;
+    for (int k = 1; k < ORC_NMOM; ++k) {
+        int o = RED[k][0] + RED[k][1] + RED[k][2];
+        q[k] *= ipow(f, o);
+    }
+}
+
+void orc_mom_scaled_add(double qr[ORC_NMOM], double ur, const double qa[ORC_NMOM], double ua) {
+    double f = ua / ur;
+    qr[0] += qa[0];
+    for (int k = 1; k < ORC_NMOM; ++k) {
+        int o = RED[k][0] + RED[k][1] + RED[k][2];
+        qr[k] += ipow(f, o) * qa[k];
+    }
+}
+
+/* MOMC order (moments.h:42-50): the 22 stored ones, then
+   zz, xzz, yzz, zzz, xxzz, xyzz, xzzz, yyzz, yzzz, zzzz */
+void orc_mom_to_momc(const double q[ORC_NMOM], double mc[32]) {
+    static const int EXTRA[10][3] = {{0, 0, 2}, {1, 0, 2}, {0, 1, 2}, {0, 0, 3}, {2, 0, 2},
+                                     {1, 1, 2}, {1, 0, 3}, {0, 2, 2}, {0, 1, 3}, {0, 0, 4}};
+    for (int k = 0; k < ORC_NMOM; ++k) mc[k] = q[k];
+    for (int k = 0; k < 10; ++k) mc[22 + k] = stf_comp(q, EXTRA[k][0], EXTRA[k][1], EXTRA[k][2]);
+}
+
+static void eval_complete(const cmom *C, const double r[3], double *pot, double a[3]) {
+    /* (2n-1)!!/n! */
+    static const double cn[5] = {1.0, 0.0, 1.5, 2.5, 4.375};
+    double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    double ir2 = 1.0 / r2;
+    double ir = sqrt(ir2);
+    double P, V[3];
+    double irn = ir; /* 1/r^(2n+1) */
+    for (int n = 0; n <= 4; ++n) {
+        if (n != 1) {
+            contract(C, n, r, &P, V);
+            *pot -= cn[n] * P * irn;
+            for (int i = 0; i < 3; ++i)
+                a[i] += cn[n] * (n * V[i] * irn - (2 * n + 1) * P * r[i] * irn * ir2);
+        }
+        irn *= ir2;
+    }
+}
+
+void orc_mom_eval(const double q[ORC_NMOM], double u, double x, double y, double z,
+                  double *pot, double a[3]) {
+    cmom C;
+    double r[3] = {x, y, z};
+    to_complete(q, u, &C);
+    eval_complete(&C, r, pot, a);
+}
+
+/* -------------------------------------------------------------------------
+ * Tree moments build
+ * ------------------------------------------------------------------------- */
+static void box_reset(double *lo, double *hi) {
+    for (int i = 0; i < 3; ++i) {
+        lo[i] = HUGE_VAL;
+        hi[i] = -HUGE_VAL;
+    }
+}
+
+void orc_build_moments(int n_nodes, const int32_t *node_first, const int32_t *node_count,
+                       const int32_t *node_child, const double *node_box_radius,
+                       const double *pos, const double *mass, const double *soft,
+                       double *geo, double *mom) {
+    /* children have larger indices than parents: a reverse sweep is a
+       post-order traversal */
+    for (int n = n_nodes - 1; n >= 0; --n) {
+        double *g = geo + (size_t)n * 12;
+        double *q = mom + (size_t)n * ORC_NMOM;
+        int c0 = node_child[2 * n], c1 = node_child[2 * n + 1];
+        memset(q, 0, sizeof(double) * ORC_NMOM);
+        if (c0 < 0) {
+            /* bucket: makeBucket (GenericTreeNode.h:219-257) */
+            int f = node_first[n], cnt = node_count[n];
+            double M = 0, sx = 0, sy = 0, sz = 0, ss = 0;
+            box_reset(g + 6, g + 9);
+            for (int i = f; i < f + cnt; ++i) {
+                M += mass[i];
+                sx += mass[i] * pos[3 * i];
+                sy += mass[i] * pos[3 * i + 1];
+                sz += mass[i] * pos[3 * i + 2];
+                ss += mass[i] * soft[i];
+                for (int k = 0; k < 3; ++k) {
+                    if (pos[3 * i + k] < g[6 + k]) g[6 + k] = pos[3 * i + k];
+                    if (pos[3 * i + k] > g[9 + k]) g[9 + k] = pos[3 * i + k];
+                }
+            }
+            g[0] = sx / M; g[1] = sy / M; g[2] = sz / M;
+            g[4] = ss / M;
+            g[5] = M;
+            /* calculateRadiusFarthestParticle, falling back to the initial
+               box radius when every particle sits on the centre of mass */
+            double r2max = 0.0;
+            for (int i = f; i < f + cnt; ++i) {
+                double dx = g[0] - pos[3 * i], dy = g[1] - pos[3 * i + 1], dz = g[2] - pos[3 * i + 2];
+                double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 > r2max) r2max = d2;
+            }
+            double u = (cnt > 1 && r2max > 0.0) ? sqrt(r2max) : node_box_radius[n];
+            g[3] = u;
+            double qp[ORC_NMOM];
+            for (int i = f; i < f + cnt; ++i) {
+                orc_mom_make(qp, mass[i], u, pos[3 * i] - g[0], pos[3 * i + 1] - g[1],
+                             pos[3 * i + 2] - g[2]);
+                for (int k = 0; k < ORC_NMOM; ++k) q[k] += qp[k];
+            }
+        } else {
+            /* doneChildren (Compute.cpp:2534-2580): operator+= of both
+               children, then calculateRadiusFarthestCorner */
+            const double *g0 = geo + (size_t)c0 * 12, *g1 = geo + (size_t)c1 * 12;
+            const double *q0 = mom + (size_t)c0 * ORC_NMOM, *q1 = mom + (size_t)c1 * ORC_NMOM;
+            double M0 = g0[5], M1 = g1[5], M = M0 + M1;
+            g[5] = M;
+            g[4] = (M0 * g0[4] + M1 * g1[4]) / M;
+            for (int k = 0; k < 3; ++k) g[k] = (M0 * g0[k] + M1 * g1[k]) / M;
+            for (int k = 0; k < 3; ++k) {
+                g[6 + k] = g0[6 + k] < g1[6 + k] ? g0[6 + k] : g1[6 + k];
+                g[9 + k] = g0[9 + k] > g1[9 + k] ? g0[9 + k] : g1[9 + k];
+            }
+            double a[ORC_NMOM], b[ORC_NMOM];
+            memcpy(a, q0, sizeof(a));
+            memcpy(b, q1, sizeof(b));
+            orc_mom_shift(a, g0[3], g0[0] - g[0], g0[1] - g[1], g0[2] - g[2]);
+            orc_mom_shift(b, g1[3], g1[0] - g[0], g1[1] - g[1], g1[2] - g[2]);
+            orc_mom_scaled_add(a, g0[3], b, g1[3]);
+            double d2 = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                double e1 = g[k] - g[6 + k], e2 = g[9 + k] - g[k];
+                double e = e1 > e2 ? e1 : e2;
+                d2 += e * e;
+            }
+            double u = sqrt(d2);
+            orc_mom_rescale(a, u, g0[3]);
+            g[3] = u;
+            memcpy(q, a, sizeof(a));
+        }
+    }
+}
+
+/* -------------------------------------------------------------------------
+ * Forces on interaction lists
+ * ------------------------------------------------------------------------- */
+/* Space::intersect(box, sphere): squared distance from the centre to the
+   box (zero inside) against r^2 (CUDAMoments.cu:137-159 states it). */
+static int box_sphere(const double *lo, const double *hi, const double c[3], double rad) {
+    double dsq = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        double e = 0.0;
+        if (c[k] < lo[k]) e = lo[k] - c[k];
+        else if (c[k] > hi[k]) e = c[k] - hi[k];
+        dsq += e * e;
+    }
+    return dsq <= rad * rad;
+}
+
+int orc_open_softening(const double *cg, const double *bg, double ox, double oy, double oz) {
+    double c[3] = {cg[0] + ox, cg[1] + oy, cg[2] + oz};
+    double rs = 2.0 * cg[4], rb = 2.0 * bg[4];
+    double dx = c[0] - bg[0], dy = c[1] - bg[1], dz = c[2] - bg[2];
+    if (dx * dx + dy * dy + dz * dz <= (rs + rb) * (rs + rb)) return 1;
+    return box_sphere(bg + 6, bg + 9, c, rs);
+}
+
+static void decode_offset(int32_t oid, double period, double off[3]) {
+    uint32_t code = ((uint32_t)oid >> 22) & 0x1ff;
+    off[0] = ((int)(code & 7) - 3) * period;
+    off[1] = ((int)((code >> 3) & 7) - 3) * period;
+    off[2] = ((int)((code >> 6) & 7) - 3) * period;
+}
+
+/* partBucketForce for one source (position already offset) */
+static int64_t pp_source(const double sp[3], double sm, double ss, int first, int count,
+                         const double *pos, const double *mass, const double *soft,
+                         const int32_t *rung, int active_rung, double *acc, double *pot,
+                         double *dtgrav) {
+    int64_t n = 0;
+    for (int j = first; j < first + count; ++j) {
+        if (rung && rung[j] < active_rung) continue;
+        ++n;
+        double r[3] = {sp[0] - pos[3 * j], sp[1] - pos[3 * j + 1], sp[2] - pos[3 * j + 2]};
+        double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        if (r2 == 0.0) continue;
+        double a, b;
+        orc_spline(r2, ss + soft[j], &a, &b);
+        for (int k = 0; k < 3; ++k) acc[3 * j + k] += r[k] * b * sm;
+        pot[j] -= sm * a;
+        double idt2 = (mass[j] + sm) * b;
+        if (idt2 > dtgrav[j]) dtgrav[j] = idt2;
+    }
+    return n;
+}
+
+void orc_forces_lists(int n_buckets, const int32_t *bucket_node,
+                      const int32_t *node_first, const int32_t *node_count,
+                      const double *geo, const double *mom,
+                      const double *pos, const double *mass, const double *soft,
+                      const int32_t *rung, int active_rung,
+                      const int32_t *cells, const int64_t *cell_marks,
+                      const int32_t *parts, const int64_t *part_marks,
+                      double period, int n_threads,
+                      double *acc, double *pot, double *dtgrav, int64_t counts[3]) {
+    int64_t n_pc = 0, n_soft = 0, n_pp = 0;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : n_pc, n_soft, n_pp)
+#endif
+    for (int b = 0; b < n_buckets; ++b) {
+        int bn = bucket_node[b];
+        int first = node_first[bn], count = node_count[bn];
+        const double *bg = geo + (size_t)bn * 12;
+        for (int64_t e = cell_marks[b]; e < cell_marks[b + 1]; ++e) {
+            int cn = cells[2 * e];
+            double off[3];
+            decode_offset(cells[2 * e + 1], period, off);
+            const double *cg = geo + (size_t)cn * 12;
+            if (orc_open_softening(cg, bg, off[0], off[1], off[2])) {
+                /* monopole pseudo-particle (gravity.h:414-421) */
+                double sp[3] = {cg[0] + off[0], cg[1] + off[1], cg[2] + off[2]};
+                n_soft += pp_source(sp, cg[5], cg[4], first, count, pos, mass, soft, rung,
+                                    active_rung, acc, pot, dtgrav);
+                continue;
+            }
+            cmom C;
+            to_complete(mom + (size_t)cn * ORC_NMOM, cg[3], &C);
+            for (int j = first; j < first + count; ++j) {
+                if (rung && rung[j] < active_rung) continue;
+                ++n_pc;
+                double r[3] = {pos[3 * j] - (cg[0] + off[0]), pos[3 * j + 1] - (cg[1] + off[1]),
+                               pos[3 * j + 2] - (cg[2] + off[2])};
+                double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                double dir = 1.0 / sqrt(r2);
+                eval_complete(&C, r, &pot[j], &acc[3 * j]);
+                double idt2 = (mass[j] + cg[5]) * dir * dir * dir;
+                if (idt2 > dtgrav[j]) dtgrav[j] = idt2;
+            }
+        }
+        for (int64_t e = part_marks[b]; e < part_marks[b + 1]; ++e) {
+            int idx = parts[3 * e], num = parts[3 * e + 2];
+            double off[3];
+            decode_offset(parts[3 * e + 1], period, off);
+            for (int s = idx; s < idx + num; ++s) {
+                double sp[3] = {pos[3 * s] + off[0], pos[3 * s + 1] + off[1], pos[3 * s + 2] + off[2]};
+                n_pp += pp_source(sp, mass[s], soft[s], first, count, pos, mass, soft, rung,
+                                  active_rung, acc, pot, dtgrav);
+            }
+        }
+    }
+    counts[0] = n_pc;
+    counts[1] = n_soft;
+    counts[2] = n_pp;
+}
+
+/* -------------------------------------------------------------------------
+ * Ewald
+ * ------------------------------------------------------------------------- */
+int orc_ewald_table(const double q_root[ORC_NMOM], double u_root, double L, double hcut,
+                    double *ewt, int max_n) {
+    cmom C;
+    to_complete(q_root, u_root, &C);
+    double alpha = 2.0 / L;
+    int hreps = (int)ceil(hcut);
+    int n = 0;
+    for (int hx = -hreps; hx <= hreps; ++hx)
+        for (int hy = -hreps; hy <= hreps; ++hy)
+            for (int hz = -hreps; hz <= hreps; ++hz) {
+                int h2 = hx * hx + hy * hy + hz * hz;
+                if (h2 == 0 || h2 > hcut * hcut) continue;
+                if (n >= max_n) return -1;
+                double k[3] = {2 * M_PI / L * hx, 2 * M_PI / L * hy, 2 * M_PI / L * hz};
+                double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+                /* (4 pi / V) exp(-k^2 / 4 alpha^2) / k^2 */
+                double w = 4 * M_PI / (L * L * L) * exp(-k2 / (4 * alpha * alpha)) / k2;
+                /* rho(k) = sum_n (-i)^n / n! Q_n . k^n */
+                double P0, P2, P3, P4, V[3];
+                contract(&C, 0, k, &P0, V);
+                contract(&C, 2, k, &P2, V);
+                contract(&C, 3, k, &P3, V);
+                contract(&C, 4, k, &P4, V);
+                double re = P0 - P2 / 2.0 + P4 / 24.0;
+                double im = P3 / 6.0;
+                ewt[5 * n + 0] = k[0];
+                ewt[5 * n + 1] = k[1];
+                ewt[5 * n + 2] = k[2];
+                ewt[5 * n + 3] = -w * re;
+                ewt[5 * n + 4] = w * im;
+                ++n;
+            }
+    return n;
+}
+
+void orc_bucket_ewald(int n_targets, const int32_t *targets, const double *pos,
+                      const double root_cm[3], double root_mass,
+                      const double q_root[ORC_NMOM], double u_root,
+                      const double *ewt, int n_ewt, double L, int n_reps, double ew_cut,
+                      int n_threads, double *acc, double *pot, int64_t counts[2]) {
+    cmom C;
+    to_complete(q_root, u_root, &C);
+    C.v[0][0][0] = q_root[0];
+    int nEwReps = (int)ceil(ew_cut);
+    if (nEwReps < n_reps) nEwReps = n_reps;
+    double cut2 = ew_cut * ew_cut * L * L;
+    double inner2 = 1.2e-3 * L * L; /* Ewald.cpp:119 */
+    double alpha = 2.0 / L, alpha2 = alpha * alpha;
+    double k1 = M_PI / (alpha2 * L * L * L);
+    double ka = 2.0 * alpha / sqrt(M_PI);
+    int64_t n_real = 0;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : n_real)
+#endif
+    for (int t = 0; t < n_targets; ++t) {
+        int j = targets[t];
+        double d[3] = {pos[3 * j] - root_cm[0], pos[3 * j + 1] - root_cm[1], pos[3 * j + 2] - root_cm[2]};
+        double fPot = root_mass * k1, a[3] = {0, 0, 0};
+        for (int ix = -nEwReps; ix <= nEwReps; ++ix)
+            for (int iy = -nEwReps; iy <= nEwReps; ++iy)
+                for (int iz = -nEwReps; iz <= nEwReps; ++iz) {
+                    int hole = abs(ix) <= n_reps && abs(iy) <= n_reps && abs(iz) <= n_reps;
+                    double r[3] = {d[0] + ix * L, d[1] + iy * L, d[2] + iz * L};
+                    double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                    if (r2 > cut2 && !hole) continue;
+                    double g[6];
+                    if (r2 < inner2) {
+                        /* two-term series about r = 0 (Ewald.cpp:139-156):
+                           g_n = ka (2 alpha^2)^n (alpha^2 r^2/(2n+3) - 1/(2n+1)) */
+                        double s = ka, x = alpha2 * r2;
+                        for (int n = 0; n < 6; ++n) {
+                            g[n] = s * (x / (2 * n + 3) - 1.0 / (2 * n + 1));
+                            s *= 2 * alpha2;
+                        }
+                    } else {
+                        double r1 = sqrt(r2), ir2 = 1.0 / r2;
+                        double e = ka * exp(-alpha2 * r2) * ir2;
+                        g[0] = (hole ? -erf(alpha * r1) : erfc(alpha * r1)) / r1;
+                        double s = 1.0;
+                        for (int n = 0; n < 5; ++n) {
+                            g[n + 1] = (2 * n + 1) * g[n] * ir2 + s * e;
+                            s *= 2 * alpha2;
+                        }
+                    }
+                    /* phi -= sum_n (1/n!) (Q_n . r^n) g_n;
+                       a += sum_n (1/n!) [n (Q_n . r^(n-1)) g_n - (Q_n . r^n) g_(n+1) r] */
+                    for (int n = 0; n <= 4; ++n) {
+                        if (n == 1) continue;
+                        double P, V[3], inv = 1.0 / fact(n);
+                        contract(&C, n, r, &P, V);
+                        fPot -= inv * P * g[n];
+                        for (int k = 0; k < 3; ++k) a[k] += inv * (n * V[k] * g[n] - P * g[n + 1] * r[k]);
+                    }
+                    ++n_real;
+                }
+        for (int i = 0; i < n_ewt; ++i) {
+            const double *e = ewt + 5 * i;
+            double hd = e[0] * d[0] + e[1] * d[1] + e[2] * d[2];
+            double c = cos(hd), s = sin(hd);
+            fPot += e[3] * c + e[4] * s;
+            double t = e[3] * s - e[4] * c;
+            a[0] += e[0] * t;
+            a[1] += e[1] * t;
+            a[2] += e[2] * t;
+        }
+        pot[j] += fPot;
+        acc[3 * j] += a[0];
+        acc[3 * j + 1] += a[1];
+        acc[3 * j + 2] += a[2];
+    }
+    counts[0] = n_real;
+    counts[1] = (int64_t)n_targets * n_ewt;
+}
+
+/* -------------------------------------------------------------------------
+ * Accuracy yardsticks
+ * ------------------------------------------------------------------------- */
+void orc_direct_sum(int n, const double *pos, const double *mass, const double *soft,
+                    int n_targets, const int32_t *targets, int n_threads,
+                    double *acc, double *pot) {
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int t = 0; t < n_targets; ++t) {
+        int j = targets[t];
+        double a[3] = {0, 0, 0}, p = 0.0;
+        for (int s = 0; s < n; ++s) {
+            double r[3] = {pos[3 * s] - pos[3 * j], pos[3 * s + 1] - pos[3 * j + 1],
+                           pos[3 * s + 2] - pos[3 * j + 2]};
+            double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            if (r2 == 0.0) continue;
+            double sa, sb;
+            orc_spline(r2, soft[s] + soft[j], &sa, &sb);
+            for (int k = 0; k < 3; ++k) a[k] += r[k] * sb * mass[s];
+            p -= mass[s] * sa;
+        }
+        acc[3 * t] = a[0];
+        acc[3 * t + 1] = a[1];
+        acc[3 * t + 2] = a[2];
+        pot[t] = p;
+    }
+}
+
+void orc_ewald_exact(int n, const double *pos, const double *mass, const double *soft,
+                     int n_targets, const int32_t *targets, double L, double alpha,
+                     double rcut, int hmax, int n_threads, double *acc, double *pot) {
+    const double V = L * L * L;
+    const double ka = 2.0 * alpha / sqrt(M_PI);
+    double Mtot = 0.0;
+    for (int i = 0; i < n; ++i) Mtot += mass[i];
+    /* half space of h: hx > 0, or hx == 0 && hy > 0, or hx == hy == 0 && hz > 0 */
+    int nh = 0;
+    int H = 2 * hmax + 1;
+    int *hl = (int *)malloc(sizeof(int) * 3 * H * H * H);
+    for (int hx = 0; hx <= hmax; ++hx)
+        for (int hy = -hmax; hy <= hmax; ++hy)
+            for (int hz = -hmax; hz <= hmax; ++hz) {
+                if (hx == 0 && (hy < 0 || (hy == 0 && hz <= 0))) continue;
+                if (hx * hx + hy * hy + hz * hz > hmax * hmax) continue;
+                hl[3 * nh] = hx;
+                hl[3 * nh + 1] = hy;
+                hl[3 * nh + 2] = hz;
+                ++nh;
+            }
+    double *Sre = (double *)calloc(nh, sizeof(double));
+    double *Sim = (double *)calloc(nh, sizeof(double));
+    int nt = 1;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel
+    {
+#pragma omp single
+        nt = omp_get_num_threads();
+    }
+#endif
+    double *part = (double *)calloc((size_t)nt * 2 * nh, sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *pre = part + (size_t)tid * 2 * nh, *pim = pre + nh;
+        double *cx = (double *)malloc(sizeof(double) * 2 * (hmax + 1));
+        double *cy = (double *)malloc(sizeof(double) * 2 * (2 * hmax + 1));
+        double *cz = (double *)malloc(sizeof(double) * 2 * (2 * hmax + 1));
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int i = 0; i < n; ++i) {
+            /* e^{-i k x} for k = 2 pi h / L, h = 0..hmax (x) and -hmax..hmax (y, z) */
+            double th[3];
+            for (int k = 0; k < 3; ++k) th[k] = -2 * M_PI * pos[3 * i + k] / L;
+            for (int h = 0; h <= hmax; ++h) {
+                cx[2 * h] = cos(h * th[0]);
+                cx[2 * h + 1] = sin(h * th[0]);
+            }
+            for (int h = -hmax; h <= hmax; ++h) {
+                cy[2 * (h + hmax)] = cos(h * th[1]);
+                cy[2 * (h + hmax) + 1] = sin(h * th[1]);
+                cz[2 * (h + hmax)] = cos(h * th[2]);
+                cz[2 * (h + hmax) + 1] = sin(h * th[2]);
+            }
+            for (int q = 0; q < nh; ++q) {
+                const double *a = cx + 2 * hl[3 * q];
+                const double *b = cy + 2 * (hl[3 * q + 1] + hmax);
+                const double *c = cz + 2 * (hl[3 * q + 2] + hmax);
+                double re = a[0] * b[0] - a[1] * b[1], im = a[0] * b[1] + a[1] * b[0];
+                double re2 = re * c[0] - im * c[1], im2 = re * c[1] + im * c[0];
+                pre[q] += mass[i] * re2;
+                pim[q] += mass[i] * im2;
+            }
+        }
+        free(cx);
+        free(cy);
+        free(cz);
+    }
+    for (int t = 0; t < nt; ++t)
+        for (int q = 0; q < nh; ++q) {
+            Sre[q] += part[(size_t)t * 2 * nh + q];
+            Sim[q] += part[(size_t)t * 2 * nh + nh + q];
+        }
+    free(part);
+
+    /* cell grid for real space */
+    int nc = (int)floor(L / rcut);
+    if (nc < 1) nc = 1;
+    if (nc > 64) nc = 64;
+    double cs = L / nc;
+    int ncell = nc * nc * nc;
+    int *cstart = (int *)calloc(ncell + 1, sizeof(int));
+    int *cidx = (int *)malloc(sizeof(int) * n);
+    int *pc = (int *)malloc(sizeof(int) * n);
+    for (int i = 0; i < n; ++i) {
+        int c[3];
+        for (int k = 0; k < 3; ++k) {
+            double x = pos[3 * i + k] - floor(pos[3 * i + k] / L) * L;
+            c[k] = (int)(x / cs);
+            if (c[k] >= nc) c[k] = nc - 1;
+        }
+        pc[i] = (c[0] * nc + c[1]) * nc + c[2];
+        cstart[pc[i] + 1]++;
+    }
+    for (int c = 0; c < ncell; ++c) cstart[c + 1] += cstart[c];
+    int *fill = (int *)calloc(ncell, sizeof(int));
+    for (int i = 0; i < n; ++i) cidx[cstart[pc[i]] + fill[pc[i]]++] = i;
+    free(fill);
+    int rng = (int)ceil(rcut / cs);
+    if (2 * rng + 1 > nc) rng = -1; /* scan every cell */
+
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int t = 0; t < n_targets; ++t) {
+        int i = targets[t];
+        const double *xi = pos + 3 * i;
+        double p = Mtot * M_PI / (alpha * alpha * V);
+        double a[3] = {0, 0, 0};
+        /* self image energy, real-space part: lim (erfc(ar)/r - 1/r) */
+        p -= mass[i] * (-ka);
+        int ci[3];
+        for (int k = 0; k < 3; ++k) {
+            double x = xi[k] - floor(xi[k] / L) * L;
+            ci[k] = (int)(x / cs);
+            if (ci[k] >= nc) ci[k] = nc - 1;
+        }
+        int lo = rng < 0 ? 0 : -rng, hi = rng < 0 ? nc - 1 : rng;
+        for (int ox = lo; ox <= hi; ++ox)
+            for (int oy = lo; oy <= hi; ++oy)
+                for (int oz = lo; oz <= hi; ++oz) {
+                    int cx = rng < 0 ? ox : ((ci[0] + ox) % nc + nc) % nc;
+                    int cy = rng < 0 ? oy : ((ci[1] + oy) % nc + nc) % nc;
+                    int cz = rng < 0 ? oz : ((ci[2] + oz) % nc + nc) % nc;
+                    int c = (cx * nc + cy) * nc + cz;
+                    for (int q = cstart[c]; q < cstart[c + 1]; ++q) {
+                        int j = cidx[q];
+                        if (j == i) continue;
+                        double d[3];
+                        for (int k = 0; k < 3; ++k) {
+                            d[k] = xi[k] - pos[3 * j + k];
+                            d[k] -= L * floor(d[k] / L + 0.5);
+                        }
+                        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                        if (r2 >= rcut * rcut || r2 == 0.0) continue;
+                        double r = sqrt(r2);
+                        double g0 = erfc(alpha * r) / r;
+                        double g1 = g0 / r2 + ka * exp(-alpha * alpha * r2) / r2;
+                        p -= mass[j] * g0;
+                        for (int k = 0; k < 3; ++k) a[k] -= mass[j] * g1 * d[k];
+                        double twoh = soft[i] + soft[j];
+                        if (r < twoh) {
+                            double sa, sb;
+                            orc_spline(r2, twoh, &sa, &sb);
+                            p -= mass[j] * (sa - 1.0 / r);
+                            for (int k = 0; k < 3; ++k) a[k] -= mass[j] * (sb - 1.0 / (r2 * r)) * d[k];
+                        }
+                    }
+                }
+        /* k space */
+        double th[3];
+        for (int k = 0; k < 3; ++k) th[k] = 2 * M_PI * xi[k] / L;
+        for (int q = 0; q < nh; ++q) {
+            double kv[3] = {2 * M_PI / L * hl[3 * q], 2 * M_PI / L * hl[3 * q + 1], 2 * M_PI / L * hl[3 * q + 2]};
+            double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+            double w = 2.0 * 4 * M_PI / V * exp(-k2 / (4 * alpha * alpha)) / k2; /* x2: h and -h */
+            double ph = hl[3 * q] * th[0] + hl[3 * q + 1] * th[1] + hl[3 * q + 2] * th[2];
+            double c = cos(ph), s = sin(ph);
+            /* e^{ik.x_i} S(h) */
+            double re = c * Sre[q] - s * Sim[q], im = c * Sim[q] + s * Sre[q];
+            p -= w * re;
+            for (int k = 0; k < 3; ++k) a[k] -= w * kv[k] * im;
+        }
+        pot[t] = p;
+        acc[3 * t] = a[0];
+        acc[3 * t + 1] = a[1];
+        acc[3 * t + 2] = a[2];
+    }
+    free(cstart);
+    free(cidx);
+    free(pc);
+    free(hl);
+    free(Sre);
+    free(Sim);
+}

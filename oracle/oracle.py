@@ -1,0 +1,233 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle.c) and of the reference's own
+moments.c compiled into oracle/_ref/.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker and by bench.py's cpu_baseline leg. The product never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NMOM = 22
+
+_lib = None
+_ref = None
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_p = ctypes.c_void_p
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def lib():
+    """Load liboracle.so (building it with make if absent and gcc exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = os.path.join(HERE, "liboracle.so")
+    if not os.path.exists(path):
+        import subprocess
+        subprocess.run(["make", "-C", HERE, "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    L = ctypes.CDLL(path)
+    L.orc_spline.argtypes = [_d, _d, _p, _p]
+    L.orc_mom_make.argtypes = [_p, _d, _d, _d, _d, _d]
+    L.orc_mom_make.restype = _d
+    L.orc_mom_shift.argtypes = [_p, _d, _d, _d, _d]
+    L.orc_mom_rescale.argtypes = [_p, _d, _d]
+    L.orc_mom_scaled_add.argtypes = [_p, _d, _p, _d]
+    L.orc_mom_to_momc.argtypes = [_p, _p]
+    L.orc_mom_eval.argtypes = [_p, _d, _d, _d, _d, _p, _p]
+    L.orc_build_moments.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p]
+    L.orc_forces_lists.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i,
+                                   _p, _p, _p, _p, _d, _i, _p, _p, _p, _p]
+    L.orc_open_softening.argtypes = [_p, _p, _d, _d, _d]
+    L.orc_open_softening.restype = _i
+    L.orc_ewald_table.argtypes = [_p, _d, _d, _d, _p, _i]
+    L.orc_ewald_table.restype = _i
+    L.orc_bucket_ewald.argtypes = [_i, _p, _p, _p, _d, _p, _d, _p, _i, _d, _i, _d, _i,
+                                   _p, _p, _p]
+    L.orc_direct_sum.argtypes = [_i, _p, _p, _p, _i, _p, _i, _p, _p]
+    L.orc_ewald_exact.argtypes = [_i, _p, _p, _p, _i, _p, _d, _d, _d, _i, _i, _p, _p]
+    _lib = L
+    return L
+
+
+def ref_lib():
+    """The reference's moments.c, compiled from its own source (oracle/_ref).
+    Returns None when it was never built (reference tree absent)."""
+    global _ref
+    if _ref is not None:
+        return _ref
+    path = os.path.join(HERE, "_ref", "libmoments_ref.so")
+    if not os.path.exists(path):
+        return None
+    R = ctypes.CDLL(path)
+    R.momMakeFmomr.argtypes = [_p, _d, _d, _d, _d, _d]
+    R.momMakeFmomr.restype = _d
+    R.momShiftFmomr.argtypes = [_p, _d, _d, _d, _d]
+    R.momRescaleFmomr.argtypes = [_p, _d, _d]
+    R.momScaledAddFmomr.argtypes = [_p, _d, _p, _d]
+    R.momFmomr2Momc.argtypes = [_p, _p]
+    R.momEvalFmomrcm.argtypes = [_p, _d, _d, _d, _d, _d, _p, _p, _p, _p, _p]
+    _ref = R
+    return R
+
+
+# ---------------------------------------------------------------- helpers
+def spline(r2, twoh):
+    a = np.zeros(1)
+    b = np.zeros(1)
+    lib().orc_spline(r2, twoh, _ptr(a), _ptr(b))
+    return a[0], b[0]
+
+
+def mom_make(m, u, x, y, z):
+    q = np.zeros(NMOM)
+    lib().orc_mom_make(_ptr(q), m, u, x, y, z)
+    return q
+
+
+def mom_shift(q, u, dx, dy, dz):
+    q = np.array(q, dtype=np.float64)
+    lib().orc_mom_shift(_ptr(q), u, dx, dy, dz)
+    return q
+
+
+def mom_rescale(q, unew, uold):
+    q = np.array(q, dtype=np.float64)
+    lib().orc_mom_rescale(_ptr(q), unew, uold)
+    return q
+
+
+def mom_scaled_add(qr, ur, qa, ua):
+    qr = np.array(qr, dtype=np.float64)
+    qa = np.ascontiguousarray(qa, dtype=np.float64)
+    lib().orc_mom_scaled_add(_ptr(qr), ur, _ptr(qa), ua)
+    return qr
+
+
+def mom_to_momc(q):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    mc = np.zeros(32)
+    lib().orc_mom_to_momc(_ptr(q), _ptr(mc))
+    return mc
+
+
+def mom_eval(q, u, x, y, z):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    pot = np.zeros(1)
+    a = np.zeros(3)
+    lib().orc_mom_eval(_ptr(q), u, x, y, z, _ptr(pot), _ptr(a))
+    return pot[0], a
+
+
+def build_moments(tree, pos, mass, soft):
+    """FP64 moments of every node of `tree` (a dict of topology arrays as
+    produced by changa_amd.host.build_tree). Returns (geo[n,12], mom[n,22])."""
+    nn = int(tree["n_nodes"])
+    geo = np.zeros((nn, 12))
+    mom = np.zeros((nn, NMOM))
+    lib().orc_build_moments(nn, _ptr(tree["node_first"]), _ptr(tree["node_count"]),
+                            _ptr(tree["node_child"]), _ptr(tree["node_box_radius"]),
+                            _ptr(pos), _ptr(mass), _ptr(soft), _ptr(geo), _ptr(mom))
+    return geo, mom
+
+
+def forces_lists(tree, geo, mom, pos, mass, soft, lists, period=0.0, rung=None,
+                 active_rung=0, n_threads=0, buckets=None):
+    """nodeBucketForce/partBucketForce over the interaction lists. `buckets`
+    restricts the evaluation to a subset of bucket indices (the lists are then
+    re-sliced). Returns acc[N,3], pot[N], dtgrav[N], counts[3]."""
+    n = pos.shape[0]
+    acc = np.zeros((n, 3))
+    pot = np.zeros(n)
+    dtg = np.zeros(n)
+    counts = np.zeros(3, dtype=np.int64)
+    bnode = tree["bucket_node"]
+    cells, cmarks = lists["cells"], lists["cell_marks"]
+    parts, pmarks = lists["parts"], lists["part_marks"]
+    if buckets is not None:
+        buckets = np.asarray(buckets, dtype=np.int64)
+        bnode = np.ascontiguousarray(bnode[buckets])
+        cells, cmarks = _reslice(cells, cmarks, buckets, 2)
+        parts, pmarks = _reslice(parts, pmarks, buckets, 3)
+    rg = None if rung is None else np.ascontiguousarray(rung, dtype=np.int32)
+    lib().orc_forces_lists(len(bnode), _ptr(bnode), _ptr(tree["node_first"]),
+                           _ptr(tree["node_count"]), _ptr(geo), _ptr(mom), _ptr(pos),
+                           _ptr(mass), _ptr(soft), _ptr(rg), active_rung,
+                           _ptr(cells), _ptr(cmarks), _ptr(parts), _ptr(pmarks),
+                           float(period), n_threads, _ptr(acc), _ptr(pot), _ptr(dtg),
+                           _ptr(counts))
+    return acc, pot, dtg, counts
+
+
+def _reslice(lst, marks, buckets, width):
+    lst = lst.reshape(-1, width)
+    lens = marks[buckets + 1] - marks[buckets]
+    newm = np.zeros(len(buckets) + 1, dtype=np.int64)
+    newm[1:] = np.cumsum(lens)
+    idx = np.concatenate([np.arange(marks[b], marks[b + 1]) for b in buckets]) \
+        if len(buckets) else np.zeros(0, dtype=np.int64)
+    return np.ascontiguousarray(lst[idx].reshape(-1)), newm
+
+
+def open_softening(cell_geo, bucket_geo, off):
+    return bool(lib().orc_open_softening(_ptr(np.ascontiguousarray(cell_geo)),
+                                         _ptr(np.ascontiguousarray(bucket_geo)),
+                                         off[0], off[1], off[2]))
+
+
+def ewald_table(q_root, u_root, L, hcut, max_n=4096):
+    q = np.ascontiguousarray(q_root, dtype=np.float64)
+    ewt = np.zeros((max_n, 5))
+    n = lib().orc_ewald_table(_ptr(q), u_root, L, hcut, _ptr(ewt), max_n)
+    if n < 0:
+        raise ValueError("Ewald table larger than max_n")
+    return ewt[:n].copy()
+
+
+def bucket_ewald(targets, pos, root_cm, root_mass, q_root, u_root, ewt, L, n_reps,
+                 ew_cut, n_threads=0):
+    n = pos.shape[0]
+    acc = np.zeros((n, 3))
+    pot = np.zeros(n)
+    counts = np.zeros(2, dtype=np.int64)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    cm = np.ascontiguousarray(root_cm, dtype=np.float64)
+    q = np.ascontiguousarray(q_root, dtype=np.float64)
+    e = np.ascontiguousarray(ewt, dtype=np.float64)
+    lib().orc_bucket_ewald(len(t), _ptr(t), _ptr(pos), _ptr(cm), root_mass, _ptr(q), u_root,
+                           _ptr(e), len(e), L, n_reps, ew_cut, n_threads, _ptr(acc),
+                           _ptr(pot), _ptr(counts))
+    return acc, pot, counts
+
+
+def direct_sum(pos, mass, soft, targets, n_threads=0):
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    acc = np.zeros((len(t), 3))
+    pot = np.zeros(len(t))
+    lib().orc_direct_sum(pos.shape[0], _ptr(pos), _ptr(mass), _ptr(soft), len(t), _ptr(t),
+                         n_threads, _ptr(acc), _ptr(pot))
+    return acc, pot
+
+
+def ewald_exact(pos, mass, soft, targets, L=1.0, alpha=None, rcut=None, hmax=None,
+                n_threads=0):
+    """FP64 periodic softened forces on `targets` by a converged Ewald sum."""
+    rcut = 0.3 * L if rcut is None else rcut
+    alpha = 4.0 / rcut if alpha is None else alpha
+    if hmax is None:
+        # exp(-k^2/4a^2) < 1e-8 at |k| = 2 pi hmax / L
+        hmax = int(np.ceil(2 * alpha * np.sqrt(np.log(1e8)) * L / (2 * np.pi)))
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    acc = np.zeros((len(t), 3))
+    pot = np.zeros(len(t))
+    lib().orc_ewald_exact(pos.shape[0], _ptr(pos), _ptr(mass), _ptr(soft), len(t), _ptr(t),
+                          L, alpha, rcut, hmax, n_threads, _ptr(acc), _ptr(pot))
+    return acc, pot
