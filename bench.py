@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--ng", type=int, default=128, help="lattice size of the periodic box")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy legs")
-    ap.add_argument("--cpu-frac", type=float, default=0.02)
+    ap.add_argument("--cpu-frac", type=float, default=0.5)
     ap.add_argument("--samples", type=int, default=256)
     return ap.parse_args()
 
@@ -124,7 +124,7 @@ def main():
             "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s",
             "frac": round(pc_achieved * 1e12 / PEAK_FP32, 4),
-            "traffic": None,
+            "traffic": traffic_from_profile(args.ng),
         },
     }
 
@@ -133,6 +133,18 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ev, args.cpu_frac)
     print(json.dumps(out))
+
+
+def traffic_from_profile(ng):
+    """HBM bytes per pc_kernel launch, from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_zeldovich{ng}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("pc_kernel traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def accuracy(ev, n_samples):

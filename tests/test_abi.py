@@ -39,3 +39,22 @@ def test_wire_struct_sizes():
     assert ctypes.sizeof(gpu.EwtData) == 20
     assert ctypes.sizeof(gpu.CudaRequest) > 0
     assert ctypes.sizeof(gpu.ChangaGravityLaunch) == 8 * 8 + 4 * 4 + 3 * 8
+
+
+def test_header_compiles_as_c_and_links(tmp_path):
+    """A plain-C caller (what a ChaNGa-side shim would be) compiles against the
+    header and links every entry point it takes the address of."""
+    import subprocess
+    names = declared()
+    src = tmp_path / "caller.c"
+    body = "\n".join(f"  (void *)&{n}," for n in names)
+    src.write_text('#include "changa_gpu.h"\n#include <stdio.h>\n'
+                   f"void *table[] = {{\n{body}\n}};\n"
+                   "int main(void) { printf(\"%d\\n\", (int)(sizeof(table)/sizeof(table[0]))); "
+                   "return sizeof(CudaMultipoleMoments) != 108; }\n")
+    inc = os.path.join(os.path.dirname(HDR))
+    libdir = os.path.dirname(build.GPU_LIB)
+    exe = tmp_path / "caller"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I" + inc, str(src), "-o", str(exe),
+                    "-L" + libdir, "-lchanga_gpu", "-Wl,-rpath," + libdir,
+                    "-Wl,--unresolved-symbols=ignore-in-shared-libs"], check=True)

@@ -73,7 +73,9 @@ def test_gpu_pp_list_formats_agree(device):
         ev.zero()
         ev.run_pp()
     torch.cuda.synchronize()
-    assert torch.equal(a.d_vars, b.d_vars)
+    # same sources and order; only where FP32 tile sums fold into FP64 differs
+    va, vb = a.d_vars.cpu().double(), b.d_vars.cpu().double()
+    assert ((va - vb).abs() <= 1e-6 * vb.abs().amax(dim=0)).all()
 
 
 @pytest.mark.parametrize("precise", [True, False])
