@@ -107,10 +107,12 @@ __device__ __forceinline__ void radial64(double r2, bool hole, const EwaldParams
 __device__ __forceinline__ void radial32(float r2, bool hole, float alpha, float alpha2, float ka,
                                          float inner2, float* g) {
   if (r2 < inner2) {
+    const float inv_odd[7] = {1.0f, 1.0f / 3.0f, 1.0f / 5.0f, 1.0f / 7.0f,
+                              1.0f / 9.0f, 1.0f / 11.0f, 1.0f / 13.0f};
     float s = ka, x = alpha2 * r2;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      g[k] = s * (x / (2 * k + 3) - 1.0f / (2 * k + 1));
+      g[k] = s * (x * inv_odd[k + 1] - inv_odd[k]);
       s *= 2 * alpha2;
     }
   } else {
@@ -134,15 +136,19 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
                                                          const int* markers, int first, int n,
                                                          const EwtData* ewt, const double* pos64,
                                                          EwaldParams p) {
-  extern __shared__ float4 sh[];  // h-loop table: 2 float4 per entry
-  __shared__ short cand[kEwBlock / kWave][kEwMaxCand];
+  // h-loop table: per entry (k, hC) and (h as integers, hS); dynamic size
+  extern __shared__ float4 sh[];
+  // lattice offsets (n L, hole flag) that some particle of the wavefront needs
+  __shared__ float4 cand[kEwBlock / kWave][kEwMaxCand];
   __shared__ int ncand[kEwBlock / kWave];
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const float kscale = (float)(p.L / (2.0 * M_PI));  // k -> integer h
   for (int i = tid; i < p.nH; i += kEwBlock) {
     EwtData e = ewt[i];
     sh[2 * i] = make_float4(e.hx, e.hy, e.hz, e.hCfac);
-    sh[2 * i + 1] = make_float4(e.hSfac, 0.f, 0.f, 0.f);
+    sh[2 * i + 1] = make_float4(rintf(e.hx * kscale), rintf(e.hy * kscale), rintf(e.hz * kscale),
+                                e.hSfac);
   }
 
   const int gi = blockIdx.x * kEwBlock + tid;
@@ -192,7 +198,11 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
     }
     unsigned long long mask = __ballot(need);
     int before = __popcll(mask & ((1ull << lane) - 1ull));
-    if (need) cand[wv][ncand[wv] + before] = (short)c;
+    if (need) {
+      int ix = c / (W * W) - p.nEwReps, iy = (c / W) % W - p.nEwReps, iz = c % W - p.nEwReps;
+      bool hole = abs(ix) <= p.nReps && abs(iy) <= p.nReps && abs(iz) <= p.nReps;
+      cand[wv][ncand[wv] + before] = make_float4(ix * Lf, iy * Lf, iz * Lf, hole ? 1.0f : 0.0f);
+    }
     __syncthreads();
     if (lane == 0) ncand[wv] += __popcll(mask);
     __syncthreads();
@@ -221,10 +231,9 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
   const float alpha = (float)p.alpha, alpha2 = (float)p.alpha2, ka = (float)p.ka,
               inner2 = (float)p.inner2, cut2 = (float)p.cut2;
   for (int k = 0; k < nc; ++k) {
-    int c = cand[wv][k];
-    int ix = c / (W * W) - p.nEwReps, iy = (c / W) % W - p.nEwReps, iz = c % W - p.nEwReps;
-    bool hole = abs(ix) <= p.nReps && abs(iy) <= p.nReps && abs(iz) <= p.nReps;
-    float x = fx + ix * Lf, y = fy + iy * Lf, z = fz + iz * Lf;
+    const float4 cv = cand[wv][k];
+    const bool hole = cv.w != 0.0f;
+    float x = fx + cv.x, y = fy + cv.y, z = fz + cv.z;
     float r2 = x * x + y * y + z * z;
     if (r2 > cut2 && !hole) continue;
     float g[6];
@@ -236,13 +245,18 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
     ay += ty;
     az += tz;
   }
-  // k space
+  // k space: phase h.d / L in revolutions (h integer), reduced exactly, then
+  // sin / cos of pi times twice that
+  const float iL = (float)(1.0 / p.L);
+  const float ux = fx * iL, uy = fy * iL, uz = fz * iL;
   for (int i = 0; i < p.nH; ++i) {
-    float4 e0 = sh[2 * i];
-    float hs = sh[2 * i + 1].x;
-    float hd = e0.x * fx + e0.y * fy + e0.z * fz;
+    const float4 e0 = sh[2 * i];
+    const float4 e1 = sh[2 * i + 1];
+    const float hs = e1.w;
+    float ph = e1.x * ux + e1.y * uy + e1.z * uz;
+    ph -= rintf(ph);
     float s, c;
-    sincosf(hd, &s, &c);
+    sincospif(2.0f * ph, &s, &c);
     pot += (double)(e0.w * c + hs * s);
     float t = e0.w * s - hs * c;
     ax += (double)(e0.x * t);

@@ -73,6 +73,7 @@ class Evaluation:
         self.w = w
         self.precise = precise
         self.pp_compact = pp_compact
+        self.pc_variant = 0
         self.device = torch.device(device)
         periodic = bool(w["periodic"])
         self.L = float(w["L"]) if periodic else 0.0
@@ -207,6 +208,7 @@ class Evaluation:
         a.numBuckets = self.tree["n_buckets"]
         a.listKind = 0 if pc else (1 if self.pp_compact else 0)
         a.fperiod = self.L
+        a.variant = self.pc_variant if pc else 0
         if self.precise:
             a.tpos64 = self.d_pos64.data_ptr()
             a.spos64 = self.d_pos64.data_ptr()

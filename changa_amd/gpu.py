@@ -65,7 +65,7 @@ class ChangaGravityLaunch(ctypes.Structure):
     _fields_ = [("targets", c_void_p), ("vars", c_void_p), ("moments", c_void_p),
                 ("sources", c_void_p), ("list", c_void_p), ("marks", c_void_p),
                 ("starts", c_void_p), ("sizes", c_void_p), ("numBuckets", c_int),
-                ("listKind", c_int), ("fperiod", c_float), ("reserved", c_int),
+                ("listKind", c_int), ("fperiod", c_float), ("variant", c_int),
                 ("tpos64", c_void_p), ("spos64", c_void_p), ("cm64", c_void_p)]
 
 

@@ -1,0 +1,162 @@
+"""The HostCUDA/EwaldCUDA mirror (include/changa_gpu.h Part 1) on the GPU.
+
+Drives the entry points the way ChaNGa's DataManager / ListCompute /
+TreePiece::EwaldGPU do: upload the local tree, send the cell and particle
+interaction lists as several requests with host-side lists and per-request
+bucket markers, run EwaldHost, copy the accelerations back, and check that
+each request's callback fires exactly once. The result must equal the
+device-resident path bit for bit (same kernels, same per-bucket lists)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from changa_amd import configs, engine, gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _hip():
+    return ctypes.CDLL("libamdhip64.so")
+
+
+def _requests(ev, lst, marks, width, n_req):
+    """Split the buckets into n_req requests with request-local markers."""
+    nb = ev.tree["n_buckets"]
+    cuts = np.linspace(0, nb, n_req + 1).astype(int)
+    out = []
+    for r in range(n_req):
+        b0, b1 = cuts[r], cuts[r + 1]
+        m = marks[b0:b1 + 1].astype(np.int64)
+        seg = np.ascontiguousarray(lst.reshape(-1, width)[m[0]:m[-1]].reshape(-1))
+        out.append(dict(list=seg, marks=np.ascontiguousarray((m - m[0]).astype(np.int32)),
+                        starts=np.ascontiguousarray(ev.tree["bucket_start"][b0:b1]),
+                        sizes=np.ascontiguousarray(ev.tree["bucket_size"][b0:b1]),
+                        n=int(m[-1] - m[0]), nb=int(b1 - b0)))
+    return out
+
+
+def _run_mirror(ev, pp_expanded):
+    import torch
+
+    L = gpu.lib()
+    fired = []
+    handler = gpu.CALLBACK(lambda cb: fired.append(cb))
+    L.changaGpuSetCallbackHandler(handler)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    n = ev.pos.shape[0]
+    moments = ev.d_moments.cpu().numpy()
+    parts = ev.d_parts.cpu().numpy()
+    varp = np.zeros((n, 5), dtype=np.float32)
+    dm, dp, dv = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    L.DataManagerTransferLocalTree(moments.ctypes.data, moments.nbytes, parts.ctypes.data,
+                                   parts.nbytes, varp.ctypes.data, varp.nbytes, ctypes.byref(dm),
+                                   ctypes.byref(dp), ctypes.byref(dv), stream, n,
+                                   ctypes.c_void_p(1001))
+    keep = []
+    cb = 2000
+
+    def send(fn, reqs, sources_remote=False):
+        nonlocal cb
+        for r in reqs:
+            q = gpu.CudaRequest()
+            q.stream = stream
+            q.d_localMoments = dm
+            q.d_localParts = dp
+            q.d_localVars = dv
+            q.list = r["list"].ctypes.data
+            q.bucketMarkers = r["marks"].ctypes.data
+            q.bucketStarts = r["starts"].ctypes.data
+            q.bucketSizes = r["sizes"].ctypes.data
+            q.numInteractions = r["n"]
+            q.numBucketsPlusOne = r["nb"] + 1
+            q.fperiod = ev.L
+            q.cb = cb
+            cb += 1
+            keep.append((q, r))
+            fn(ctypes.byref(q))
+
+    send(L.TreePieceCellListDataTransferLocal, _requests(ev, ev.lists["cells"],
+                                                         ev.lists["cell_marks"], 2, 3))
+    if pp_expanded:
+        cells, marks = engine.expand_parts(ev.lists)
+        send(L.TreePiecePartListDataTransferLocal, _requests(ev, cells, marks, 2, 3))
+    else:
+        send(L.TreePiecePartListDataTransferLocalCompact,
+             _requests(ev, ev.lists["parts"], ev.lists["part_marks"], 3, 3))
+    ncb = cb - 2000
+    if ev.w["periodic"]:
+        h = gpu.EwaldData()
+        L.EwaldHostMemorySetup(ctypes.byref(h), n, len(ev.ewt64), 1)
+        for i in range(n):
+            h.EwaldMarkers[i] = i
+        for i, e in enumerate(ev.ewt64.astype(np.float32)):
+            h.ewt[i] = gpu.EwtData(*[float(x) for x in e])
+        ctypes.memmove(h.cachedData, ctypes.byref(ev.ewald_ro), ctypes.sizeof(ev.ewald_ro))
+        h.EwaldRange[0], h.EwaldRange[1] = 0, n - 1
+        L.EwaldHost(dp, dv, ctypes.byref(h), stream, ctypes.c_void_p(3001), 0, 1)
+    host_out = ctypes.c_void_p()
+    L.allocatePinnedHostMemory(ctypes.byref(host_out), n * 20)
+    L.TransferParticleVarsBack(host_out, n * 20, dv, stream, ctypes.c_void_p(4001))
+    torch.cuda.synchronize()
+    out = np.ctypeslib.as_array(ctypes.cast(host_out, ctypes.POINTER(ctypes.c_float)),
+                                shape=(n, 5)).copy()
+    if ev.w["periodic"]:
+        L.EwaldHostMemoryFree(ctypes.byref(h), 1)
+    L.freePinnedHostMemory(host_out)
+    hip = _hip()
+    for p in (dm, dp, dv):
+        assert hip.hipFree(p) == 0
+    L.changaGpuSetCallbackHandler(gpu.CALLBACK(0))
+    expect = [1001] + list(range(2000, 2000 + ncb)) + ([3001] if ev.w["periodic"] else []) + [4001]
+    return out, fired, expect
+
+
+@pytest.mark.parametrize("case", ["plummer_soft", "periodic"])
+def test_gpu_mirror_matches_device_path(device, case):
+    import torch
+    if case == "plummer_soft":
+        w = configs.plummer(n=3000, eps=0.5)
+    else:
+        w = configs.zeldovich(ng=16, rms=0.03)
+    ev = engine.Evaluation(w, precise=False, pp_compact=False, device=device)
+    assert case != "plummer_soft" or ev.counts[1] > 0
+    out, fired, expect = _run_mirror(ev, pp_expanded=True)
+    assert sorted(fired) == sorted(expect) and len(fired) == len(set(fired))
+    ev.zero()
+    ev.run_pc()
+    ev.run_pp()
+    ev.run_ewald()
+    torch.cuda.synchronize()
+    ref = ev.d_vars.cpu().numpy()
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_gpu_mirror_compact_part_lists(device):
+    import torch
+    w = configs.plummer(n=3000)
+    ev = engine.Evaluation(w, precise=False, pp_compact=True, device=device)
+    out, fired, expect = _run_mirror(ev, pp_expanded=False)
+    assert sorted(fired) == sorted(expect)
+    ev.zero()
+    ev.run_pc()
+    ev.run_pp()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out, ev.d_vars.cpu().numpy())
+
+
+def test_gpu_error_mode_returns(device):
+    """With abort-on-error off, a bad launch returns an error code and the
+    message names the call (no fault: the library rejects it on the host)."""
+    L = gpu.lib()
+    L.changaGpuSetAbortOnError(0)
+    try:
+        a = gpu.ChangaGravityLaunch()
+        a.numBuckets = 5  # no arrays
+        rc = L.changaNodeGravity(ctypes.byref(a), None)
+        assert rc != 0
+        msg = ctypes.c_char_p()
+        assert L.changaGpuLastError(ctypes.byref(msg)) != 0
+        assert b"missing" in msg.value
+    finally:
+        L.changaGpuSetAbortOnError(1)
