@@ -2,11 +2,18 @@
 
 One step = one pass of the hot path over the resident workload: GPU
 multipole build, zeroing of the accelerations, particle-cell (hexadecapole)
-kernel, particle-particle kernel and Ewald correction, on interaction lists
-already in HBM. Workload: the 128^3 periodic Zel'dovich box (2,097,152
-particles, Ewald nReplicas = 1, fEwCut = 2.6, |h|^2 <= 10, theta = 0.7,
-bucket 16). The metric is G interactions/s, an interaction being one (active
-target, source cell / particle / Ewald term) pair as ChaNGa counts them.
+and particle-particle evaluation, and the Ewald correction, on interaction
+lists already in HBM. The cell and particle lists run either in the batched
+form (one persistent launch replayed from a hipGraph, autotuned; the
+default) or per bucket (one launch per list kind, ChaNGa's pattern); both are
+timed and reported.
+
+Workloads: --config 2 (default), the 128^3 periodic Zel'dovich box
+(2,097,152 particles, Ewald nReplicas = 1, fEwCut = 2.6, |h|^2 <= 10,
+theta = 0.7, bucket 16); --config 3, 2^22 particles in 512 Hernquist halos
+plus a 20% uniform background (theta = 0.6, bucket 32, isolated). The metric
+is G interactions/s, an interaction being one (active target, source cell /
+particle / Ewald term) pair as ChaNGa counts them.
 
 Prints one JSON line. --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -32,10 +39,15 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--form", choices=["precise", "plain"], default="precise")
+    ap.add_argument("--path", choices=["batched", "per-bucket"], default="batched")
+    ap.add_argument("--config", type=int, choices=[2, 3], default=2)
     ap.add_argument("--ng", type=int, default=128, help="lattice size of the periodic box")
+    ap.add_argument("--n", type=int, default=1 << 22, help="particles of the halo workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy legs")
-    ap.add_argument("--cpu-frac", type=float, default=0.5)
+    ap.add_argument("--cpu-frac", type=float, default=None,
+                    help="fraction of the buckets the CPU baseline evaluates "
+                         "(default 1 for config 2, 1/4 for config 3: about 10-20 s of CPU work)")
     ap.add_argument("--samples", type=int, default=256)
     return ap.parse_args()
 
@@ -51,42 +63,70 @@ def main():
     from changa_amd import configs, engine
 
     t0 = time.time()
-    w = configs.zeldovich(ng=args.ng, rms=0.02, cfg=2)
+    if args.config == 2:
+        w = configs.zeldovich(ng=args.ng, rms=0.02, cfg=2)
+        workload = (f"periodic Zel'dovich box {args.ng}^3 ({args.ng ** 3} particles), "
+                    "P(k)~k^-2.5, rms 0.02, Ewald nReplicas=1 fEwCut=2.6 |h|^2<=10, "
+                    "theta=0.7, bucket 16, hexadecapole")
+        data = "synthetic (seeded Zel'dovich lattice, numpy PCG64 seed 1002)"
+        tag = f"zeldovich{args.ng}"
+    else:
+        w = configs.halos(n=args.n, cfg=3)
+        workload = (f"{args.n} particles: 512 Hernquist halos (dN/dM~M^-1.9 over 3 decades, "
+                    "a~M^(1/3)) + 20% uniform background, isolated, theta=0.6, bucket 32, "
+                    "hexadecapole")
+        data = "synthetic (seeded halos, numpy PCG64 seed 1003)"
+        tag = f"halos{args.n}"
     ev = engine.Evaluation(w, precise=(args.form == "precise"))
     setup_s = time.time() - t0
     counts = ev.n_interactions()
     n_int = sum(counts.values())
     flops = sum(FLOP[k] * v for k, v in counts.items())
-
-    for _ in range(args.warmup):
-        ev.step()
-    torch.cuda.synchronize()
+    t0 = time.time()
+    opts = ev.batch_plan()
+    tune_s = time.time() - t0
     stream = torch.cuda.current_stream()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record(stream)
-    for _ in range(args.steps):
-        ev.step()
-    stop.record(stream)
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / args.steps
+
+    def time_steps(step):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(args.steps):
+            step()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop) / args.steps
+
+    step_ms = {"batched": time_steps(ev.step_batched), "per-bucket": time_steps(ev.step)}
+    ms = step_ms[args.path]
     value = n_int / (ms * 1e-3) / 1e9
 
     # per-kernel durations (HIP events on the launch stream), separate pass
-    kern = {}
-    for name, fn in [("moments", ev.build_moments), ("zero", ev.zero), ("pc", ev.run_pc),
-                     ("pp", ev.run_pp), ("ewald", ev.run_ewald)]:
+    def time_fn(fn):
         evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
         for i in range(args.steps):
             evs[2 * i].record(stream)
             fn()
             evs[2 * i + 1].record(stream)
         torch.cuda.synchronize()
-        kern[name] = float(np.mean([evs[2 * i].elapsed_time(evs[2 * i + 1])
-                                    for i in range(args.steps)]))
+        return float(np.mean([evs[2 * i].elapsed_time(evs[2 * i + 1]) for i in range(args.steps)]))
+
+    kern = {"moments": time_fn(ev.build_moments), "zero": time_fn(ev.zero),
+            "pc": time_fn(ev.run_pc), "pp": time_fn(ev.run_pp),
+            "batched_graph": time_fn(ev.run_batched)}
+    if w["periodic"]:
+        kern["ewald"] = time_fn(ev.run_ewald)
     pc_flop = FLOP["pc"] * counts["pc"] + FLOP["soft"] * counts["soft"]
-    pc_achieved = pc_flop / (kern["pc"] * 1e-3) / 1e12
+    if args.path == "batched":
+        # the graph holds the zeroing kernel, the queue reset and batch_kernel
+        dom, dom_ms = "batch_kernel", kern["batched_graph"] - kern["zero"]
+        dom_flop = pc_flop + FLOP["pp"] * counts["pp"]
+    else:
+        dom, dom_ms, dom_flop = "pc_kernel", kern["pc"], pc_flop
+    achieved = dom_flop / (dom_ms * 1e-3) / 1e12
 
     out = {
         "metric": "G interactions/s (PC+PP+Ewald) & % FP32 VALU peak, 1 GPU, at <=1e-4 force err",
@@ -100,18 +140,23 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded Zel'dovich lattice, numpy PCG64 seed 1002)",
+        "data": data,
         "config": {
-            "workload": f"periodic Zel'dovich box {args.ng}^3 ({ev.pos.shape[0]} particles), "
-                        "P(k)~k^-2.5, rms 0.02, Ewald nReplicas=1 fEwCut=2.6 |h|^2<=10, "
-                        "theta=0.7, bucket 16, hexadecapole",
+            "workload": workload,
             "form": args.form,
-            "step": "GPU multipole build + zero + PC + PP + Ewald on resident lists",
+            "path": args.path,
+            "step": "GPU multipole build + zero + PC + PP" + (" + Ewald" if w["periodic"] else "")
+                    + " on resident lists",
             "interactions_per_step": counts,
             "n_buckets": int(ev.tree["n_buckets"]),
             "n_nodes": int(ev.tree["n_nodes"]),
             "host_setup_s": round(setup_s, 2),
         },
+        "step_ms": {k: round(v, 4) for k, v in step_ms.items()},
+        "batched_options": {"pcTile": opts.pcTile, "ppTile": opts.ppTile,
+                            "listsPerGrab": opts.listsPerGrab,
+                            "order": {1: "tree", 2: "costliest first"}[opts.order],
+                            "tuned_ms": round(opts.tunedMs, 4), "tune_s": round(tune_s, 2)},
         "fp32_valu_frac": round(flops / (ms * 1e-3) / PEAK_FP32, 4),
         "algorithmic_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
@@ -119,43 +164,46 @@ def main():
             "bound": "mfma",
             "pipe": "FP32 VALU (no matrix work; packed FP32 disabled, so the non-packed "
                     "ceiling is 78.6 TFLOP/s)",
-            "kernel": "pc_kernel",
-            "achieved": round(pc_achieved, 3),
+            "kernel": dom,
+            "achieved": round(achieved, 3),
             "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s",
-            "frac": round(pc_achieved * 1e12 / PEAK_FP32, 4),
-            "traffic": traffic_from_profile(args.ng),
+            "frac": round(achieved * 1e12 / PEAK_FP32, 4),
+            "traffic": traffic_from_profile(tag, dom),
         },
     }
 
     if not args.no_accuracy:
-        out["accuracy"] = accuracy(ev, args.samples)
+        out["accuracy"] = accuracy(ev, ev.step_batched if args.path == "batched" else ev.step,
+                                   args.samples)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ev, args.cpu_frac)
+        frac = args.cpu_frac if args.cpu_frac is not None else {2: 1.0, 3: 0.25}[args.config]
+        out["cpu_baseline"] = cpu_baseline(ev, frac)
     print(json.dumps(out))
 
 
-def traffic_from_profile(ng):
-    """HBM bytes per pc_kernel launch, from the committed rocprofv3 --pmc
+def traffic_from_profile(tag, kernel):
+    """HBM bytes per launch of `kernel`, from the committed rocprofv3 --pmc
     summary of this workload (FETCH_SIZE and WRITE_SIZE, copy-calibrated)."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_zeldovich{ng}.md")
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("pc_kernel traffic_bytes_per_launch:"):
+        if line.startswith(f"{kernel} traffic_bytes_per_launch:"):
             return int(line.split(":")[1])
     return None
 
 
-def accuracy(ev, n_samples):
+def accuracy(ev, step, n_samples):
     """Force error of this GPU step against (a) the FP64 oracle on the same
-    lists and (b) an FP64 converged Ewald sum, on seeded samples."""
+    lists and (b) FP64 exact forces (a converged Ewald sum for the periodic
+    box, direct summation otherwise), on seeded samples."""
     import torch
 
     from oracle import oracle as O
     from tests.helpers import oracle_forces, rel_err
 
-    ev.step()
+    step()
     torch.cuda.synchronize()
     acc, pot, _ = ev.results()
     rng = np.random.default_rng(7)
@@ -167,12 +215,17 @@ def accuracy(ev, n_samples):
     e_or = rel_err(acc[idx], oa[idx])
     samp = np.sort(rng.choice(ev.pos.shape[0], size=n_samples, replace=False)).astype(np.int32)
     t = time.time()
-    xa, xp = O.ewald_exact(ev.pos, ev.mass, ev.soft, samp, L=ev.L, rcut=0.45 * ev.L)
+    if ev.w["periodic"]:
+        xa, xp = O.ewald_exact(ev.pos, ev.mass, ev.soft, samp, L=ev.L, rcut=0.45 * ev.L)
+        exact = "vs_fp64_ewald"
+    else:
+        xa, xp = O.direct_sum(ev.pos, ev.mass, ev.soft, samp)
+        exact = "vs_fp64_direct"
     e_ex = rel_err(acc[samp], xa)
     return {
         "vs_oracle_same_lists": {"particles": int(len(idx)), "median": float(np.median(e_or)),
                                  "p99": float(np.percentile(e_or, 99)), "max": float(e_or.max())},
-        "vs_fp64_ewald": {"samples": int(n_samples), "median": float(np.median(e_ex)),
+        exact: {"samples": int(n_samples), "median": float(np.median(e_ex)),
                           "p99": float(np.percentile(e_ex, 99)), "max": float(e_ex.max()),
                           "oracle_s": round(time.time() - t, 1)},
     }
@@ -195,9 +248,11 @@ def cpu_baseline(ev, frac):
                                 period=ev.L, n_threads=threads, buckets=bsel)
     targets = np.concatenate([np.arange(tr["bucket_start"][b], tr["bucket_start"][b] + tr["bucket_size"][b])
                               for b in bsel])
-    ewt = O.ewald_table(mom[0], geo[0, 3], ev.L, ev.w["ewh_cut"])
-    _, _, ce = O.bucket_ewald(targets, ev.pos, geo[0, 0:3], geo[0, 5], mom[0], geo[0, 3], ewt,
-                              ev.L, ev.w["n_replicas"], ev.w["ew_cut"], n_threads=threads)
+    ce = np.zeros(1, dtype=np.int64)
+    if ev.w["periodic"]:
+        ewt = O.ewald_table(mom[0], geo[0, 3], ev.L, ev.w["ewh_cut"])
+        _, _, ce = O.bucket_ewald(targets, ev.pos, geo[0, 0:3], geo[0, 5], mom[0], geo[0, 3], ewt,
+                                  ev.L, ev.w["n_replicas"], ev.w["ew_cut"], n_threads=threads)
     dt = time.time() - t
     n = int(c.sum() + ce.sum())
     return {"value": round(n / dt / 1e9, 4), "unit": "G interactions/s", "cores": threads,

@@ -1,0 +1,434 @@
+// Batched form: every bucket's particle-cell and particle-particle lists in
+// one persistent launch, replayed as a hipGraph.
+//
+// The per-bucket kernels (gravity.hip) follow ChaNGa's one-launch-per-list
+// pattern (nodeGravityComputation / particleGravityComputation,
+// HostCUDA.cu:1008-1205, 1565-1751). Here one wavefront takes a bucket from a
+// work queue, runs its cell list and then its particle list into the same
+// FP64 accumulators (the chunk loops of gravity_body.h), and updates the
+// bucket's VariablePartData once.
+//
+// Work queues. Buckets are cut into 8 contiguous ranges of equal estimated
+// cost, one per XCD, so that the waves of one XCD (workgroup id mod 8) share
+// source cells in their L2. A wave claims `grab` buckets of its XCD's range
+// per atomic; when the range is exhausted it moves on to the next XCD's, so
+// no XCD idles while another has work. Within a range buckets run either in
+// tree order or longest list first.
+//
+// Plan. changaBatchPlanCreate estimates each bucket's cost on the GPU, builds
+// the queues, times the candidate configurations (cell tile, particle tile,
+// buckets per claim, order) on the caller's data with a scratch result array,
+// and captures the chosen one (queue reset, optional zeroing, persistent
+// kernel) into a graph. Every node is a kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "gravity_body.h"
+
+struct ChangaBatchPlan;
+
+namespace cgpu {
+
+constexpr int kQueueHdr = 16;       // ints before the bucket ids: range starts [0..8]
+constexpr int kCounterStride = 32;  // queue counters 128 B apart
+constexpr int kMaxGrab = 64;
+
+struct BatchArgs {
+  ChangaGravityLaunch pc, pp;
+  const int* queue;  // [0..kNumXcd]: range starts, then bucket ids from kQueueHdr
+  int* counters;     // kNumXcd counters, kCounterStride ints apart
+  int grab;          // buckets claimed per atomic
+};
+
+template <int PCT, int PPT>
+struct BatchLds {
+  static constexpr size_t pc = (size_t)PCT * kPcRec * sizeof(float);
+  static constexpr size_t pp = sizeof(PpLds<PPT>);
+  static constexpr size_t red = 4 * kWave * sizeof(double) + kWave * sizeof(float);
+  static constexpr size_t bytes = pc > pp ? (pc > red ? pc : red) : (pp > red ? pp : red);
+  static constexpr size_t f4 = (bytes + 15) / 16;
+};
+
+// One bucket: cells, then particles, then one update of its results. Every
+// phase that uses LDS ends with a barrier, so the phases share one buffer.
+template <bool PRECISE, int KIND, int PCT, int PPT>
+__device__ __forceinline__ void batch_bucket(const BatchArgs& a, int b, int lane, float4* lds) {
+  const int pcs = a.pc.marks[b], pce = a.pc.marks[b + 1];
+  const int pps = a.pp.marks[b], ppe = a.pp.marks[b + 1];
+  const int bstart = a.pc.starts[b], n = a.pc.sizes[b];
+  double cx0, cy0, cz0;
+  bucket_centre<PRECISE>(a.pc.tpos64, bstart, n, cx0, cy0, cz0);
+  double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(lds);
+  float* redi = reinterpret_cast<float*>(reinterpret_cast<double*>(lds) + 4 * kWave);
+  PpLds<PPT>& s = *reinterpret_cast<PpLds<PPT>*>(lds);
+  for (int t0 = 0; t0 < n; t0 += kWave) {
+    const Lanes l = lanes_for(lane, min(kWave, n - t0));
+    const Target t = load_target<PRECISE>(a.pc, bstart + t0 + (l.live ? l.lane - l.slot * l.nt : 0),
+                                          cx0, cy0, cz0);
+    Acc64 acc = {0.0, 0.0, 0.0, 0.0, 0.0f};
+    pc_cells<PRECISE, PCT>(a.pc, pcs, pce, cx0, cy0, cz0, t, l, lds, acc);
+    pp_sources<PRECISE, KIND, PPT>(a.pp, pps, ppe, cx0, cy0, cz0, t, l, s, acc);
+    reduce_store(red, redi, lane, l.nt, l.nslot, l.slot, acc.ax, acc.ay, acc.az, acc.pot, acc.idt,
+                 a.pc.vars + bstart + t0 + (lane < l.nt ? lane : 0));
+  }
+}
+
+// Persistent kernel: one wavefront per workgroup; runs until every queue is
+// exhausted. Each wave visits the 8 queues once, starting at its own XCD's,
+// and stays on a queue until a claim comes back past its end, so every wave
+// exits after finitely many claims and no wave waits on another.
+template <bool PRECISE, int KIND, int PCT, int PPT>
+__global__ __launch_bounds__(64) void batch_kernel(BatchArgs a) {
+  __shared__ float4 lds[BatchLds<PCT, PPT>::f4];
+  const int lane = threadIdx.x;
+  const int* qbeg = a.queue;
+  const int* ids = a.queue + kQueueHdr;
+  int q = blockIdx.x % kNumXcd;
+  for (int tried = 0; tried < kNumXcd;) {
+    int k = 0;
+    if (lane == 0) k = atomicAdd(a.counters + q * kCounterStride, a.grab);
+    k = __builtin_amdgcn_readfirstlane(k);
+    const int q0 = qbeg[q], qn = qbeg[q + 1] - q0;
+    if (k >= qn) {
+      ++tried;
+      q = q + 1 == kNumXcd ? 0 : q + 1;
+      continue;
+    }
+    const int kend = min(k + a.grab, qn);
+    for (; k < kend; ++k) batch_bucket<PRECISE, KIND, PCT, PPT>(a, ids[q0 + k], lane, lds);
+  }
+}
+
+__global__ void batch_reset_kernel(int* counters) {
+  const int i = threadIdx.x;
+  if (i < kNumXcd * kCounterStride) counters[i] = 0;
+}
+
+// Estimated wavefront time of each bucket (VALU instructions per slot
+// iteration: ~130 for a cell, ~25 for a particle, plus per-chunk overhead).
+template <int KIND>
+__global__ void bucket_cost_kernel(ChangaGravityLaunch pc, ChangaGravityLaunch pp, float* cost) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= pc.numBuckets) return;
+  const int nc = pc.marks[b + 1] - pc.marks[b];
+  float np = 0.0f;
+  if (KIND == 0) {
+    np = (float)(pp.marks[b + 1] - pp.marks[b]);
+  } else {
+    const ILPart* r = static_cast<const ILPart*>(pp.list);
+    for (int i = pp.marks[b]; i < pp.marks[b + 1]; ++i) np += (float)r[i].num;
+  }
+  const int n = pc.sizes[b];
+  float c = 0.0f;
+  for (int t0 = 0; t0 < n; t0 += kWave) {
+    const int nt = min(kWave, n - t0);
+    c += (130.0f * nc + 25.0f * np) / (float)(kWave / nt) + 400.0f;
+  }
+  cost[b] = c;
+}
+
+using BatchFn = void (*)(BatchArgs);
+
+template <bool PR, int KIND>
+static BatchFn pick_tiles(int pct, int ppt) {
+  if (pct == 32) return ppt == 128 ? batch_kernel<PR, KIND, 32, 128> : batch_kernel<PR, KIND, 32, 256>;
+  return ppt == 128 ? batch_kernel<PR, KIND, 64, 128> : batch_kernel<PR, KIND, 64, 256>;
+}
+
+static BatchFn pick(bool pr, int kind, int pct, int ppt) {
+  if (pr) return kind ? pick_tiles<true, 1>(pct, ppt) : pick_tiles<true, 0>(pct, ppt);
+  return kind ? pick_tiles<false, 1>(pct, ppt) : pick_tiles<false, 0>(pct, ppt);
+}
+
+}  // namespace cgpu
+
+using namespace cgpu;
+
+struct ChangaBatchPlan {
+  BatchArgs args;
+  bool precise;
+  int kind;
+  ChangaBatchOptions opt;
+  int grid;
+  int nParticles;
+  int* d_queue;
+  int* d_counters;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+namespace {
+
+// Resident workgroups of the persistent kernel on the whole device; -1 on error.
+int grid_for(BatchFn fn) {
+  int dev = 0, ncu = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)fn, kWave, 0) != hipSuccess)
+    return -1;
+  return ncu * std::max(per, 1);
+}
+
+// Enqueue one batched evaluation with the plan's chosen configuration.
+int enqueue(const ChangaBatchPlan* p, const BatchArgs& args, const ChangaBatchOptions& o, int grid,
+            hipStream_t s) {
+  if (p->args.pc.numBuckets == 0) return 0;
+  if (o.zeroVars) {
+    if (int e = changaZeroVars(args.pc.vars, p->nParticles, s)) return e;
+  }
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kNumXcd * kCounterStride), 0, s,
+                     args.counters);
+  CGPU_CHECK(hipGetLastError());
+  BatchFn fn = pick(p->precise, p->kind, o.pcTile, o.ppTile);
+  BatchArgs a = args;
+  a.grab = o.listsPerGrab;
+  void* kargs[] = {&a};
+  CGPU_CHECK(hipLaunchKernel((const void*)fn, dim3(grid), dim3(kWave), kargs, 0, s));
+  return 0;
+}
+
+bool same_buckets(const ChangaGravityLaunch& a, const ChangaGravityLaunch& b) {
+  return a.targets == b.targets && a.vars == b.vars && a.starts == b.starts &&
+         a.sizes == b.sizes && a.numBuckets == b.numBuckets && a.fperiod == b.fperiod &&
+         a.tpos64 == b.tpos64;
+}
+
+// Queue layout for one order: range starts, then bucket ids.
+std::vector<int> make_queue(const std::vector<float>& cost, bool longest_first) {
+  const int nb = (int)cost.size();
+  std::vector<int> q(kQueueHdr + nb, 0);
+  double total = 0.0;
+  for (float c : cost) total += c;
+  int x = 1;
+  double run = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    while (x < kNumXcd && run >= total * x / kNumXcd) q[x++] = b;
+    run += cost[b];
+  }
+  while (x < kNumXcd) q[x++] = nb;
+  q[kNumXcd] = nb;
+  int* ids = q.data() + kQueueHdr;
+  std::iota(ids, ids + nb, 0);
+  if (longest_first)
+    for (int r = 0; r < kNumXcd; ++r)
+      std::stable_sort(ids + q[r], ids + q[r + 1],
+                       [&](int u, int v) { return cost[u] > cost[v]; });
+  return q;
+}
+
+void release(ChangaBatchPlan* p) {
+  if (!p) return;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->d_queue) (void)hipFree(p->d_queue);
+  if (p->d_counters) (void)hipFree(p->d_counters);
+  delete p;
+}
+
+}  // namespace
+
+extern "C" int changaBatchPlanCreate(const ChangaGravityLaunch* cells,
+                                     const ChangaGravityLaunch* parts, ChangaBatchOptions* opts,
+                                     ChangaBatchPlan** plan, void* stream) {
+  if (!cells || !parts || !opts || !plan)
+    return fail("changaBatchPlanCreate: missing argument", hipErrorInvalidValue, __FILE__,
+                __LINE__);
+  *plan = nullptr;
+  const ChangaGravityLaunch& pc = *cells;
+  const ChangaGravityLaunch& pp = *parts;
+  if (pc.numBuckets < 0 || !pc.marks || !pc.starts || !pc.sizes || !pc.vars || !pc.targets ||
+      !pc.moments || !pc.list || !pp.marks || !pp.list || !pp.sources)
+    return fail("changaBatchPlanCreate: missing array", hipErrorInvalidValue, __FILE__, __LINE__);
+  if (!same_buckets(pc, pp))
+    return fail("changaBatchPlanCreate: cell and particle launches differ in their buckets",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  if (pc.tpos64 && (!pc.cm64 || !pp.spos64))
+    return fail("changaBatchPlanCreate: precision form needs cm64 and spos64",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  if (pp.listKind != 0 && pp.listKind != 1)
+    return fail("changaBatchPlanCreate: listKind", hipErrorInvalidValue, __FILE__, __LINE__);
+  ChangaBatchOptions o = *opts;
+  if ((o.pcTile != 0 && o.pcTile != 32 && o.pcTile != 64) ||
+      (o.ppTile != 0 && o.ppTile != 128 && o.ppTile != 256) || o.listsPerGrab < 0 ||
+      o.listsPerGrab > kMaxGrab || o.order < 0 || o.order > 2)
+    return fail("changaBatchPlanCreate: option out of range", hipErrorInvalidValue, __FILE__,
+                __LINE__);
+  hipStream_t s = (hipStream_t)stream;
+  ChangaBatchPlan* p = new ChangaBatchPlan();
+  std::memset(p, 0, sizeof(*p));
+  p->precise = pc.tpos64 != nullptr;
+  p->kind = pp.listKind;
+  p->args.pc = pc;
+  p->args.pp = pp;
+  const int nb = pc.numBuckets;
+
+  auto bail = [&](int e) {
+    release(p);
+    return e;
+  };
+  // bucket extents -> particle count; per-bucket cost
+  std::vector<int> starts(nb), sizes(nb);
+  std::vector<float> cost(nb);
+  float* d_cost = nullptr;
+  if (nb > 0) {
+    hipError_t e = hipMalloc(&d_cost, sizeof(float) * nb);
+    if (e != hipSuccess) return bail(fail("hipMalloc cost", e, __FILE__, __LINE__));
+    if (p->kind)
+      hipLaunchKernelGGL(bucket_cost_kernel<1>, dim3((nb + 255) / 256), dim3(256), 0, s, pc, pp,
+                         d_cost);
+    else
+      hipLaunchKernelGGL(bucket_cost_kernel<0>, dim3((nb + 255) / 256), dim3(256), 0, s, pc, pp,
+                         d_cost);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(cost.data(), d_cost, sizeof(float) * nb,
+                                            hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(starts.data(), pc.starts, sizeof(int) * nb,
+                                            hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(sizes.data(), pc.sizes, sizeof(int) * nb,
+                                            hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d_cost);
+    if (e != hipSuccess) return bail(fail("batch plan: cost estimate", e, __FILE__, __LINE__));
+  }
+  for (int b = 0; b < nb; ++b) p->nParticles = std::max(p->nParticles, starts[b] + sizes[b]);
+
+  // queues for both orders (index 0: tree order, 1: longest first)
+  const size_t qbytes = sizeof(int) * (kQueueHdr + (size_t)nb);
+  int* d_q[2] = {nullptr, nullptr};
+  for (int k = 0; k < 2; ++k) {
+    std::vector<int> q = make_queue(cost, k == 1);
+    hipError_t e = hipMalloc(&d_q[k], qbytes);
+    if (e == hipSuccess) e = hipMemcpy(d_q[k], q.data(), qbytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      for (int j = 0; j < 2; ++j)
+        if (d_q[j]) (void)hipFree(d_q[j]);
+      return bail(fail("batch plan: queue upload", e, __FILE__, __LINE__));
+    }
+  }
+  {
+    hipError_t e = hipMalloc(&p->d_counters, sizeof(int) * kNumXcd * kCounterStride);
+    if (e != hipSuccess) {
+      (void)hipFree(d_q[0]);
+      (void)hipFree(d_q[1]);
+      return bail(fail("hipMalloc counters", e, __FILE__, __LINE__));
+    }
+  }
+  p->args.counters = p->d_counters;
+
+  // autotune whatever the caller left at 0, on a scratch result array
+  const bool tune = o.pcTile == 0 || o.ppTile == 0 || o.listsPerGrab == 0 || o.order == 0;
+  o.tunedMs = 0.0f;
+  if (tune && nb > 0) {
+    VariablePartData* scratch = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&scratch, sizeof(VariablePartData) * (size_t)std::max(p->nParticles, 1));
+    if (e == hipSuccess)
+      e = hipMemsetAsync(scratch, 0, sizeof(VariablePartData) * (size_t)std::max(p->nParticles, 1), s);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float best = -1.0f;
+    ChangaBatchOptions bo = o;
+    const int pcts[2] = {32, 64}, ppts[2] = {128, 256}, grabs[2] = {1, 4}, orders[2] = {1, 2};
+    for (int i0 = 0; i0 < 2 && e == hipSuccess; ++i0)
+      for (int i1 = 0; i1 < 2 && e == hipSuccess; ++i1)
+        for (int i2 = 0; i2 < 2 && e == hipSuccess; ++i2)
+          for (int i3 = 0; i3 < 2 && e == hipSuccess; ++i3) {
+            ChangaBatchOptions c = o;
+            c.pcTile = o.pcTile ? o.pcTile : pcts[i0];
+            c.ppTile = o.ppTile ? o.ppTile : ppts[i1];
+            c.listsPerGrab = o.listsPerGrab ? o.listsPerGrab : grabs[i2];
+            c.order = o.order ? o.order : orders[i3];
+            c.zeroVars = 0;
+            // skip duplicates of a fixed option
+            if ((o.pcTile && i0) || (o.ppTile && i1) || (o.listsPerGrab && i2) || (o.order && i3))
+              continue;
+            BatchArgs a = p->args;
+            a.pc.vars = scratch;
+            a.pp.vars = scratch;
+            a.queue = d_q[c.order - 1];
+            const int grid = grid_for(pick(p->precise, p->kind, c.pcTile, c.ppTile));
+            if (grid <= 0) {
+              e = hipErrorInvalidValue;
+              break;
+            }
+            if (enqueue(p, a, c, grid, s)) {
+              e = hipErrorLaunchFailure;
+              break;
+            }
+            e = hipEventRecord(e0, s);
+            for (int r = 0; r < 2 && e == hipSuccess; ++r)
+              if (enqueue(p, a, c, grid, s)) e = hipErrorLaunchFailure;
+            if (e == hipSuccess) e = hipEventRecord(e1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            ms *= 0.5f;
+            if (e == hipSuccess && (best < 0.0f || ms < best)) {
+              best = ms;
+              bo = c;
+            }
+          }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (scratch) (void)hipFree(scratch);
+    if (e != hipSuccess) {
+      (void)hipFree(d_q[0]);
+      (void)hipFree(d_q[1]);
+      return bail(fail("batch plan: autotune", e, __FILE__, __LINE__));
+    }
+    bo.zeroVars = o.zeroVars;
+    bo.tunedMs = best;
+    o = bo;
+  } else {
+    if (!o.pcTile) o.pcTile = 64;
+    if (!o.ppTile) o.ppTile = 256;
+    if (!o.listsPerGrab) o.listsPerGrab = 1;
+    if (!o.order) o.order = 1;
+  }
+  p->opt = o;
+  p->d_queue = d_q[o.order - 1];
+  (void)hipFree(d_q[2 - o.order]);
+  p->args.queue = p->d_queue;
+  p->args.grab = o.listsPerGrab;
+  if (nb > 0) {
+    p->grid = grid_for(pick(p->precise, p->kind, o.pcTile, o.ppTile));
+    if (p->grid <= 0) return bail(fail("batch plan: occupancy", hipErrorInvalidValue, __FILE__,
+                                       __LINE__));
+    // capture reset (+ zeroing) + persistent kernel; every node is a kernel
+    hipStream_t cs = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      int le = enqueue(p, p->args, p->opt, p->grid, cs);
+      hipError_t ee = hipStreamEndCapture(cs, &p->graph);
+      e = le ? hipErrorLaunchFailure : ee;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) return bail(fail("batch plan: graph capture", e, __FILE__, __LINE__));
+  }
+  *opts = o;
+  *plan = p;
+  return 0;
+}
+
+extern "C" int changaBatchPlanLaunch(ChangaBatchPlan* plan, void* stream) {
+  if (!plan) return fail("changaBatchPlanLaunch: no plan", hipErrorInvalidValue, __FILE__, __LINE__);
+  return enqueue(plan, plan->args, plan->opt, plan->grid, (hipStream_t)stream);
+}
+
+extern "C" int changaBatchPlanRun(ChangaBatchPlan* plan, void* stream) {
+  if (!plan) return fail("changaBatchPlanRun: no plan", hipErrorInvalidValue, __FILE__, __LINE__);
+  if (!plan->exec) return 0;
+  CGPU_CHECK(hipGraphLaunch(plan->exec, (hipStream_t)stream));
+  return 0;
+}
+
+extern "C" int changaBatchPlanDestroy(ChangaBatchPlan* plan) {
+  release(plan);
+  return 0;
+}

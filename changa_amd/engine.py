@@ -247,6 +247,50 @@ class Evaluation:
         self.run_pp()
         self.run_ewald()
 
+    # ------------------------------------------------------------------ batched form
+    def batch_plan(self, pc_tile=0, pp_tile=0, lists_per_grab=0, order=0, zero_vars=True):
+        """Make (replacing any previous) the batched-form plan over this
+        evaluation's lists; options left at 0 are autotuned. Returns the
+        chosen ChangaBatchOptions."""
+        self.free_batch()
+        o = gpu.ChangaBatchOptions(pc_tile, pp_tile, lists_per_grab, order, int(zero_vars), 0.0)
+        self._batch_launches = (self._launch(True), self._launch(False))
+        plan = ctypes.c_void_p()
+        gpu.check(gpu.lib().changaBatchPlanCreate(ctypes.byref(self._batch_launches[0]),
+                                                  ctypes.byref(self._batch_launches[1]),
+                                                  ctypes.byref(o), ctypes.byref(plan), self._sp()),
+                  "changaBatchPlanCreate")
+        self._batch_plan = plan
+        self.batch_options = o
+        return o
+
+    def run_batched(self, graph=True):
+        """PC + PP of every bucket in one persistent launch (graph replay, or
+        the same kernels enqueued directly)."""
+        fn = gpu.lib().changaBatchPlanRun if graph else gpu.lib().changaBatchPlanLaunch
+        gpu.check(fn(self._batch_plan, self._sp()), "changaBatchPlan")
+
+    def step_batched(self, moments=True):
+        """One force evaluation in the batched form: multipole build, the
+        plan's graph (zeroing, PC + PP), Ewald."""
+        if moments:
+            self.build_moments()
+        self.run_batched()
+        self.run_ewald()
+
+    def free_batch(self):
+        plan = getattr(self, "_batch_plan", None)
+        if plan is not None and plan.value:
+            torch.cuda.synchronize(self.device)
+            gpu.lib().changaBatchPlanDestroy(plan)
+        self._batch_plan = None
+
+    def __del__(self):
+        try:
+            self.free_batch()
+        except Exception:
+            pass
+
     def results(self):
         """(acc[N,3], pot[N], dtgrav[N]) in tree order, FP32 as the wire."""
         v = self.d_vars.cpu().numpy()

@@ -69,6 +69,11 @@ class ChangaGravityLaunch(ctypes.Structure):
                 ("tpos64", c_void_p), ("spos64", c_void_p), ("cm64", c_void_p)]
 
 
+class ChangaBatchOptions(ctypes.Structure):
+    _fields_ = [("pcTile", c_int), ("ppTile", c_int), ("listsPerGrab", c_int), ("order", c_int),
+                ("zeroVars", c_int), ("tunedMs", c_float)]
+
+
 CALLBACK = ctypes.CFUNCTYPE(None, c_void_p)
 
 # every extern "C" symbol include/changa_gpu.h declares
@@ -82,6 +87,7 @@ EXPORTS = [
     "EwaldHost", "changaGpuSetCallbackHandler", "changaGpuSetAbortOnError", "changaGpuLastError",
     "changaGpuReleasePools", "TreePiecePartListDataTransferLocalCompact", "changaNodeGravity",
     "changaPartGravity", "changaZeroVars", "changaEwald", "changaBuildMoments",
+    "changaBatchPlanCreate", "changaBatchPlanLaunch", "changaBatchPlanRun", "changaBatchPlanDestroy",
 ]
 
 _lib = None
@@ -127,6 +133,16 @@ def lib():
     L.changaEwald.restype = c_int
     L.changaBuildMoments.argtypes = [c_int, c_int, P, P, P, P, P, P, P, P, P, P, P, P]
     L.changaBuildMoments.restype = c_int
+    L.changaBatchPlanCreate.argtypes = [ctypes.POINTER(ChangaGravityLaunch),
+                                        ctypes.POINTER(ChangaGravityLaunch),
+                                        ctypes.POINTER(ChangaBatchOptions),
+                                        ctypes.POINTER(c_void_p), P]
+    L.changaBatchPlanCreate.restype = c_int
+    for name in ["changaBatchPlanLaunch", "changaBatchPlanRun"]:
+        getattr(L, name).argtypes = [P, P]
+        getattr(L, name).restype = c_int
+    L.changaBatchPlanDestroy.argtypes = [P]
+    L.changaBatchPlanDestroy.restype = c_int
     _lib = L
     return L
 

@@ -164,6 +164,41 @@ int changaBuildMoments(int nNodes, int nDepths, const int *levelOffsets /* host 
                        const double *soft, CudaMultipoleMoments *moments, double *node64,
                        double *cm64, void *stream);
 
+/* ======================================================================
+ * Part 3: batched form
+ *
+ * Every bucket's cell list and particle list in one persistent launch: a
+ * wavefront claims buckets from per-XCD work queues and evaluates the cell
+ * list, then the particle list, of each into one set of accumulators (one
+ * += per particle and bucket instead of one per list). The plan is bound to
+ * the buffers of the two launch descriptions it was made from; make a new
+ * plan when they change. The graph it holds contains kernels only.
+ * ====================================================================== */
+typedef struct ChangaBatchPlan ChangaBatchPlan;
+
+typedef struct {
+  int pcTile;        /* cells per LDS tile: 32 or 64; 0 = autotune */
+  int ppTile;        /* source particles per LDS tile: 128 or 256; 0 = autotune */
+  int listsPerGrab;  /* bucket lists a wavefront claims per queue access, 1..64; 0 = autotune */
+  int order;         /* 1 tree order, 2 costliest bucket first (per XCD range); 0 = autotune */
+  int zeroVars;      /* 1: each run first zeroes vars (particles covered by the buckets) */
+  float tunedMs;     /* out: time of one evaluation with the chosen options, or 0 */
+} ChangaBatchOptions;
+
+/* cells: the particle-cell launch (listKind 0); parts: the particle-particle
+   launch (listKind 0 or 1) over the same buckets, targets, vars and
+   precision-form positions. Options left at 0 are chosen by timing the
+   candidates on these inputs (results go to a scratch array; vars is not
+   touched); *opts returns the choice. Synchronizes the stream. */
+int changaBatchPlanCreate(const ChangaGravityLaunch *cells, const ChangaGravityLaunch *parts,
+                          ChangaBatchOptions *opts, ChangaBatchPlan **plan, void *stream);
+/* One evaluation, enqueued as kernels on `stream` (usable inside the
+   caller's own stream capture). */
+int changaBatchPlanLaunch(ChangaBatchPlan *plan, void *stream);
+/* One evaluation, replaying the plan's graph on `stream`. */
+int changaBatchPlanRun(ChangaBatchPlan *plan, void *stream);
+int changaBatchPlanDestroy(ChangaBatchPlan *plan);
+
 #ifdef __cplusplus
 }
 #endif

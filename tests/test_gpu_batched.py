@@ -1,0 +1,162 @@
+"""Batched form (changaBatchPlan*, include/changa_gpu.h Part 3) on the GPU.
+
+The persistent kernel evaluates each bucket's cell list and particle list into
+one set of FP64 accumulators; the per-bucket path rounds the two partial sums
+to FP32 separately. Bars:
+  - against the per-bucket path: every component within 1e-6 of that
+    component's largest magnitude (one FP32 rounding of each partial sum);
+    dtGrav, a max over the same FP32 terms, equal bit for bit;
+  - against the FP64 oracle: the per-bucket path's parity bars;
+  - graph replays: each equal bit for bit to the directly enqueued kernels
+    (every bucket's arithmetic is fixed; only which wave runs it varies).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from changa_amd import configs, engine, gpu
+from tests.helpers import oracle_forces, rel_err, summary
+
+pytestmark = pytest.mark.gpu
+
+
+def _plain(ev):
+    import torch
+    ev.zero()
+    ev.run_pc()
+    ev.run_pp()
+    torch.cuda.synchronize()
+    return ev.d_vars.cpu().numpy().copy()
+
+
+def _batched(ev, graph=True):
+    import torch
+    ev.run_batched(graph=graph)
+    torch.cuda.synchronize()
+    return ev.d_vars.cpu().numpy().copy()
+
+
+def _close(v, ref, tol=1e-6):
+    scale = np.abs(ref[:, :4]).max(axis=0)
+    d = np.abs(v[:, :4].astype(np.float64) - ref[:, :4]) / scale
+    assert d.max() <= tol, d.max(axis=0)
+    np.testing.assert_array_equal(v[:, 4], ref[:, 4])
+
+
+@pytest.mark.parametrize("precise,compact", [(True, True), (False, True), (True, False)])
+def test_gpu_batched_matches_per_bucket(device, precise, compact):
+    ev = engine.Evaluation(configs.plummer(n=32768), precise=precise, pp_compact=compact,
+                           device=device)
+    ref = _plain(ev)
+    for pct in (32, 64):
+        for ppt in (128, 256):
+            for grab, order in ((1, 1), (4, 2)):
+                o = ev.batch_plan(pct, ppt, grab, order)
+                assert (o.pcTile, o.ppTile, o.listsPerGrab, o.order) == (pct, ppt, grab, order)
+                _close(_batched(ev), ref)
+    ev.free_batch()
+
+
+def test_gpu_batched_graph_replays(device):
+    """Replays of the captured graph, with eager work and synchronizations in
+    between, each equal the directly enqueued kernels."""
+    import torch
+    ev = engine.Evaluation(configs.plummer(n=16384), precise=True, device=device)
+    ev.batch_plan(64, 256, 1, 2)
+    eager = _batched(ev, graph=False)
+    junk = torch.randn(1 << 20, device=device)
+    for _ in range(4):
+        ev.d_vars.fill_(3.0)  # zeroing is part of the graph
+        (junk * 2.0).sum().item()
+        np.testing.assert_array_equal(_batched(ev, graph=True), eager)
+    ev.free_batch()
+
+
+def test_gpu_batched_autotune_leaves_results_alone(device):
+    import torch
+    ev = engine.Evaluation(configs.plummer(n=16384), precise=True, device=device)
+    ref = _plain(ev)
+    ev.d_vars.fill_(7.0)
+    torch.cuda.synchronize()
+    o = ev.batch_plan()  # everything autotuned
+    torch.cuda.synchronize()
+    assert (ev.d_vars.cpu().numpy() == 7.0).all(), "tuning must write only scratch memory"
+    assert o.pcTile in (32, 64) and o.ppTile in (128, 256)
+    assert o.listsPerGrab in (1, 4) and o.order in (1, 2) and o.tunedMs > 0
+    print(f"tuned: pcTile {o.pcTile} ppTile {o.ppTile} grab {o.listsPerGrab} order {o.order} "
+          f"{o.tunedMs:.3f} ms", flush=True)
+    _close(_batched(ev), ref)
+    ev.free_batch()
+
+
+def test_gpu_batched_halos_oracle(device):
+    """Clustered halos, bucket 32, theta 0.6 (the uneven-list workload at
+    65536 particles) in the batched form against the FP64 oracle and the
+    per-bucket path. Near halo centres the net acceleration is a small
+    remainder of large opposing contributions, each exact to FP32 rounding,
+    so the per-particle bar there is the plain form's (median 1e-6, 99th
+    percentile 1e-5, max 1e-4)."""
+    import torch
+    w = configs.halos(n=65536)
+    ev = engine.Evaluation(w, precise=True, device=device)
+    ref = _plain(ev)
+    ev.batch_plan()
+    v = _batched(ev)
+    _close(v, ref)
+    acc, pot, dtg = ev.results()
+    oa, op, od = oracle_forces(ev)
+    for name, e in (("acc", rel_err(acc, oa)), ("pot", rel_err(pot, op)),
+                    ("dtgrav", rel_err(dtg, od))):
+        print(f"halos batched {name} {summary(e)}", flush=True)
+        assert np.median(e) <= 1e-6 and np.percentile(e, 99) <= 1e-5, summary(e)
+        assert e.max() <= 1e-4, summary(e)
+    ev.free_batch()
+    torch.cuda.synchronize()
+
+
+def test_gpu_batched_periodic_step(device):
+    """Periodic box: the batched tree part (cells and particles with replica
+    offsets) equals the per-bucket one; the batched step (graph with zeroing,
+    then Ewald) meets the periodic parity bars. The comparison is made before
+    the Ewald correction: the net potential is a small remainder of the tree
+    and Ewald parts, so one FP32 rounding of each partial sum is ~1e-6 of the
+    partial sums but more of the remainder."""
+    import torch
+    ev = engine.Evaluation(configs.zeldovich(ng=32, rms=0.03), precise=True, device=device)
+    ref = _plain(ev)
+    ev.batch_plan()
+    _close(_batched(ev), ref)
+    ev.step_batched()
+    torch.cuda.synchronize()
+    v = ev.d_vars.cpu().numpy()
+    acc = v[:, :3].astype(np.float64)
+    oa, op, od = oracle_forces(ev)
+    e = rel_err(acc, oa)
+    print(f"periodic batched acc {summary(e)}", flush=True)
+    assert np.median(e) <= 5e-6 and np.percentile(e, 99) <= 5e-5, summary(e)
+    ev.free_batch()
+
+
+def test_gpu_batched_rejects_bad_arguments(device):
+    """Rejected on the host with an error code (no launch happens)."""
+    ev = engine.Evaluation(configs.plummer(n=4096), device=device)
+    L = gpu.lib()
+    L.changaGpuSetAbortOnError(0)
+    try:
+        pc, pp = ev._launch(True), ev._launch(False)
+        plan = ctypes.c_void_p()
+        bad = gpu.ChangaBatchOptions(48, 0, 0, 0, 1, 0.0)
+        assert L.changaBatchPlanCreate(ctypes.byref(pc), ctypes.byref(pp), ctypes.byref(bad),
+                                       ctypes.byref(plan), None) != 0
+        assert not plan.value
+        pp2 = ev._launch(False)
+        pp2.numBuckets = pc.numBuckets - 1
+        ok = gpu.ChangaBatchOptions(0, 0, 0, 0, 1, 0.0)
+        assert L.changaBatchPlanCreate(ctypes.byref(pc), ctypes.byref(pp2), ctypes.byref(ok),
+                                       ctypes.byref(plan), None) != 0
+        msg = ctypes.c_char_p()
+        L.changaGpuLastError(ctypes.byref(msg))
+        assert b"differ" in msg.value
+    finally:
+        L.changaGpuSetAbortOnError(1)
