@@ -494,7 +494,7 @@ void orc_forces_lists(int n_buckets, const int32_t *bucket_node,
                       const int32_t *rung, int active_rung,
                       const int32_t *cells, const int64_t *cell_marks,
                       const int32_t *parts, const int64_t *part_marks,
-                      double period, int n_threads,
+                      double period, int soft_flags, int n_threads,
                       double *acc, double *pot, double *dtgrav, int64_t counts[3]) {
     int64_t n_pc = 0, n_soft = 0, n_pp = 0;
 #ifdef _OPENMP
@@ -510,7 +510,9 @@ void orc_forces_lists(int n_buckets, const int32_t *bucket_node,
             double off[3];
             decode_offset(cells[2 * e + 1], period, off);
             const double *cg = geo + (size_t)cn * 12;
-            if (orc_open_softening(cg, bg, off[0], off[1], off[2])) {
+            int softened = soft_flags ? cells[2 * e + 1] < 0
+                                      : orc_open_softening(cg, bg, off[0], off[1], off[2]);
+            if (softened) {
                 /* monopole pseudo-particle (gravity.h:414-421) */
                 double sp[3] = {cg[0] + off[0], cg[1] + off[1], cg[2] + off[2]};
                 n_soft += pp_source(sp, cg[5], cg[4], first, count, pos, mass, soft, rung,
@@ -877,4 +879,353 @@ void orc_ewald_exact(int n, const double *pos, const double *mass, const double 
     free(hl);
     free(Sre);
     free(Sim);
+}
+
+/* -------------------------------------------------------------------------
+ * Adjoint of the list evaluation (no reference counterpart).
+ *
+ * L = sum_i w_i . a_i, with a_i the tree acceleration of orc_forces_lists on
+ * fixed lists (softened cells as flagged in the lists). The exact gradient
+ * with respect to every particle position and mass:
+ *   - target side: d(w_i . a_i)/dx_i = (Hessian of the potential of each list
+ *     entry at x_i) . w_i;
+ *   - particle sources: pairwise terms;
+ *   - cell sources: a cell's term depends on its particles through M, cm and
+ *     the moments Q_n about cm. With s = x_k - cm, sum_i w_i . a_ic =
+ *     sum_k m_k H_c(s_k), H_c(s) = sum_alpha h_alpha s^alpha, h_alpha =
+ *     (-1)^n / alpha! D_alpha, D_alpha = sum_i w_i . grad d^alpha (1/r)(r_i)
+ *     (n = |alpha| in {0,2,3,4}). Through cm (dcm/dx_k = m_k/M, dcm/dm_k =
+ *     s_k/M) this gives dL/dm_k = Hhat_c(s_k), dL/dx_k = m_k grad Hhat_c(s_k)
+ *     with Hhat_c(s) = H_c(s) - U_c . s, U_c = (sum_k m_k grad H_c(s_k) +
+ *     sum_i t_ic) / M, t_ic the target-side term of pair (i, c).
+ * The polynomials Hhat are summed down the tree (re-expanded exactly about
+ * each child's centre of mass) and evaluated at the particles of the
+ * buckets. Softening lengths are constants (the mass-weighted softening of
+ * a cell is not differentiated; it is constant when its particles share one
+ * softening).
+ * ------------------------------------------------------------------------- */
+#define ORC_TD 7
+typedef double orc_tder[ORC_TD][ORC_TD][ORC_TD];
+
+/* T[a][b][c] = d^(a+b+c)(1/|r|) / dx^a dy^b dz^c for a+b+c <= nmax (<= 6):
+   differentiating r^2 d_x f + x f = 0 (f = 1/r) by Leibniz' rule gives
+   r^2 T(al+e_x) = -(2a+1) x T(al) - 2b y T(al+e_x-e_y) - 2c z T(al+e_x-e_z)
+                   - a^2 T(al-e_x) - b(b-1) T(al+e_x-2e_y) - c(c-1) T(al+e_x-2e_z)
+   for al = (a,b,c), and the same with the axes permuted. */
+static void rinv_derivs(const double r[3], int nmax, orc_tder T) {
+    const double x = r[0], y = r[1], z = r[2];
+    const double ir2 = 1.0 / (x * x + y * y + z * z);
+    T[0][0][0] = sqrt(ir2);
+    for (int n = 0; n < nmax; ++n)
+        for (int a = n + 1; a >= 0; --a)
+            for (int b = n + 1 - a; b >= 0; --b) {
+                const int c = n + 1 - a - b;
+                double v;
+                if (a > 0) { /* raise x from (a-1, b, c) */
+                    const int A = a - 1;
+                    v = -(2 * A + 1) * x * T[A][b][c];
+                    if (b >= 1) v -= 2.0 * b * y * T[a][b - 1][c];
+                    if (c >= 1) v -= 2.0 * c * z * T[a][b][c - 1];
+                    if (A >= 1) v -= (double)A * A * T[A - 1][b][c];
+                    if (b >= 2) v -= (double)b * (b - 1) * T[a][b - 2][c];
+                    if (c >= 2) v -= (double)c * (c - 1) * T[a][b][c - 2];
+                } else if (b > 0) { /* raise y from (0, b-1, c) */
+                    const int B = b - 1;
+                    v = -(2 * B + 1) * y * T[0][B][c];
+                    if (c >= 1) v -= 2.0 * c * z * T[0][b][c - 1];
+                    if (B >= 1) v -= (double)B * B * T[0][B - 1][c];
+                    if (c >= 2) v -= (double)c * (c - 1) * T[0][b][c - 2];
+                } else { /* raise z from (0, 0, c-1) */
+                    const int C = c - 1;
+                    v = -(2 * C + 1) * z * T[0][0][C];
+                    if (C >= 1) v -= (double)C * C * T[0][0][C - 1];
+                }
+                T[a][b][c] = v * ir2;
+            }
+}
+
+void orc_rinv_derivs(const double r[3], int nmax, double *out) {
+    orc_tder T;
+    memset(T, 0, sizeof(T));
+    rinv_derivs(r, nmax, T);
+    memcpy(out, T, sizeof(T));
+}
+
+/* -(1/r) db/dr of the spline force kernel b (orc_spline): the pairwise
+   Jacobian is d(d b)/dd = b I - c d d^T. */
+static double spline_c(double r2, double twoh) {
+    double r = sqrt(r2);
+    if (r < twoh) {
+        double dih = 2.0 / twoh, u = r * dih;
+        double d5 = dih * dih * dih * dih * dih;
+        if (u < 1.0) return d5 * (12.0 / 5.0 - 1.5 * u);
+        return -0.2 / (r2 * r2 * r) + d5 * (3.0 / u - 12.0 / 5.0 + 0.5 * u);
+    }
+    return 3.0 / (r2 * r2 * r);
+}
+
+/* monomials of degree <= 4: index and inverse factorials */
+static int mono_idx[5][5][5];
+static int mono_abc[35][3];
+static int mono_n = 0;
+static void mono_init(void) {
+    if (mono_n) return;
+    int k = 0;
+    for (int n = 0; n <= 4; ++n)
+        for (int a = n; a >= 0; --a)
+            for (int b = n - a; b >= 0; --b) {
+                int c = n - a - b;
+                mono_idx[a][b][c] = k;
+                mono_abc[k][0] = a;
+                mono_abc[k][1] = b;
+                mono_abc[k][2] = c;
+                ++k;
+            }
+    mono_n = k;
+}
+static double inv_fact3(int a, int b, int c) { return 1.0 / (fact(a) * fact(b) * fact(c)); }
+
+/* per-cell adjoint accumulator: D_alpha (35 monomial slots, orders 0,2,3,4
+   used) and V = sum_i t_ic */
+#define ORC_ADJ 38
+
+static void atomic_add(double *p, double v) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+    *p += v;
+}
+
+/* pairwise Jacobian term (b I - c d d^T) w */
+static void pair_jw(const double d[3], double b, double c, const double w[3], double out[3]) {
+    double dw = d[0] * w[0] + d[1] * w[1] + d[2] * w[2];
+    for (int k = 0; k < 3; ++k) out[k] = b * w[k] - c * d[k] * dw;
+}
+
+void orc_adjoint_lists(int n_nodes, const int32_t *node_child, int n_buckets,
+                       const int32_t *bucket_node, const int32_t *node_first,
+                       const int32_t *node_count, const double *geo, const double *mom,
+                       const double *pos, const double *mass, const double *soft,
+                       const int32_t *cells, const int64_t *cell_marks,
+                       const int32_t *parts, const int64_t *part_marks, double period,
+                       const double *w, int n_threads, double *grad) {
+    mono_init();
+    double *acc = (double *)calloc((size_t)n_nodes * ORC_ADJ, sizeof(double));
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 8)
+#endif
+    for (int bk = 0; bk < n_buckets; ++bk) {
+        const int bn = bucket_node[bk];
+        const int first = node_first[bn], count = node_count[bn];
+        for (int64_t e = cell_marks[bk]; e < cell_marks[bk + 1]; ++e) {
+            const int cn = cells[2 * e];
+            double off[3];
+            decode_offset(cells[2 * e + 1], period, off);
+            const double *cg = geo + (size_t)cn * 12;
+            double *A = acc + (size_t)cn * ORC_ADJ;
+            const double cx = cg[0] + off[0], cy = cg[1] + off[1], cz = cg[2] + off[2];
+            if (cells[2 * e + 1] < 0) {
+                /* softened: spline monopole pseudo-particle at the cm */
+                const double M = cg[5];
+                double d0 = 0.0, V[3] = {0, 0, 0};
+                for (int i = first; i < first + count; ++i) {
+                    const double *wi = w + 3 * i;
+                    double d[3] = {cx - pos[3 * i], cy - pos[3 * i + 1], cz - pos[3 * i + 2]};
+                    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    if (r2 == 0.0) continue;
+                    double a, b;
+                    orc_spline(r2, soft[i] + cg[4], &a, &b);
+                    double c = spline_c(r2, soft[i] + cg[4]), jw[3];
+                    pair_jw(d, b, c, wi, jw);
+                    for (int k = 0; k < 3; ++k) {
+                        atomic_add(&grad[4 * i + k], -M * jw[k]);
+                        V[k] -= M * jw[k];
+                    }
+                    d0 += b * (wi[0] * d[0] + wi[1] * d[1] + wi[2] * d[2]);
+                }
+                atomic_add(&A[0], d0);
+                for (int k = 0; k < 3; ++k) atomic_add(&A[35 + k], V[k]);
+                continue;
+            }
+            cmom C;
+            to_complete(mom + (size_t)cn * ORC_NMOM, cg[3], &C);
+            double Dl[35], V[3] = {0, 0, 0};
+            memset(Dl, 0, sizeof(Dl));
+            for (int i = first; i < first + count; ++i) {
+                const double *wi = w + 3 * i;
+                double r[3] = {pos[3 * i] - cx, pos[3 * i + 1] - cy, pos[3 * i + 2] - cz};
+                orc_tder T;
+                rinv_derivs(r, 6, T);
+                /* target side: t_j = sum_n sum_al (-1)^n/al! C_al sum_k w_k T[al+e_j+e_k] */
+                double t[3] = {0, 0, 0};
+                for (int q = 0; q < 35; ++q) {
+                    const int a = mono_abc[q][0], b = mono_abc[q][1], c = mono_abc[q][2];
+                    const int n = a + b + c;
+                    if (n == 1) continue;
+                    const double f = ((n & 1) ? -1.0 : 1.0) * inv_fact3(a, b, c) * C.v[a][b][c];
+                    if (f == 0.0) continue;
+                    t[0] += f * (wi[0] * T[a + 2][b][c] + wi[1] * T[a + 1][b + 1][c] +
+                                 wi[2] * T[a + 1][b][c + 1]);
+                    t[1] += f * (wi[0] * T[a + 1][b + 1][c] + wi[1] * T[a][b + 2][c] +
+                                 wi[2] * T[a][b + 1][c + 1]);
+                    t[2] += f * (wi[0] * T[a + 1][b][c + 1] + wi[1] * T[a][b + 1][c + 1] +
+                                 wi[2] * T[a][b][c + 2]);
+                }
+                for (int k = 0; k < 3; ++k) {
+                    atomic_add(&grad[4 * i + k], t[k]);
+                    V[k] += t[k];
+                }
+                /* source side: D_al += sum_k w_k T[al+e_k], |al| in {0,2,3,4} */
+                for (int q = 0; q < 35; ++q) {
+                    const int a = mono_abc[q][0], b = mono_abc[q][1], c = mono_abc[q][2];
+                    if (a + b + c == 1) continue;
+                    Dl[q] += wi[0] * T[a + 1][b][c] + wi[1] * T[a][b + 1][c] + wi[2] * T[a][b][c + 1];
+                }
+            }
+            for (int q = 0; q < 35; ++q) atomic_add(&A[q], Dl[q]);
+            for (int k = 0; k < 3; ++k) atomic_add(&A[35 + k], V[k]);
+        }
+        for (int64_t e = part_marks[bk]; e < part_marks[bk + 1]; ++e) {
+            const int idx = parts[3 * e], num = parts[3 * e + 2];
+            double off[3];
+            decode_offset(parts[3 * e + 1], period, off);
+            for (int s = idx; s < idx + num; ++s) {
+                double gs[4] = {0, 0, 0, 0};
+                for (int i = first; i < first + count; ++i) {
+                    const double *wi = w + 3 * i;
+                    double d[3] = {pos[3 * s] + off[0] - pos[3 * i],
+                                   pos[3 * s + 1] + off[1] - pos[3 * i + 1],
+                                   pos[3 * s + 2] + off[2] - pos[3 * i + 2]};
+                    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    if (r2 == 0.0) continue;
+                    double a, b;
+                    orc_spline(r2, soft[i] + soft[s], &a, &b);
+                    double c = spline_c(r2, soft[i] + soft[s]), jw[3];
+                    pair_jw(d, b, c, wi, jw);
+                    for (int k = 0; k < 3; ++k) {
+                        atomic_add(&grad[4 * i + k], -mass[s] * jw[k]);
+                        gs[k] += mass[s] * jw[k];
+                    }
+                    gs[3] += b * (wi[0] * d[0] + wi[1] * d[1] + wi[2] * d[2]);
+                }
+                for (int k = 0; k < 4; ++k) atomic_add(&grad[4 * s + k], gs[k]);
+            }
+        }
+    }
+
+    /* top-down: Hhat per node, summed with the parent's polynomial
+       re-expanded about this node's centre of mass */
+    double *P = (double *)calloc((size_t)n_nodes * 35, sizeof(double));
+    int *parent = (int *)malloc(sizeof(int) * (size_t)n_nodes);
+    for (int n = 0; n < n_nodes; ++n) parent[n] = -1;
+    for (int n = 0; n < n_nodes; ++n)
+        for (int k = 0; k < 2; ++k)
+            if (node_child[2 * n + k] >= 0) parent[node_child[2 * n + k]] = n;
+    for (int n = 0; n < n_nodes; ++n) { /* parents precede children */
+        const double *g = geo + (size_t)n * 12;
+        const double *A = acc + (size_t)n * ORC_ADJ;
+        double *Pn = P + (size_t)n * 35;
+        const double M = g[5];
+        cmom C;
+        to_complete(mom + (size_t)n * ORC_NMOM, g[3], &C);
+        /* H coefficients */
+        for (int q = 0; q < 35; ++q) {
+            const int a = mono_abc[q][0], b = mono_abc[q][1], c = mono_abc[q][2];
+            const int o = a + b + c;
+            if (o == 1) continue;
+            Pn[q] = ((o & 1) ? -1.0 : 1.0) * inv_fact3(a, b, c) * A[q];
+        }
+        /* U = (sum_k m_k grad H(s_k) + V) / M; sum_k m_k d_j H(s_k) =
+           sum_{|be| in {2,3}} (-1)^(|be|+1) / be! D[be+e_j] C[be] */
+        double U[3];
+        for (int j = 0; j < 3; ++j) {
+            double S = 0.0;
+            for (int q = 0; q < 35; ++q) {
+                const int a = mono_abc[q][0], b = mono_abc[q][1], c = mono_abc[q][2];
+                const int o = a + b + c;
+                if (o != 2 && o != 3) continue;
+                const int e[3] = {a + (j == 0), b + (j == 1), c + (j == 2)};
+                S += (((o + 1) & 1) ? -1.0 : 1.0) * inv_fact3(a, b, c) *
+                     A[mono_idx[e[0]][e[1]][e[2]]] * C.v[a][b][c];
+            }
+            U[j] = M > 0.0 ? (S + A[35 + j]) / M : 0.0;
+        }
+        Pn[mono_idx[1][0][0]] = -U[0];
+        Pn[mono_idx[0][1][0]] = -U[1];
+        Pn[mono_idx[0][0][1]] = -U[2];
+        const int p = parent[n];
+        if (p >= 0) {
+            const double *gp = geo + (size_t)p * 12;
+            const double dd[3] = {g[0] - gp[0], g[1] - gp[1], g[2] - gp[2]};
+            const double *Pp = P + (size_t)p * 35;
+            /* Pp(s + dd) = sum_al p_al prod_axis sum_k C(al_axis, k) s^k dd^(al_axis - k) */
+            for (int q = 0; q < 35; ++q) {
+                if (Pp[q] == 0.0) continue;
+                const int a = mono_abc[q][0], b = mono_abc[q][1], c = mono_abc[q][2];
+                for (int i = 0; i <= a; ++i)
+                    for (int j = 0; j <= b; ++j)
+                        for (int k = 0; k <= c; ++k) {
+                            double f = Pp[q] * multinom(i, a - i, 0) * multinom(j, b - j, 0) *
+                                       multinom(k, c - k, 0) * ipow(dd[0], a - i) *
+                                       ipow(dd[1], b - j) * ipow(dd[2], c - k);
+                            Pn[mono_idx[i][j][k]] += f;
+                        }
+            }
+        }
+    }
+    /* buckets: dL/dm_k += P(s_k), dL/dx_k += m_k grad P(s_k) */
+    for (int n = 0; n < n_nodes; ++n) {
+        if (node_child[2 * n] >= 0) continue;
+        const double *g = geo + (size_t)n * 12;
+        const double *Pn = P + (size_t)n * 35;
+        for (int k = node_first[n]; k < node_first[n] + node_count[n]; ++k) {
+            const double s[3] = {pos[3 * k] - g[0], pos[3 * k + 1] - g[1], pos[3 * k + 2] - g[2]};
+            double val = 0.0, gr[3] = {0, 0, 0};
+            for (int q = 0; q < 35; ++q) {
+                const int a = mono_abc[q][0], b = mono_abc[q][1], c = mono_abc[q][2];
+                const double pa = ipow(s[0], a), pb = ipow(s[1], b), pc = ipow(s[2], c);
+                val += Pn[q] * pa * pb * pc;
+                if (a) gr[0] += Pn[q] * a * ipow(s[0], a - 1) * pb * pc;
+                if (b) gr[1] += Pn[q] * b * pa * ipow(s[1], b - 1) * pc;
+                if (c) gr[2] += Pn[q] * c * pa * pb * ipow(s[2], c - 1);
+            }
+            for (int j = 0; j < 3; ++j) grad[4 * k + j] += mass[k] * gr[j];
+            grad[4 * k + 3] += val;
+        }
+    }
+    free(parent);
+    free(P);
+    free(acc);
+}
+
+/* Exact gradient of L = sum_i w_i . a_i for the softened direct sum:
+   dL/dx_k = sum_j J(x_k - x_j) (m_k w_j - m_j w_k), J = b I - c d d^T;
+   dL/dm_k = sum_i w_i . g(x_k - x_i), g(d) = d b(|d|). */
+void orc_direct_grad(int n, const double *pos, const double *mass, const double *soft,
+                     const double *w, int n_threads, double *grad) {
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(static)
+#endif
+    for (int k = 0; k < n; ++k) {
+        double gx[3] = {0, 0, 0}, gm = 0.0;
+        for (int j = 0; j < n; ++j) {
+            if (j == k) continue;
+            double d[3] = {pos[3 * k] - pos[3 * j], pos[3 * k + 1] - pos[3 * j + 1],
+                           pos[3 * k + 2] - pos[3 * j + 2]};
+            double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (r2 == 0.0) continue;
+            double a, b;
+            orc_spline(r2, soft[k] + soft[j], &a, &b);
+            double c = spline_c(r2, soft[k] + soft[j]);
+            double v[3], jv[3];
+            for (int q = 0; q < 3; ++q) v[q] = mass[k] * w[3 * j + q] - mass[j] * w[3 * k + q];
+            pair_jw(d, b, c, v, jv);
+            for (int q = 0; q < 3; ++q) gx[q] += jv[q];
+            gm += b * (w[3 * j] * d[0] + w[3 * j + 1] * d[1] + w[3 * j + 2] * d[2]);
+        }
+        for (int q = 0; q < 3; ++q) grad[4 * k + q] = gx[q];
+        grad[4 * k + 3] = gm;
+    }
 }

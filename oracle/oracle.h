@@ -92,7 +92,7 @@ void orc_forces_lists(int n_buckets, const int32_t *bucket_node,
                       const int32_t *rung, int active_rung,
                       const int32_t *cells, const int64_t *cell_marks,
                       const int32_t *parts, const int64_t *part_marks,
-                      double period, int n_threads,
+                      double period, int soft_flags, int n_threads,
                       double *acc, double *pot, double *dtgrav, int64_t counts[3]);
 
 /* openSoftening (gravity.h:251-260) for one (cell, bucket, offset) triple. */
@@ -122,6 +122,22 @@ void orc_direct_sum(int n, const double *pos, const double *mass, const double *
 void orc_ewald_exact(int n, const double *pos, const double *mass, const double *soft,
                      int n_targets, const int32_t *targets, double L, double alpha,
                      double rcut, int hmax, int n_threads, double *acc, double *pot);
+
+/* Adjoint of orc_forces_lists (softened cells as flagged in the lists) for
+   L = sum_i w_i . a_i (w: 3 per particle): grad[4k..4k+2] += dL/dx_k,
+   grad[4k+3] += dL/dm_k. Isolated or periodic tree part (no Ewald term). */
+void orc_adjoint_lists(int n_nodes, const int32_t *node_child, int n_buckets,
+                       const int32_t *bucket_node, const int32_t *node_first,
+                       const int32_t *node_count, const double *geo, const double *mom,
+                       const double *pos, const double *mass, const double *soft,
+                       const int32_t *cells, const int64_t *cell_marks,
+                       const int32_t *parts, const int64_t *part_marks, double period,
+                       const double *w, int n_threads, double *grad);
+/* Exact gradient of L for the softened direct sum over all n particles. */
+void orc_direct_grad(int n, const double *pos, const double *mass, const double *soft,
+                     const double *w, int n_threads, double *grad);
+/* d^(a+b+c)(1/|r|)/dx^a dy^b dz^c into out[a*49 + b*7 + c], a+b+c <= nmax <= 6. */
+void orc_rinv_derivs(const double r[3], int nmax, double *out);
 
 #ifdef __cplusplus
 }

@@ -45,7 +45,7 @@ def lib():
     L.orc_mom_eval.argtypes = [_p, _d, _d, _d, _d, _p, _p]
     L.orc_build_moments.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p]
     L.orc_forces_lists.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i,
-                                   _p, _p, _p, _p, _d, _i, _p, _p, _p, _p]
+                                   _p, _p, _p, _p, _d, _i, _i, _p, _p, _p, _p]
     L.orc_open_softening.argtypes = [_p, _p, _d, _d, _d]
     L.orc_open_softening.restype = _i
     L.orc_ewald_table.argtypes = [_p, _d, _d, _d, _p, _i]
@@ -54,6 +54,10 @@ def lib():
                                    _p, _p, _p]
     L.orc_direct_sum.argtypes = [_i, _p, _p, _p, _i, _p, _i, _p, _p]
     L.orc_ewald_exact.argtypes = [_i, _p, _p, _p, _i, _p, _d, _d, _d, _i, _i, _p, _p]
+    L.orc_adjoint_lists.argtypes = [_i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                    _p, _d, _p, _i, _p]
+    L.orc_direct_grad.argtypes = [_i, _p, _p, _p, _p, _i, _p]
+    L.orc_rinv_derivs.argtypes = [_p, _i, _p]
     _lib = L
     return L
 
@@ -140,10 +144,12 @@ def build_moments(tree, pos, mass, soft):
 
 
 def forces_lists(tree, geo, mom, pos, mass, soft, lists, period=0.0, rung=None,
-                 active_rung=0, n_threads=0, buckets=None):
+                 active_rung=0, n_threads=0, buckets=None, soft_flags=False):
     """nodeBucketForce/partBucketForce over the interaction lists. `buckets`
     restricts the evaluation to a subset of bucket indices (the lists are then
-    re-sliced). Returns acc[N,3], pot[N], dtgrav[N], counts[3]."""
+    re-sliced). soft_flags: take the softened-cell decision from bit 31 of
+    the list entries instead of openSoftening on `geo` (keeps it fixed when
+    positions are perturbed). Returns acc[N,3], pot[N], dtgrav[N], counts[3]."""
     n = pos.shape[0]
     acc = np.zeros((n, 3))
     pot = np.zeros(n)
@@ -162,8 +168,8 @@ def forces_lists(tree, geo, mom, pos, mass, soft, lists, period=0.0, rung=None,
                            _ptr(tree["node_count"]), _ptr(geo), _ptr(mom), _ptr(pos),
                            _ptr(mass), _ptr(soft), _ptr(rg), active_rung,
                            _ptr(cells), _ptr(cmarks), _ptr(parts), _ptr(pmarks),
-                           float(period), n_threads, _ptr(acc), _ptr(pot), _ptr(dtg),
-                           _ptr(counts))
+                           float(period), int(soft_flags), n_threads, _ptr(acc), _ptr(pot),
+                           _ptr(dtg), _ptr(counts))
     return acc, pot, dtg, counts
 
 
@@ -231,3 +237,36 @@ def ewald_exact(pos, mass, soft, targets, L=1.0, alpha=None, rcut=None, hmax=Non
     lib().orc_ewald_exact(pos.shape[0], _ptr(pos), _ptr(mass), _ptr(soft), len(t), _ptr(t),
                           L, alpha, rcut, hmax, n_threads, _ptr(acc), _ptr(pot))
     return acc, pot
+
+
+def adjoint_lists(tree, geo, mom, pos, mass, soft, lists, w, period=0.0, n_threads=0):
+    """Exact gradient of L = sum_i w_i . a_i for the list evaluation with
+    softened cells as flagged in the lists (forces_lists(..., soft_flags=True)).
+    Returns grad[N, 4]: dL/dx, dL/dm."""
+    n = pos.shape[0]
+    grad = np.zeros((n, 4))
+    wv = np.ascontiguousarray(w, dtype=np.float64)
+    lib().orc_adjoint_lists(tree["n_nodes"], _ptr(tree["node_child"]), tree["n_buckets"],
+                            _ptr(tree["bucket_node"]), _ptr(tree["node_first"]),
+                            _ptr(tree["node_count"]), _ptr(geo), _ptr(mom), _ptr(pos),
+                            _ptr(mass), _ptr(soft), _ptr(lists["cells"]),
+                            _ptr(lists["cell_marks"]), _ptr(lists["parts"]),
+                            _ptr(lists["part_marks"]), float(period), _ptr(wv), n_threads,
+                            _ptr(grad))
+    return grad
+
+
+def direct_grad(pos, mass, soft, w, n_threads=0):
+    """Exact gradient of L = sum_i w_i . a_i for the softened direct sum."""
+    n = pos.shape[0]
+    grad = np.zeros((n, 4))
+    wv = np.ascontiguousarray(w, dtype=np.float64)
+    lib().orc_direct_grad(n, _ptr(pos), _ptr(mass), _ptr(soft), _ptr(wv), n_threads, _ptr(grad))
+    return grad
+
+
+def rinv_derivs(r, nmax=6):
+    """T[a, b, c] = d^(a+b+c)(1/|r|)/dx^a dy^b dz^c (zero beyond order nmax)."""
+    out = np.zeros(343)
+    lib().orc_rinv_derivs(_ptr(np.ascontiguousarray(r, dtype=np.float64)), nmax, _ptr(out))
+    return out.reshape(7, 7, 7)
