@@ -20,7 +20,7 @@ HOST_LIB = os.path.join(PKG, "libchanga_host.so")
 GPU_LIB = os.path.join(PKG, "libchanga_gpu.so")
 
 HOST_SRC = ["host/host_tree.cpp", "host/host_walk.cpp", "host/host_api.cpp"]
-GPU_SRC = ["gpu/gravity.hip", "gpu/batch.hip", "gpu/ewald.hip", "gpu/moments.hip",
+GPU_SRC = ["gpu/gravity.hip", "gpu/batch.hip", "gpu/adjoint.hip", "gpu/ewald.hip", "gpu/moments.hip",
            "gpu/hostcuda.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

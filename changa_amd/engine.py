@@ -278,6 +278,50 @@ class Evaluation:
         self.run_batched()
         self.run_ewald()
 
+    # ------------------------------------------------------------------ adjoint form
+    def adjoint(self, w):
+        """dL/dx and dL/dm (tree order, [N, 4] FP64 on the device) of
+        L = sum_i w_i . a_i for this evaluation's lists (precision form), with
+        the moments of the last build. w: [N, 3] in tree order (array or
+        tensor)."""
+        if not self.precise:
+            raise ValueError("the adjoint runs on the precision form (FP64 positions)")
+        n = self.pos.shape[0]
+        wt = torch.as_tensor(w, dtype=torch.float32).to(self.device).contiguous()
+        if tuple(wt.shape) != (n, 3):
+            raise ValueError(f"w must be [{n}, 3]")
+        tr = self.tree
+        grad = torch.empty((n, 4), dtype=torch.float64, device=self.device)
+        entries = int(self.lists["cell_marks"][-1])
+        if getattr(self, "d_node_work", None) is None:
+            self.d_node_work = torch.empty((tr["n_nodes"], 80), dtype=torch.float64,
+                                           device=self.device)
+            ints = gpu.lib().changaGravityAdjointWorkInts(tr["n_nodes"], entries)
+            self.d_list_work = torch.empty(ints, dtype=torch.int32, device=self.device)
+        lo = np.ascontiguousarray(tr["level_offsets"], dtype=np.int32)
+        a = gpu.ChangaAdjointLaunch()
+        a.cells = self._launch(True)
+        a.parts = self._launch(False)
+        a.w = wt.data_ptr()
+        a.grad = grad.data_ptr()
+        a.nodeWork = self.d_node_work.data_ptr()
+        a.listWork = self.d_list_work.data_ptr()
+        a.listWorkInts = self.d_list_work.numel()
+        a.numCellEntries = entries
+        a.nNodes = tr["n_nodes"]
+        a.nDepths = tr["n_depths"]
+        a.nParticles = n
+        a.levelOffsets = lo.ctypes.data
+        a.first = self.d_first.data_ptr()
+        a.count = self.d_count.data_ptr()
+        a.child = self.d_child.data_ptr()
+        a.node64 = self.d_node64.data_ptr()
+        a.pos64 = self.d_pos64.data_ptr()
+        a.mass64 = self.d_mass64.data_ptr()
+        gpu.check(gpu.lib().changaGravityAdjoint(ctypes.byref(a), self._sp()),
+                  "changaGravityAdjoint")
+        return grad
+
     def free_batch(self):
         plan = getattr(self, "_batch_plan", None)
         if plan is not None and plan.value:

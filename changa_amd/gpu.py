@@ -74,6 +74,15 @@ class ChangaBatchOptions(ctypes.Structure):
                 ("zeroVars", c_int), ("tunedMs", c_float)]
 
 
+class ChangaAdjointLaunch(ctypes.Structure):
+    _fields_ = [("cells", ChangaGravityLaunch), ("parts", ChangaGravityLaunch), ("w", c_void_p),
+                ("grad", c_void_p), ("nodeWork", c_void_p), ("listWork", c_void_p),
+                ("listWorkInts", c_size_t), ("numCellEntries", c_int), ("nNodes", c_int),
+                ("nDepths", c_int), ("nParticles", c_int), ("levelOffsets", c_void_p),
+                ("first", c_void_p), ("count", c_void_p), ("child", c_void_p),
+                ("node64", c_void_p), ("pos64", c_void_p), ("mass64", c_void_p)]
+
+
 CALLBACK = ctypes.CFUNCTYPE(None, c_void_p)
 
 # every extern "C" symbol include/changa_gpu.h declares
@@ -88,6 +97,7 @@ EXPORTS = [
     "changaGpuReleasePools", "TreePiecePartListDataTransferLocalCompact", "changaNodeGravity",
     "changaPartGravity", "changaZeroVars", "changaEwald", "changaBuildMoments",
     "changaBatchPlanCreate", "changaBatchPlanLaunch", "changaBatchPlanRun", "changaBatchPlanDestroy",
+    "changaGravityAdjoint", "changaGravityAdjointWorkInts",
 ]
 
 _lib = None
@@ -143,6 +153,10 @@ def lib():
         getattr(L, name).restype = c_int
     L.changaBatchPlanDestroy.argtypes = [P]
     L.changaBatchPlanDestroy.restype = c_int
+    L.changaGravityAdjoint.argtypes = [ctypes.POINTER(ChangaAdjointLaunch), P]
+    L.changaGravityAdjoint.restype = c_int
+    L.changaGravityAdjointWorkInts.argtypes = [c_int, c_int]
+    L.changaGravityAdjointWorkInts.restype = c_size_t
     _lib = L
     return L
 

@@ -199,6 +199,38 @@ int changaBatchPlanLaunch(ChangaBatchPlan *plan, void *stream);
 int changaBatchPlanRun(ChangaBatchPlan *plan, void *stream);
 int changaBatchPlanDestroy(ChangaBatchPlan *plan);
 
+/* ======================================================================
+ * Part 4: adjoint form
+ *
+ * For L = sum_i w_i . a_i, with a the tree acceleration of the cell and
+ * particle lists (precision form), the exact gradient dL/dx and dL/dm of
+ * every particle on the same lists: target-side tidal terms, particle-source
+ * terms, and cell-source terms carried down the tree as polynomials about
+ * each node's centre of mass. Softening lengths are constants. Periodic
+ * replicas in the lists are handled; the Ewald correction is not part of L.
+ * ====================================================================== */
+typedef struct {
+  ChangaGravityLaunch cells;      /* particle-cell launch (listKind 0, tpos64 and cm64 set) */
+  ChangaGravityLaunch parts;      /* particle-particle launch (spos64 set), same buckets */
+  const float *w;                 /* dL/da, 3 per particle, tree order (device) */
+  double *grad;                   /* out, overwritten: dL/dx (3), dL/dm per particle (device) */
+  double *nodeWork;               /* scratch: 80 doubles per node (device) */
+  int *listWork;                  /* scratch: changaGravityAdjointWorkInts(nNodes,
+                                     numCellEntries) ints (device) */
+  size_t listWorkInts;            /* its size in ints */
+  int numCellEntries;             /* entries of the cell list (cells.marks[numBuckets]) */
+  int nNodes, nDepths, nParticles;
+  const int *levelOffsets;        /* host, nDepths + 1 (as for changaBuildMoments) */
+  const int *first, *count, *child;  /* device, as for changaBuildMoments */
+  const double *node64;           /* device: node64 of changaBuildMoments */
+  const double *pos64, *mass64;   /* device: particles in tree order */
+} ChangaAdjointLaunch;
+
+/* Ints of listWork for a tree of nNodes nodes and a cell list of
+   numCellEntries entries (the cell list transposed per cell). */
+size_t changaGravityAdjointWorkInts(int nNodes, int numCellEntries);
+int changaGravityAdjoint(const ChangaAdjointLaunch *a, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

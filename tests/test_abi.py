@@ -13,7 +13,7 @@ HDR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 
 def declared():
     txt = open(HDR).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    names = re.findall(r"^\s*(?:void|int)\s*\*?\s*(\w+)\s*\(", txt, flags=re.M)
+    names = re.findall(r"^\s*(?:void|int|size_t)\s*\*?\s*(\w+)\s*\(", txt, flags=re.M)
     return sorted(set(names))
 
 
