@@ -11,9 +11,10 @@ timed and reported.
 Workloads: --config 2 (default), the 128^3 periodic Zel'dovich box
 (2,097,152 particles, Ewald nReplicas = 1, fEwCut = 2.6, |h|^2 <= 10,
 theta = 0.7, bucket 16); --config 3, 2^22 particles in 512 Hernquist halos
-plus a 20% uniform background (theta = 0.6, bucket 32, isolated). The metric
-is G interactions/s, an interaction being one (active target, source cell /
-particle / Ewald term) pair as ChaNGa counts them.
+plus a 20% uniform background (theta = 0.6, bucket 32, isolated); --config 5,
+the 256^3 box (rms 0.15) with a rung of 10% active particles, the only
+targets. The metric is G interactions/s, an interaction being one (active
+target, source cell / particle / Ewald term) pair as ChaNGa counts them.
 
 Prints one JSON line. --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -40,14 +41,15 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--form", choices=["precise", "plain"], default="precise")
     ap.add_argument("--path", choices=["batched", "per-bucket"], default="batched")
-    ap.add_argument("--config", type=int, choices=[2, 3], default=2)
+    ap.add_argument("--config", type=int, choices=[2, 3, 5], default=2)
     ap.add_argument("--ng", type=int, default=128, help="lattice size of the periodic box")
     ap.add_argument("--n", type=int, default=1 << 22, help="particles of the halo workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy legs")
     ap.add_argument("--cpu-frac", type=float, default=None,
                     help="fraction of the buckets the CPU baseline evaluates "
-                         "(default 1 for config 2, 1/4 for config 3: about 10-20 s of CPU work)")
+                         "(default 1 for config 2, 1/4 for 3, 1/10 for 5: about 10-30 s of "
+                         "CPU work)")
     ap.add_argument("--samples", type=int, default=256)
     return ap.parse_args()
 
@@ -70,6 +72,14 @@ def main():
                     "theta=0.7, bucket 16, hexadecapole")
         data = "synthetic (seeded Zel'dovich lattice, numpy PCG64 seed 1002)"
         tag = f"zeldovich{args.ng}"
+    elif args.config == 5:
+        ng = 256 if args.ng == 128 else args.ng
+        w = configs.zeldovich(ng=ng, rms=0.15, cfg=5, active_frac=0.1)
+        workload = (f"periodic Zel'dovich box {ng}^3 ({ng ** 3} particles), P(k)~k^-2.5, "
+                    "rms 0.15, rung mask with 10% of particles active (targets), Ewald "
+                    "nReplicas=1 fEwCut=2.6 |h|^2<=10, theta=0.7, bucket 16, hexadecapole")
+        data = "synthetic (seeded Zel'dovich lattice and Bernoulli(0.1) rung, numpy PCG64 seed 1005)"
+        tag = f"zeldovich{ng}_active"
     else:
         w = configs.halos(n=args.n, cfg=3)
         workload = (f"{args.n} particles: 512 Hernquist halos (dN/dM~M^-1.9 over 3 decades, "
@@ -177,7 +187,7 @@ def main():
         out["accuracy"] = accuracy(ev, ev.step_batched if args.path == "batched" else ev.step,
                                    args.samples)
     if not args.no_cpu:
-        frac = args.cpu_frac if args.cpu_frac is not None else {2: 1.0, 3: 0.25}[args.config]
+        frac = args.cpu_frac if args.cpu_frac is not None else {2: 1.0, 3: 0.25, 5: 0.1}[args.config]
         out["cpu_baseline"] = cpu_baseline(ev, frac)
     print(json.dumps(out))
 
@@ -212,8 +222,11 @@ def accuracy(ev, step, n_samples):
     oa, op, _ = oracle_forces(ev, buckets=bsel)
     idx = np.concatenate([np.arange(tr["bucket_start"][b], tr["bucket_start"][b] + tr["bucket_size"][b])
                           for b in bsel])
+    targets = np.arange(ev.pos.shape[0]) if ev.active is None else np.nonzero(ev.active)[0]
+    if ev.active is not None:
+        idx = idx[ev.active[idx]]
     e_or = rel_err(acc[idx], oa[idx])
-    samp = np.sort(rng.choice(ev.pos.shape[0], size=n_samples, replace=False)).astype(np.int32)
+    samp = np.sort(rng.choice(targets, size=n_samples, replace=False)).astype(np.int32)
     t = time.time()
     if ev.w["periodic"]:
         xa, xp = O.ewald_exact(ev.pos, ev.mass, ev.soft, samp, L=ev.L, rcut=0.45 * ev.L)
@@ -243,11 +256,15 @@ def cpu_baseline(ev, frac):
     nb = max(1, int(frac * tr["n_buckets"]))
     bsel = np.sort(rng.choice(tr["n_buckets"], size=nb, replace=False))
     geo, mom = O.build_moments(tr, ev.pos, ev.mass, ev.soft)
+    rung = None if ev.active is None else ev.active.astype(np.int32)
     t = time.time()
     _, _, _, c = O.forces_lists(tr, geo, mom, ev.pos, ev.mass, ev.soft, ev.lists,
-                                period=ev.L, n_threads=threads, buckets=bsel)
+                                period=ev.L, rung=rung, active_rung=1, n_threads=threads,
+                                buckets=bsel)
     targets = np.concatenate([np.arange(tr["bucket_start"][b], tr["bucket_start"][b] + tr["bucket_size"][b])
                               for b in bsel])
+    if ev.active is not None:
+        targets = targets[ev.active[targets]]
     ce = np.zeros(1, dtype=np.int64)
     if ev.w["periodic"]:
         ewt = O.ewald_table(mom[0], geo[0, 3], ev.L, ev.w["ewh_cut"])

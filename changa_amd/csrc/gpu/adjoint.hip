@@ -23,21 +23,28 @@
 // Softening lengths are constants (a cell's mass-weighted softening is not
 // differentiated).
 //
-// Passes: (1) the cell list is transposed on the GPU (count, scan, scatter:
-// for every cell, the (bucket, replica) pairs that use it); (2) a persistent
-// cell-major kernel accumulates D for chunks of each cell's users, one wave
-// per chunk, lanes over the users' targets; (3) a bucket-major kernel adds
-// the target-side terms of cells (lanes as in the forward kernel: slots of
-// targets over LDS-staged cells) and both sides of particle pairs (one source
-// per lane, targets broadcast); (4) top-down polynomial pass per depth;
-// (5) evaluation at the particles.
+// Passes: (1) the cell list is transposed on the GPU: a stable radix sort of
+// the entries by cell (rocPRIM) gives, for every cell, the (bucket, replica)
+// pairs that use it in bucket order; (2) a persistent cell-major kernel
+// accumulates D for chunks of each cell's users, one wave per chunk, lanes
+// over the users' targets; (3) a bucket-major kernel adds the target-side
+// terms of cells (lanes as in the forward kernel: slots of targets over
+// LDS-staged cells) and both sides of particle pairs (one source per lane,
+// targets broadcast); (4) Hhat of every node, then a top-down pass per depth
+// (one wave per child, one lane per polynomial coefficient); (5) evaluation
+// at the particles.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
 #include "gravity_body.h"
 
 namespace cgpu {
 
-constexpr int kAdjStride = 80;   // doubles per node: D[36], Vs[3], pad, P[35] at 40
+constexpr int kAdjStride = 112;  // doubles per node: D[36], Vs[3], pad, Hhat[35] at 40,
+                                 // P[35] at 75
 constexpr int kAdjVs = 36;
-constexpr int kAdjPoly = 40;
+constexpr int kAdjHhat = 40;
+constexpr int kAdjPoly = 75;
 constexpr int kUsersPerItem = 128;
 constexpr int kAdjPcTile = 64;
 
@@ -104,49 +111,49 @@ __device__ __forceinline__ void pc_tidal(const float4* rec, float rx, float ry, 
 }
 
 // ---------------------------------------------------------------------------
-// Derivatives of 1/r: T[a][b][c] = d^(a+b+c)(1/|r|)/dx^a dy^b dz^c for
-// a+b+c <= 6, c <= 2 (a set closed under the recursion). From
-// r^2 d_x f + x f = 0 differentiated by Leibniz' rule:
-//   r^2 T(al+e_x) = -(2a+1) x T(al) - 2b y T(al+e_x-e_y) - 2c z T(al+e_x-e_z)
-//                   - a^2 T(al-e_x) - b(b-1) T(al+e_x-2e_y) - c(c-1) T(al+e_x-2e_z),
+// D_alpha = d^alpha f for the dipole potential f = w . grad(1/r) = -(w.r) g,
+// g = 1/r^3. By Leibniz' rule
+//   D_alpha = -[(w.r) G_alpha + sum_k alpha_k w_k G_(alpha-e_k)],
+// with G_alpha = d^alpha g for |alpha| <= 5, c <= 1 (a set closed under the
+// recursion). Differentiating r^2 d_x g + 3 x g = 0:
+//   r^2 G(al+e_x) = -(2a+3) x G(al) - 2b y G(al+e_x-e_y) - 2c z G(al+e_x-e_z)
+//                   - a(a+2) G(al-e_x) - b(b-1) G(al+e_x-2e_y) - c(c-1) G(al+e_x-2e_z),
 // and the same with the axes permuted.
 // ---------------------------------------------------------------------------
-struct RinvDer {
-  float t[7][7][3];
+struct R3Der {
+  float t[6][6][2];
 };
 
-__device__ __forceinline__ void rinv_derivs6(float x, float y, float z, float ir, float ir2,
-                                             RinvDer& T) {
-  T.t[0][0][0] = ir;
+__device__ __forceinline__ void r3_derivs5(float x, float y, float z, float ir, float ir2,
+                                           R3Der& G) {
+  G.t[0][0][0] = ir * ir2;
 #pragma unroll
-  for (int n = 0; n < 6; ++n) {
+  for (int n = 0; n < 5; ++n) {
 #pragma unroll
     for (int a = n + 1; a >= 0; --a) {
 #pragma unroll
       for (int b = n + 1 - a; b >= 0; --b) {
         const int c = n + 1 - a - b;
-        if (c > 2) continue;
+        if (c > 1) continue;
         float v;
         if (a > 0) {
           const int A = a - 1;
-          v = -(float)(2 * A + 1) * x * T.t[A][b][c];
-          if (b >= 1) v -= (float)(2 * b) * y * T.t[a][b - 1][c];
-          if (c >= 1) v -= (float)(2 * c) * z * T.t[a][b][c - 1];
-          if (A >= 1) v -= (float)(A * A) * T.t[A - 1][b][c];
-          if (b >= 2) v -= (float)(b * (b - 1)) * T.t[a][b - 2][c];
-          if (c >= 2) v -= (float)(c * (c - 1)) * T.t[a][b][c - 2];
+          v = -(float)(2 * A + 3) * x * G.t[A][b][c];
+          if (b >= 1) v -= (float)(2 * b) * y * G.t[a][b - 1][c];
+          if (c >= 1) v -= (float)(2 * c) * z * G.t[a][b][c - 1];
+          if (A >= 1) v -= (float)(A * (A + 2)) * G.t[A - 1][b][c];
+          if (b >= 2) v -= (float)(b * (b - 1)) * G.t[a][b - 2][c];
         } else if (b > 0) {
           const int B = b - 1;
-          v = -(float)(2 * B + 1) * y * T.t[0][B][c];
-          if (c >= 1) v -= (float)(2 * c) * z * T.t[0][b][c - 1];
-          if (B >= 1) v -= (float)(B * B) * T.t[0][B - 1][c];
-          if (c >= 2) v -= (float)(c * (c - 1)) * T.t[0][b][c - 2];
+          v = -(float)(2 * B + 3) * y * G.t[0][B][c];
+          if (c >= 1) v -= (float)(2 * c) * z * G.t[0][b][c - 1];
+          if (B >= 1) v -= (float)(B * (B + 2)) * G.t[0][B - 1][c];
         } else {
           const int C = c - 1;
-          v = -(float)(2 * C + 1) * z * T.t[0][0][C];
-          if (C >= 1) v -= (float)(C * C) * T.t[0][0][C - 1];
+          v = -(float)(2 * C + 3) * z * G.t[0][0][C];
+          if (C >= 1) v -= (float)(C * (C + 2)) * G.t[0][0][C - 1];
         }
-        T.t[a][b][c] = v * ir2;
+        G.t[a][b][c] = v * ir2;
       }
     }
   }
@@ -164,17 +171,23 @@ __device__ __forceinline__ void rinv_derivs6(float x, float y, float z, float ir
   X(27, 3, 2, 0) X(28, 2, 3, 0) X(29, 1, 4, 0) X(30, 0, 5, 0) X(31, 4, 0, 1)                \
   X(32, 3, 1, 1) X(33, 2, 2, 1) X(34, 1, 3, 1) X(35, 0, 4, 1)
 
-__constant__ int kD36[36][3] = {
-#define CGPU_ROW(k, a, b, c) {a, b, c},
-    CGPU_D36(CGPU_ROW)
-#undef CGPU_ROW
-};
+template <int A, int B, int C>
+__device__ __forceinline__ float dipole_term(const R3Der& G, float wr, float wx, float wy,
+                                             float wz) {
+  float v = wr * G.t[A][B][C];
+  if constexpr (A > 0) v += (float)A * wx * G.t[A - 1][B][C];
+  if constexpr (B > 0) v += (float)B * wy * G.t[A][B - 1][C];
+  if constexpr (C > 0) v += (float)C * wz * G.t[A][B][C - 1];
+  return v;
+}
 
-// D_alpha += w . grad d^alpha(1/r)
-__device__ __forceinline__ void accumulate_d(const RinvDer& T, float wx, float wy, float wz,
-                                             float* D) {
-#define CGPU_DACC(k, a, b, c) \
-  D[k] += wx * T.t[(a) + 1][b][c] + wy * T.t[a][(b) + 1][c] + wz * T.t[a][b][(c) + 1];
+// D_alpha += d^alpha (w . grad(1/r)) at r
+__device__ __forceinline__ void accumulate_d(float rx, float ry, float rz, float ir, float ir2,
+                                             float wx, float wy, float wz, float* D) {
+  R3Der G;
+  r3_derivs5(rx, ry, rz, ir, ir2, G);
+  const float wr = wx * rx + wy * ry + wz * rz;
+#define CGPU_DACC(k, a, b, c) D[k] -= dipole_term<a, b, c>(G, wr, wx, wy, wz);
   CGPU_D36(CGPU_DACC)
 #undef CGPU_DACC
 }
@@ -226,86 +239,63 @@ __device__ __forceinline__ int find_run(const int* ends, int n, int q) {
   return lo;
 }
 
-// Transposed cell lists (int scratch):
-//   count[nNodes], cursor[nNodes], start[nNodes + 1], item[nNodes + 1],
-//   nItems[1], users[2 * entries] ({bucket, offsetID} per entry).
+// Transposed cell lists (scratch):
+//   key[E], value[E] (the entries as sorted: cell, then {bucket, offsetID}
+//   packed as bucket << 32 | offsetID), start[nNodes + 1] (users of cell c at
+//   [start[c], start[c+1])), chunks[nNodes + 1], item[nNodes + 1] (work
+//   items per cell, exclusive prefix; item[nNodes] is their number).
 struct Transpose {
-  int* count;
-  int* cursor;
+  const unsigned* key;
+  const unsigned long long* value;
   int* start;
+  int* chunks;
   int* item;
-  int* nItems;
-  int2* users;
 };
 
 struct AdjArgs {
   ChangaGravityLaunch pc, pp;
   const float* w;
+  const int* tindex;  // particle of each target, or NULL (identity)
   double* grad;
   double* node;  // kAdjStride doubles per node
   Transpose tr;
   int nNodes;
+  int entries;
 };
 
 // ---------------------------------------------------------------------------
 // (1) transpose
 // ---------------------------------------------------------------------------
-__global__ void adj_zero_int_kernel(int* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
-__global__ __launch_bounds__(64) void adj_count_kernel(AdjArgs a) {
+__global__ __launch_bounds__(64) void adj_keys_kernel(ChangaGravityLaunch pc, unsigned* key,
+                                                      unsigned long long* value) {
   const int b = blockIdx.x;
-  const ILCell* cells = static_cast<const ILCell*>(a.pc.list);
-  for (int e = a.pc.marks[b] + threadIdx.x; e < a.pc.marks[b + 1]; e += kWave)
-    atomicAdd(a.tr.count + cells[e].index, 1);
-}
-
-// one workgroup: exclusive scans of the user counts and of the work items
-// (ceil(count / kUsersPerItem) per cell)
-__global__ __launch_bounds__(1024) void adj_scan_kernel(AdjArgs a) {
-  __shared__ int s_u[1024], s_i[1024];
-  const int n = a.nNodes, t = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int lo = min(n, t * per), hi = min(n, lo + per);
-  int su = 0, si = 0;
-  for (int c = lo; c < hi; ++c) {
-    su += a.tr.count[c];
-    si += (a.tr.count[c] + kUsersPerItem - 1) / kUsersPerItem;
-  }
-  s_u[t] = su;
-  s_i[t] = si;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    const int yu = t >= d ? s_u[t - d] : 0, yi = t >= d ? s_i[t - d] : 0;
-    __syncthreads();
-    s_u[t] += yu;
-    s_i[t] += yi;
-    __syncthreads();
-  }
-  int ru = s_u[t] - su, ri = s_i[t] - si;
-  for (int c = lo; c < hi; ++c) {
-    a.tr.start[c] = ru;
-    a.tr.item[c] = ri;
-    ru += a.tr.count[c];
-    ri += (a.tr.count[c] + kUsersPerItem - 1) / kUsersPerItem;
-  }
-  if (t == 1023) {
-    a.tr.start[n] = s_u[1023];
-    a.tr.item[n] = s_i[1023];
-    a.tr.nItems[0] = s_i[1023];
-  }
-}
-
-__global__ __launch_bounds__(64) void adj_scatter_kernel(AdjArgs a) {
-  const int b = blockIdx.x;
-  const ILCell* cells = static_cast<const ILCell*>(a.pc.list);
-  for (int e = a.pc.marks[b] + threadIdx.x; e < a.pc.marks[b + 1]; e += kWave) {
+  const ILCell* cells = static_cast<const ILCell*>(pc.list);
+  for (int e = pc.marks[b] + threadIdx.x; e < pc.marks[b + 1]; e += kWave) {
     const ILCell c = cells[e];
-    const int k = atomicAdd(a.tr.cursor + c.index, 1);
-    a.tr.users[a.tr.start[c.index] + k] = make_int2(b, c.offsetID);
+    key[e] = (unsigned)c.index;
+    value[e] = ((unsigned long long)(unsigned)b << 32) | (unsigned)c.offsetID;
   }
+}
+
+// start[c] = first sorted position whose cell is >= c
+__global__ void adj_bounds_kernel(const unsigned* key, int entries, int n_nodes, int* start) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (entries == 0) {
+    if (k <= n_nodes) start[k] = 0;
+    return;
+  }
+  if (k >= entries) return;
+  const int hi = (int)key[k];
+  const int lo = k ? (int)key[k - 1] + 1 : 0;
+  for (int c = lo; c <= hi; ++c) start[c] = k;
+  if (k == entries - 1)
+    for (int c = hi + 1; c <= n_nodes; ++c) start[c] = entries;
+}
+
+__global__ void adj_chunks_kernel(const int* start, int n_nodes, int* chunks) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n_nodes) chunks[c] = (start[c + 1] - start[c] + kUsersPerItem - 1) / kUsersPerItem;
+  if (c == n_nodes) chunks[c] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -317,7 +307,7 @@ __global__ __launch_bounds__(64) void adj_cells_kernel(AdjArgs a) {
   __shared__ int u_start[kUsersPerItem];
   __shared__ int u_oid[kUsersPerItem];
   const int lane = threadIdx.x;
-  const int n_items = a.tr.nItems[0];
+  const int n_items = a.tr.item[a.nNodes];
   const float* wire = reinterpret_cast<const float*>(a.pc.moments);
   const double L = a.pc.fperiod;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
@@ -340,10 +330,11 @@ __global__ __launch_bounds__(64) void adj_cells_kernel(AdjArgs a) {
       const int k = k0 + lane;
       int sz = 0;
       if (k < nu) {
-        const int2 u = a.tr.users[u0 + k];
-        sz = a.pc.sizes[u.x];
-        u_start[k] = a.pc.starts[u.x];
-        u_oid[k] = u.y;
+        const unsigned long long u = a.tr.value[u0 + k];
+        const int bk = (int)(u >> 32);
+        sz = a.pc.sizes[bk];
+        u_start[k] = a.pc.starts[bk];
+        u_oid[k] = (int)(unsigned)(u & 0xffffffffull);
       }
       const int incl = wave_incl_scan(sz, lane);
       if (k < nu) u_end[k] = base + incl;
@@ -379,9 +370,7 @@ __global__ __launch_bounds__(64) void adj_cells_kernel(AdjArgs a) {
             vs2 -= cmass * (bb * wz + cc * rz * dw);
           } else {
             const float ir = rsq(r2);
-            RinvDer T;
-            rinv_derivs6(rx, ry, rz, ir, ir * ir, T);
-            accumulate_d(T, wx, wy, wz, D);
+            accumulate_d(rx, ry, rz, ir, ir * ir, wx, wy, wz, D);
           }
         }
       }
@@ -406,15 +395,26 @@ __global__ __launch_bounds__(64) void adj_cells_kernel(AdjArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// (3) target side of cells and both sides of particle pairs, per bucket
+// (3) target side of cells and particles, and source side of particles, per
+// bucket. Target sides run in the forward kernels' layout (slots of targets
+// over LDS-staged sources, sums in registers); the particle source side puts
+// one source on each lane with the targets broadcast.
 // ---------------------------------------------------------------------------
 template <int KIND>
+struct TargetsLds {
+  static constexpr size_t pc = (size_t)kAdjPcTile * kPcRec * sizeof(float);
+  static constexpr size_t pp = sizeof(PpLds<kPpTile>);
+  static constexpr size_t bytes = pc > pp ? pc : pp;
+  static constexpr size_t f4 = (bytes + 15) / 16;
+};
+
+template <int KIND>
 __global__ __launch_bounds__(64) void adj_targets_kernel(AdjArgs a) {
-  __shared__ float4 rec4[kAdjPcTile * (kPcRec / 4)];
-  __shared__ float4 tp4[kWave];   // target x, y, z (bucket-centred), softening
-  __shared__ float4 tw4[kWave];   // w
-  __shared__ double tacc[3][kWave];
-  __shared__ int r_idx[kWave], r_oid[kWave], r_end[kWave];
+  __shared__ float4 lds[TargetsLds<KIND>::f4];
+  __shared__ float4 tp4[kWave];  // target x, y, z (bucket-centred), softening
+  __shared__ float4 tw4[kWave];  // w
+  float4* rec4 = lds;
+  PpLds<kPpTile>& ps = *reinterpret_cast<PpLds<kPpTile>*>(lds);
 
   const int lane = threadIdx.x;
   const int b = xcd_bucket(blockIdx.x, a.pc.numBuckets);
@@ -438,15 +438,12 @@ __global__ __launch_bounds__(64) void adj_targets_kernel(AdjArgs a) {
       tw4[lane] = make_float4(a.w[3 * (size_t)tp], a.w[3 * (size_t)tp + 1],
                               a.w[3 * (size_t)tp + 2], 0.0f);
     }
-    tacc[0][lane] = 0.0;
-    tacc[1][lane] = 0.0;
-    tacc[2][lane] = 0.0;
     __syncthreads();
     const int j = l.live ? l.lane - l.slot * l.nt : 0;
     const float4 me = tp4[j], mw = tw4[j];
-
-    // ---- cells: slots of targets over LDS-staged tiles
     double ax = 0.0, ay = 0.0, az = 0.0;
+
+    // ---- cells, target side
     WireCell wc;
     wc.oid = 0;
     load_cell<true>(a.pc, cells, pcs + lane, pce, wc);
@@ -485,57 +482,58 @@ __global__ __launch_bounds__(64) void adj_targets_kernel(AdjArgs a) {
       __syncthreads();
     }
 
-    // ---- particles: one source per lane, targets broadcast
-    auto source_pass = [&](int p, int oid, bool live) {
-      float sx = 0.f, sy = 0.f, sz = 0.f, sm = 0.f, ss = 0.f;
-      if (live) {
-        const CompactPartData q = a.pp.sources[p];
-        sx = (float)(a.pp.spos64[3 * (size_t)p] + (double)off_x(oid) * (double)L - cx0);
-        sy = (float)(a.pp.spos64[3 * (size_t)p + 1] + (double)off_y(oid) * (double)L - cy0);
-        sz = (float)(a.pp.spos64[3 * (size_t)p + 2] + (double)off_z(oid) * (double)L - cz0);
-        sm = q.mass;
-        ss = q.soft;
+    // ---- particles, target side: -m_s (b w - c d (d.w)), d = x_s - x_i
+    pp_tiles<true, KIND, kPpTile>(a.pp, pps, ppe, cx0, cy0, cz0, lane, ps, [&](int ns) {
+      if (l.live) {
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        for (int k = l.slot; k < ns; k += l.nslot) {
+          const float4 v = ps.src4[k];
+          const float dx = v.x - me.x, dy = v.y - me.y, dz = v.z - me.z;
+          const float r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 == 0.0f) continue;
+          float bb, cc;
+          spline_bc(r2, me.w + ps.srcs[k], bb, cc);
+          const float dw = dx * mw.x + dy * mw.y + dz * mw.z;
+          sx -= v.w * (bb * mw.x - cc * dx * dw);
+          sy -= v.w * (bb * mw.y - cc * dy * dw);
+          sz -= v.w * (bb * mw.z - cc * dz * dw);
+        }
+        ax += sx;
+        ay += sy;
+        az += sz;
       }
+    });
+
+    // ---- particles, source side: one source per lane
+    auto source_pass = [&](int p, int oid, bool live) {
+      if (!live) return;
+      const CompactPartData q = a.pp.sources[p];
+      const float sx = (float)(a.pp.spos64[3 * (size_t)p] + (double)off_x(oid) * (double)L - cx0);
+      const float sy =
+          (float)(a.pp.spos64[3 * (size_t)p + 1] + (double)off_y(oid) * (double)L - cy0);
+      const float sz =
+          (float)(a.pp.spos64[3 * (size_t)p + 2] + (double)off_z(oid) * (double)L - cz0);
+      const float sm = q.mass, ss = q.soft;
       float gx = 0.f, gy = 0.f, gz = 0.f, gm = 0.f;
       for (int jj = 0; jj < nt; ++jj) {
         const float4 t = tp4[jj], wv = tw4[jj];
-        float tx = 0.f, ty = 0.f, tz = 0.f;
-        if (live) {
-          const float dx = sx - t.x, dy = sy - t.y, dz = sz - t.z;
-          const float r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 != 0.0f) {
-            float bb, cc;
-            spline_bc(r2, t.w + ss, bb, cc);
-            const float dw = dx * wv.x + dy * wv.y + dz * wv.z;
-            const float jx = bb * wv.x - cc * dx * dw, jy = bb * wv.y - cc * dy * dw,
-                        jz = bb * wv.z - cc * dz * dw;
-            tx = -sm * jx;
-            ty = -sm * jy;
-            tz = -sm * jz;
-            gx += sm * jx;
-            gy += sm * jy;
-            gz += sm * jz;
-            gm += bb * dw;
-          }
-        }
-        tx = wave_sum(tx);
-        ty = wave_sum(ty);
-        tz = wave_sum(tz);
-        if (lane == 0) {
-          tacc[0][jj] += tx;
-          tacc[1][jj] += ty;
-          tacc[2][jj] += tz;
-        }
+        const float dx = sx - t.x, dy = sy - t.y, dz = sz - t.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 == 0.0f) continue;
+        float bb, cc;
+        spline_bc(r2, t.w + ss, bb, cc);
+        const float dw = dx * wv.x + dy * wv.y + dz * wv.z;
+        gx += sm * (bb * wv.x - cc * dx * dw);
+        gy += sm * (bb * wv.y - cc * dy * dw);
+        gz += sm * (bb * wv.z - cc * dz * dw);
+        gm += bb * dw;
       }
-      if (live) {
-        double* g = a.grad + 4 * (size_t)p;
-        atomicAdd(g, (double)gx);
-        atomicAdd(g + 1, (double)gy);
-        atomicAdd(g + 2, (double)gz);
-        atomicAdd(g + 3, (double)gm);
-      }
+      double* g = a.grad + 4 * (size_t)p;
+      atomicAdd(g, (double)gx);
+      atomicAdd(g + 1, (double)gy);
+      atomicAdd(g + 2, (double)gz);
+      atomicAdd(g + 3, (double)gm);
     };
-
     if (KIND == 0) {
       const ILCell* lst = static_cast<const ILCell*>(a.pp.list);
       for (int base = pps; base < ppe; base += kWave) {
@@ -551,12 +549,12 @@ __global__ __launch_bounds__(64) void adj_targets_kernel(AdjArgs a) {
         int num = 0;
         if (lane < nr) {
           ILPart q = lst[rb + lane];
-          r_idx[lane] = q.index;
-          r_oid[lane] = q.off;
+          ps.r_idx[lane] = q.index;
+          ps.r_oid[lane] = q.off;
           num = q.num;
         }
         const int incl = wave_incl_scan(num, lane);
-        r_end[lane] = incl;
+        ps.r_end[lane] = incl;
         const int total = __shfl(incl, kWave - 1, kWave);
         __syncthreads();
         for (int sb = 0; sb < total; sb += kWave) {
@@ -564,9 +562,9 @@ __global__ __launch_bounds__(64) void adj_targets_kernel(AdjArgs a) {
           const bool live = q < total;
           int p = 0, oid = 0;
           if (live) {
-            const int k = find_run(r_end, nr, q);
-            p = r_idx[k] + (q - (k ? r_end[k - 1] : 0));
-            oid = r_oid[k];
+            const int k = find_run(ps.r_end, nr, q);
+            p = ps.r_idx[k] + (q - (k ? ps.r_end[k - 1] : 0));
+            oid = ps.r_oid[k];
           }
           source_pass(p, oid, live);
         }
@@ -574,21 +572,22 @@ __global__ __launch_bounds__(64) void adj_targets_kernel(AdjArgs a) {
       }
     }
 
-    // ---- per target: slots of the cell part + the particle part
+    // ---- per target: sum of the slots
     __syncthreads();
-    double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(rec4);
+    double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(lds);
     red[0][lane] = l.live ? ax : 0.0;
     red[1][lane] = l.live ? ay : 0.0;
     red[2][lane] = l.live ? az : 0.0;
     __syncthreads();
     if (lane < nt) {
-      double s0 = tacc[0][lane], s1 = tacc[1][lane], s2 = tacc[2][lane];
-      for (int s = 0; s < l.nslot; ++s) {
-        s0 += red[0][s * nt + lane];
-        s1 += red[1][s * nt + lane];
-        s2 += red[2][s * nt + lane];
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+      for (int sl = 0; sl < l.nslot; ++sl) {
+        s0 += red[0][sl * nt + lane];
+        s1 += red[1][sl * nt + lane];
+        s2 += red[2][sl * nt + lane];
       }
-      double* g = a.grad + 4 * (size_t)(bstart + t0 + lane);
+      const int tp = bstart + t0 + lane;
+      double* g = a.grad + 4 * (size_t)(a.tindex ? a.tindex[tp] : tp);
       atomicAdd(g, s0);
       atomicAdd(g + 1, s1);
       atomicAdd(g + 2, s2);
@@ -601,116 +600,127 @@ __global__ __launch_bounds__(64) void adj_targets_kernel(AdjArgs a) {
 // (4, 5) tree passes (FP64). Monomials of degree <= 5 are indexed degree by
 // degree, a descending, then b descending.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int mono(int a, int b, int c) {
-  const int n = a + b + c;
-  return n * (n + 1) * (n + 2) / 6 + (n - a) * (n - a + 1) / 2 + (n - a - b);
+__host__ __device__ constexpr int mono(int a, int b, int c) {
+  return (a + b + c) * (a + b + c + 1) * (a + b + c + 2) / 6 +
+         (b + c) * (b + c + 1) / 2 + c;
+}
+__host__ __device__ constexpr double inv_fact(int n) {
+  return n <= 1 ? 1.0 : n == 2 ? 0.5 : n == 3 ? 1.0 / 6.0 : n == 4 ? 1.0 / 24.0 : 1.0 / 120.0;
 }
 
-__constant__ int kRedAbc[22][3] = {
-    {0, 0, 0}, {2, 0, 0}, {0, 2, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {3, 0, 0}, {1, 2, 0},
-    {2, 1, 0}, {0, 3, 0}, {2, 0, 1}, {0, 2, 1}, {1, 1, 1}, {4, 0, 0}, {1, 3, 0}, {3, 1, 0},
-    {0, 4, 0}, {3, 0, 1}, {0, 3, 1}, {2, 2, 0}, {2, 1, 1}, {1, 2, 1}};
-__constant__ double kInvFact[6] = {1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0};
+__constant__ int kMonoAbc[35][3] = {
+    {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {1, 1, 0}, {1, 0, 1},
+    {0, 2, 0}, {0, 1, 1}, {0, 0, 2}, {3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0},
+    {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}, {4, 0, 0},
+    {3, 1, 0}, {3, 0, 1}, {2, 2, 0}, {2, 1, 1}, {2, 0, 2}, {1, 3, 0}, {1, 2, 1},
+    {1, 1, 2}, {1, 0, 3}, {0, 4, 0}, {0, 3, 1}, {0, 2, 2}, {0, 1, 3}, {0, 0, 4}};
 __constant__ double kBinom[5][5] = {{1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0},
                                     {1, 3, 3, 1, 0}, {1, 4, 6, 4, 1}};
 
-// trace-free completion in place over monomial slots of orders 2..nmax:
+// trace-free completion over monomial slots of orders 2..NMAX:
 // X(a,b,c>=2) = -X(a+2,b,c-2) - X(a,b+2,c-2)
-__device__ void complete_traceless(double* full, int nmax) {
-  for (int n = 2; n <= nmax; ++n)
+template <int NMIN, int NMAX>
+__device__ __forceinline__ void complete_traceless(double* f) {
+#pragma unroll
+  for (int n = NMIN; n <= NMAX; ++n)
+#pragma unroll
     for (int c = 2; c <= n; ++c)
+#pragma unroll
       for (int a = n - c; a >= 0; --a) {
         const int b = n - c - a;
-        full[mono(a, b, c)] = -full[mono(a + 2, b, c - 2)] - full[mono(a, b + 2, c - 2)];
+        f[mono(a, b, c)] = -f[mono(a + 2, b, c - 2)] - f[mono(a, b + 2, c - 2)];
       }
 }
 
-// Hhat of one node into P (35 coefficients)
-__device__ void node_hhat(const double* nd, const double* g64, double* P) {
-  double Df[56], Q[35];
+// Hhat of every node (one thread per node); the root's polynomial is its Hhat
+__global__ void adj_hhat_kernel(int n_nodes, double* node, const double* node64) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  double* nd = node + (size_t)n * kAdjStride;
+  const double* g = node64 + (size_t)n * 34;
+  double Df[56];
+#pragma unroll
   for (int q = 0; q < 56; ++q) Df[q] = 0.0;
-  for (int k = 0; k < 36; ++k) Df[mono(kD36[k][0], kD36[k][1], kD36[k][2])] = nd[k];
-  complete_traceless(Df, 5);
+#define CGPU_LOAD(k, a, b, c) Df[mono(a, b, c)] = nd[k];
+  CGPU_D36(CGPU_LOAD)
+#undef CGPU_LOAD
+  complete_traceless<2, 5>(Df);
+  // order-4 moments about cm (unscaled), completed
+  double Q[35];
+  const double u = g[3], u4 = (u * u) * (u * u);
+#pragma unroll
   for (int q = 0; q < 35; ++q) Q[q] = 0.0;
-  const double u = g64[3];
-  for (int k = 0; k < 22; ++k) {
-    const int a = kRedAbc[k][0], b = kRedAbc[k][1], c = kRedAbc[k][2];
-    double s = g64[12 + k];
-    for (int i = 0; i < a + b + c; ++i) s *= u;
-    Q[mono(a, b, c)] = s;
-  }
-  complete_traceless(Q, 4);
-  for (int q = 0; q < 35; ++q) P[q] = 0.0;
-  for (int n = 0; n <= 4; ++n) {
-    if (n == 1) continue;
-    for (int a = n; a >= 0; --a)
-      for (int b = n - a; b >= 0; --b) {
-        const int c = n - a - b;
-        P[mono(a, b, c)] =
-            ((n & 1) ? -1.0 : 1.0) * kInvFact[a] * kInvFact[b] * kInvFact[c] * Df[mono(a, b, c)];
+  Q[mono(4, 0, 0)] = u4 * g[12 + 13];
+  Q[mono(1, 3, 0)] = u4 * g[12 + 14];
+  Q[mono(3, 1, 0)] = u4 * g[12 + 15];
+  Q[mono(0, 4, 0)] = u4 * g[12 + 16];
+  Q[mono(3, 0, 1)] = u4 * g[12 + 17];
+  Q[mono(0, 3, 1)] = u4 * g[12 + 18];
+  Q[mono(2, 2, 0)] = u4 * g[12 + 19];
+  Q[mono(2, 1, 1)] = u4 * g[12 + 20];
+  Q[mono(1, 2, 1)] = u4 * g[12 + 21];
+  complete_traceless<4, 4>(Q);
+  double H[35];
+#pragma unroll
+  for (int m = 0; m <= 4; ++m)
+#pragma unroll
+    for (int a = m; a >= 0; --a)
+#pragma unroll
+      for (int b = m - a; b >= 0; --b) {
+        const int c = m - a - b;
+        H[mono(a, b, c)] = (m == 1) ? 0.0
+                                    : ((m & 1) ? -1.0 : 1.0) * inv_fact(a) * inv_fact(b) *
+                                          inv_fact(c) * Df[mono(a, b, c)];
       }
-  }
-  const double M = g64[5];
+  const double M = g[5];
+#pragma unroll
   for (int j = 0; j < 3; ++j) {
     double S = nd[kAdjVs + j];
+#pragma unroll
     for (int a = 4; a >= 0; --a)
+#pragma unroll
       for (int b = 4 - a; b >= 0; --b) {
         const int c = 4 - a - b;
-        S += kInvFact[a] * kInvFact[b] * kInvFact[c] * Q[mono(a, b, c)] *
+        S += inv_fact(a) * inv_fact(b) * inv_fact(c) * Q[mono(a, b, c)] *
              Df[mono(a + (j == 0), b + (j == 1), c + (j == 2))];
       }
-    const double U = Df[mono(j == 0, j == 1, j == 2)] + (M > 0.0 ? S / M : 0.0);
-    P[mono(j == 0, j == 1, j == 2)] = -U;
+    H[mono(j == 0, j == 1, j == 2)] = -(Df[mono(j == 0, j == 1, j == 2)] + (M > 0.0 ? S / M : 0.0));
   }
+#pragma unroll
+  for (int q = 0; q < 35; ++q) nd[kAdjHhat + q] = H[q];
+  if (n == 0)
+#pragma unroll
+    for (int q = 0; q < 35; ++q) nd[kAdjPoly + q] = H[q];
 }
 
-// P += Pp(s + d)
-__device__ void add_shifted(const double* Pp, const double d[3], double* P) {
-  double pw[3][5];
-  for (int k = 0; k < 3; ++k) {
-    pw[k][0] = 1.0;
-    for (int i = 1; i < 5; ++i) pw[k][i] = pw[k][i - 1] * d[k];
-  }
-  for (int n = 0; n <= 4; ++n)
-    for (int a = n; a >= 0; --a)
-      for (int b = n - a; b >= 0; --b) {
-        const int c = n - a - b;
-        const double p = Pp[mono(a, b, c)];
-        if (p == 0.0) continue;
-        for (int i = 0; i <= a; ++i)
-          for (int j = 0; j <= b; ++j)
-            for (int k = 0; k <= c; ++k)
-              P[mono(i, j, k)] += p * kBinom[a][i] * kBinom[b][j] * kBinom[c][k] *
-                                  pw[0][a - i] * pw[1][b - j] * pw[2][c - k];
-      }
+__device__ __forceinline__ double ipow_small(double x, int e) {
+  double r = 1.0;
+  for (int i = 0; i < e; ++i) r *= x;
+  return r;
 }
 
-__global__ void adjoint_root_kernel(double* node, const double* node64) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double P[35];
-    node_hhat(node, node64, P);
-    for (int q = 0; q < 35; ++q) node[kAdjPoly + q] = P[q];
+// children of the nodes [lo, lo + gridDim.x / 2): one wave per (parent,
+// child slot), lane q computes coefficient q of
+// P_child(s) = Hhat_child(s) + P_parent(s + cm_child - cm_parent)
+__global__ __launch_bounds__(64) void adj_shift_kernel(int lo, const int* child, double* node,
+                                                       const double* node64) {
+  const int parent = lo + (int)(blockIdx.x >> 1);
+  const int c = child[2 * parent + (blockIdx.x & 1)];
+  const int q = threadIdx.x;
+  if (c < 0 || q >= 35) return;
+  const double* gp = node64 + (size_t)parent * 34;
+  const double* gc = node64 + (size_t)c * 34;
+  const double d0 = gc[0] - gp[0], d1 = gc[1] - gp[1], d2 = gc[2] - gp[2];
+  const int i = kMonoAbc[q][0], j = kMonoAbc[q][1], k = kMonoAbc[q][2];
+  const double* Pp = node + (size_t)parent * kAdjStride + kAdjPoly;
+  double sum = node[(size_t)c * kAdjStride + kAdjHhat + q];
+  for (int t = q; t < 35; ++t) {
+    const int a = kMonoAbc[t][0], b = kMonoAbc[t][1], cc = kMonoAbc[t][2];
+    if (a < i || b < j || cc < k) continue;
+    sum += Pp[t] * kBinom[a][i] * kBinom[b][j] * kBinom[cc][k] * ipow_small(d0, a - i) *
+           ipow_small(d1, b - j) * ipow_small(d2, cc - k);
   }
-}
-
-// children of the nodes [lo, hi): P_child = Hhat_child + P_parent(s + cm_child - cm_parent)
-__global__ void adjoint_down_kernel(int lo, int hi, const int* child, double* node,
-                                    const double* node64) {
-  const int n = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= hi) return;
-  const double* gp = node64 + (size_t)n * 34;
-  const double* Pp = node + (size_t)n * kAdjStride + kAdjPoly;
-  for (int k = 0; k < 2; ++k) {
-    const int c = child[2 * n + k];
-    if (c < 0) continue;
-    const double* gc = node64 + (size_t)c * 34;
-    double* nc = node + (size_t)c * kAdjStride;
-    double P[35];
-    node_hhat(nc, gc, P);
-    const double d[3] = {gc[0] - gp[0], gc[1] - gp[1], gc[2] - gp[2]};
-    add_shifted(Pp, d, P);
-    for (int q = 0; q < 35; ++q) nc[kAdjPoly + q] = P[q];
-  }
+  node[(size_t)c * kAdjStride + kAdjPoly + q] = sum;
 }
 
 // bucket particles: dL/dm += P(s), dL/dx += m grad P(s)
@@ -721,20 +731,27 @@ __global__ void adjoint_particles_kernel(int n_nodes, const int* child, const in
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= n_nodes || child[2 * n] >= 0) return;
   const double* g = node64 + (size_t)n * 34;
-  const double* P = node + (size_t)n * kAdjStride + kAdjPoly;
+  double P[35];
+#pragma unroll
+  for (int q = 0; q < 35; ++q) P[q] = node[(size_t)n * kAdjStride + kAdjPoly + q];
   for (int k = first[n]; k < first[n] + count[n]; ++k) {
     const double s[3] = {pos64[3 * (size_t)k] - g[0], pos64[3 * (size_t)k + 1] - g[1],
                          pos64[3 * (size_t)k + 2] - g[2]};
     double pw[3][5];
-    for (int q = 0; q < 3; ++q) {
-      pw[q][0] = 1.0;
-      for (int i = 1; i < 5; ++i) pw[q][i] = pw[q][i - 1] * s[q];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      pw[t][0] = 1.0;
+#pragma unroll
+      for (int e = 1; e < 5; ++e) pw[t][e] = pw[t][e - 1] * s[t];
     }
     double val = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
-    for (int nn = 0; nn <= 4; ++nn)
-      for (int a = nn; a >= 0; --a)
-        for (int b = nn - a; b >= 0; --b) {
-          const int c = nn - a - b;
+#pragma unroll
+    for (int m = 0; m <= 4; ++m)
+#pragma unroll
+      for (int a = m; a >= 0; --a)
+#pragma unroll
+        for (int b = m - a; b >= 0; --b) {
+          const int c = m - a - b;
           const double p = P[mono(a, b, c)];
           val += p * pw[0][a] * pw[1][b] * pw[2][c];
           if (a) gx += p * a * pw[0][a - 1] * pw[1][b] * pw[2][c];
@@ -762,13 +779,58 @@ static int zero(double* p, size_t n, hipStream_t s) {
   return 0;
 }
 
+// Scratch layout of listWork (offsets in ints, 256-byte aligned segments).
+struct WorkLayout {
+  size_t key, key_out, val, val_out, start, chunks, item, temp, total;
+  size_t temp_bytes;
+  unsigned end_bit;
+};
+
+static size_t align64(size_t x) { return (x + 63) & ~(size_t)63; }
+
+static hipError_t work_layout(int n_nodes, int entries, WorkLayout& L) {
+  L.end_bit = 1;
+  while ((1u << L.end_bit) < (unsigned)n_nodes && L.end_bit < 32) ++L.end_bit;
+  size_t sort_bytes = 0, scan_bytes = 0;
+  hipError_t e = rocprim::radix_sort_pairs(
+      nullptr, sort_bytes, (const unsigned*)nullptr, (unsigned*)nullptr,
+      (const unsigned long long*)nullptr, (unsigned long long*)nullptr, (size_t)entries, 0,
+      L.end_bit);
+  if (e != hipSuccess) return e;
+  e = rocprim::exclusive_scan(nullptr, scan_bytes, (const int*)nullptr, (int*)nullptr, 0,
+                              (size_t)n_nodes + 1, rocprim::plus<int>());
+  if (e != hipSuccess) return e;
+  L.temp_bytes = sort_bytes > scan_bytes ? sort_bytes : scan_bytes;
+  size_t o = 0;
+  L.key = o;
+  o = align64(o + (size_t)entries);
+  L.key_out = o;
+  o = align64(o + (size_t)entries);
+  L.val = o;
+  o = align64(o + 2 * (size_t)entries);
+  L.val_out = o;
+  o = align64(o + 2 * (size_t)entries);
+  L.start = o;
+  o = align64(o + (size_t)n_nodes + 1);
+  L.chunks = o;
+  o = align64(o + (size_t)n_nodes + 1);
+  L.item = o;
+  o = align64(o + (size_t)n_nodes + 1);
+  L.temp = o;
+  o = align64(o + (L.temp_bytes + 3) / 4);
+  L.total = o;
+  return hipSuccess;
+}
+
 }  // namespace cgpu
 
 using namespace cgpu;
 
 extern "C" size_t changaGravityAdjointWorkInts(int nNodes, int numCellEntries) {
   if (nNodes <= 0 || numCellEntries < 0) return 0;
-  return 4 * (size_t)nNodes + 2 + 1 + 2 * (size_t)numCellEntries + 8;
+  WorkLayout L;
+  if (work_layout(nNodes, numCellEntries, L) != hipSuccess) return 0;
+  return L.total;
 }
 
 extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) {
@@ -776,9 +838,6 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
       !a->count || !a->child || !a->node64 || !a->pos64 || !a->mass64 || a->nNodes <= 0 ||
       a->nDepths <= 0 || a->nParticles <= 0 || a->numCellEntries < 0)
     return fail("changaGravityAdjoint: missing argument", hipErrorInvalidValue, __FILE__, __LINE__);
-  if (a->listWorkInts < changaGravityAdjointWorkInts(a->nNodes, a->numCellEntries))
-    return fail("changaGravityAdjoint: listWork too small", hipErrorInvalidValue, __FILE__,
-                __LINE__);
   const ChangaGravityLaunch& pc = a->cells;
   const ChangaGravityLaunch& pp = a->parts;
   if (!pc.marks || !pc.starts || !pc.sizes || !pc.targets || !pc.moments || !pc.list ||
@@ -787,37 +846,58 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
       pc.fperiod != pp.fperiod || pc.listKind != 0 || (pp.listKind != 0 && pp.listKind != 1))
     return fail("changaGravityAdjoint: cell/particle launches (precision form, same buckets)",
                 hipErrorInvalidValue, __FILE__, __LINE__);
+  WorkLayout W;
+  CGPU_CHECK(work_layout(a->nNodes, a->numCellEntries, W));
+  if (a->listWorkInts < W.total)
+    return fail("changaGravityAdjoint: listWork too small", hipErrorInvalidValue, __FILE__,
+                __LINE__);
   hipStream_t s = (hipStream_t)stream;
-  const int nn = a->nNodes;
+  const int nn = a->nNodes, nb = pc.numBuckets, E = a->numCellEntries;
   if (int e = zero(a->grad, 4 * (size_t)a->nParticles, s)) return e;
   if (int e = zero(a->nodeWork, (size_t)kAdjStride * nn, s)) return e;
+  int* iw = a->listWork;
+  unsigned* key = reinterpret_cast<unsigned*>(iw + W.key);
+  unsigned* key_out = reinterpret_cast<unsigned*>(iw + W.key_out);
+  unsigned long long* val = reinterpret_cast<unsigned long long*>(iw + W.val);
+  unsigned long long* val_out = reinterpret_cast<unsigned long long*>(iw + W.val_out);
   AdjArgs args;
   args.pc = pc;
   args.pp = pp;
   args.w = a->w;
+  args.tindex = a->targetIndex;
   args.grad = a->grad;
   args.node = a->nodeWork;
   args.nNodes = nn;
-  int* iw = a->listWork;
-  args.tr.count = iw;
-  args.tr.cursor = iw + nn;
-  args.tr.start = iw + 2 * (size_t)nn;
-  args.tr.item = iw + 3 * (size_t)nn + 1;
-  args.tr.nItems = iw + 4 * (size_t)nn + 2;
-  args.tr.users = reinterpret_cast<int2*>(iw + 4 * (size_t)nn + 4);  // 8-byte aligned
-  const int nb = pc.numBuckets;
-  hipLaunchKernelGGL(adj_zero_int_kernel, dim3((2 * nn + 255) / 256), dim3(256), 0, s, iw,
-                     2 * nn);
-  CGPU_CHECK(hipGetLastError());
+  args.entries = E;
+  args.tr.key = key_out;
+  args.tr.value = val_out;
+  args.tr.start = iw + W.start;
+  args.tr.chunks = iw + W.chunks;
+  args.tr.item = iw + W.item;
+  // (1) transpose: entries sorted by cell, stable (bucket order within a cell)
   if (nb > 0) {
-    hipLaunchKernelGGL(adj_count_kernel, dim3(nb), dim3(kWave), 0, s, args);
+    hipLaunchKernelGGL(adj_keys_kernel, dim3(nb), dim3(kWave), 0, s, pc, key, val);
     CGPU_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL(adj_scan_kernel, dim3(1), dim3(1024), 0, s, args);
-  CGPU_CHECK(hipGetLastError());
-  if (nb > 0) {
-    hipLaunchKernelGGL(adj_scatter_kernel, dim3(nb), dim3(kWave), 0, s, args);
+  if (E > 0) {
+    size_t tb = W.temp_bytes;
+    CGPU_CHECK(rocprim::radix_sort_pairs(iw + W.temp, tb, key, key_out, val, val_out, (size_t)E,
+                                         0, W.end_bit, s));
+  }
+  {
+    const int n = E > nn + 1 ? E : nn + 1;
+    hipLaunchKernelGGL(adj_bounds_kernel, dim3((n + 255) / 256), dim3(256), 0, s, key_out, E, nn,
+                       args.tr.start);
     CGPU_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(adj_chunks_kernel, dim3((nn + 1 + 255) / 256), dim3(256), 0, s,
+                       args.tr.start, nn, args.tr.chunks);
+    CGPU_CHECK(hipGetLastError());
+    size_t tb = W.temp_bytes;
+    CGPU_CHECK(rocprim::exclusive_scan(iw + W.temp, tb, args.tr.chunks, args.tr.item, 0,
+                                       (size_t)nn + 1, rocprim::plus<int>(), s));
+  }
+  // (2) cell sources, (3) target side and particle pairs
+  if (nb > 0) {
     int dev = 0, ncu = 0;
     CGPU_CHECK(hipGetDevice(&dev));
     CGPU_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -829,15 +909,18 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
       hipLaunchKernelGGL(adj_targets_kernel<0>, dim3(nb), dim3(kWave), 0, s, args);
     CGPU_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL(adjoint_root_kernel, dim3(1), dim3(1), 0, s, a->nodeWork, a->node64);
+  // (4) Hhat per node, then top-down
+  hipLaunchKernelGGL(adj_hhat_kernel, dim3((nn + 127) / 128), dim3(128), 0, s, nn, a->nodeWork,
+                     a->node64);
   CGPU_CHECK(hipGetLastError());
   for (int d = 0; d < a->nDepths; ++d) {
     const int lo = a->levelOffsets[d], hi = a->levelOffsets[d + 1];
     if (hi <= lo) continue;
-    hipLaunchKernelGGL(adjoint_down_kernel, dim3((hi - lo + 127) / 128), dim3(128), 0, s, lo, hi,
-                       a->child, a->nodeWork, a->node64);
+    hipLaunchKernelGGL(adj_shift_kernel, dim3(2 * (hi - lo)), dim3(kWave), 0, s, lo, a->child,
+                       a->nodeWork, a->node64);
     CGPU_CHECK(hipGetLastError());
   }
+  // (5) particles
   hipLaunchKernelGGL(adjoint_particles_kernel, dim3((nn + 127) / 128), dim3(128), 0, s, nn,
                      a->child, a->first, a->count, a->nodeWork, a->node64, a->pos64, a->mass64,
                      a->grad);

@@ -373,16 +373,17 @@ struct PpLds {
   int r_idx[kWave], r_oid[kWave], r_end[kWave];
 };
 
-// Sources [start, end) of a.list onto the chunk's targets. KIND 0: one ILCell
-// per source particle (the list ChaNGa's GenericList<ILPart>::serialize
+// Tiles of the sources [start, end) of a.list staged into LDS (s.src4: x, y,
+// z relative to the bucket centre with the replica offset applied, mass;
+// s.srcs: softening); consume(ns) runs after each tile is staged. KIND 0: one
+// ILCell per source particle (the list ChaNGa's GenericList<ILPart>::serialize
 // produces, Compute.cpp:1118-1218); KIND 1: ILPart runs expanded here through
 // a wavefront prefix sum.
-template <bool PRECISE, int KIND, int TILE>
-__device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int start, int end,
-                                           double cx0, double cy0, double cz0, const Target& t,
-                                           const Lanes& l, PpLds<TILE>& s, Acc64& acc) {
+template <bool PRECISE, int KIND, int TILE, typename Consume>
+__device__ __forceinline__ void pp_tiles(const ChangaGravityLaunch& a, int start, int end,
+                                         double cx0, double cy0, double cz0, int lane,
+                                         PpLds<TILE>& s, Consume&& consume) {
   const float L = a.fperiod;
-  const int lane = l.lane;
 
   // stage source particle p (replica oid) into tile slot i
   auto stage = [&](int i, int p, int oid) {
@@ -400,21 +401,6 @@ __device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int sta
     v.w = q.mass;
     s.src4[i] = v;
     s.srcs[i] = q.soft;
-  };
-
-  auto consume = [&](int ns) {
-    if (l.live) {
-      Acc32 A = {0.f, 0.f, 0.f, 0.f, acc.idt};
-      for (int k = l.slot; k < ns; k += l.nslot) {
-        float4 v = s.src4[k];
-        pp_interact(v.x - t.x, v.y - t.y, v.z - t.z, v.w, s.srcs[k], t.m, t.soft, A);
-      }
-      acc.ax += A.ax;
-      acc.ay += A.ay;
-      acc.az += A.az;
-      acc.pot += A.pot;
-      acc.idt = A.idt;
-    }
   };
 
   if (KIND == 0) {
@@ -470,6 +456,27 @@ __device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int sta
       }
     }
   }
+}
+
+// Sources [start, end) of a.list onto the chunk's targets (partBucketForce).
+template <bool PRECISE, int KIND, int TILE>
+__device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int start, int end,
+                                           double cx0, double cy0, double cz0, const Target& t,
+                                           const Lanes& l, PpLds<TILE>& s, Acc64& acc) {
+  pp_tiles<PRECISE, KIND, TILE>(a, start, end, cx0, cy0, cz0, l.lane, s, [&](int ns) {
+    if (l.live) {
+      Acc32 A = {0.f, 0.f, 0.f, 0.f, acc.idt};
+      for (int k = l.slot; k < ns; k += l.nslot) {
+        float4 v = s.src4[k];
+        pp_interact(v.x - t.x, v.y - t.y, v.z - t.z, v.w, s.srcs[k], t.m, t.soft, A);
+      }
+      acc.ax += A.ax;
+      acc.ay += A.ay;
+      acc.az += A.az;
+      acc.pot += A.pot;
+      acc.idt = A.idt;
+    }
+  });
 }
 
 }  // namespace cgpu

@@ -63,11 +63,32 @@ def ewald_real_counts(pos, cm, L, n_reps, ew_cut, chunk=1 << 18):
     return out
 
 
+def slice_lists(lists, buckets):
+    """The interaction lists of a subset of buckets (in the given order),
+    with re-based markers."""
+    out = {}
+    buckets = np.asarray(buckets, dtype=np.int64)
+    for key, mkey, width in (("cells", "cell_marks", 2), ("parts", "part_marks", 3)):
+        lst, marks = lists[key].reshape(-1, width), np.asarray(lists[mkey], dtype=np.int64)
+        lens = marks[buckets + 1] - marks[buckets]
+        new_marks = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        idx = np.repeat(marks[buckets] - new_marks[:-1], lens) + np.arange(new_marks[-1])
+        out[key] = np.ascontiguousarray(lst[idx].reshape(-1))
+        out[mkey] = new_marks
+    return out
+
+
 class Evaluation:
     """Tree, lists and device state for one workload (see configs.py).
 
     precise: FP64 bucket-centre subtraction form; pp_compact: ILPart lists
-    expanded on the GPU (else ChaNGa's per-particle ILCell lists)."""
+    expanded on the GPU (else ChaNGa's per-particle ILCell lists).
+
+    Targets: every particle, or, when the workload carries an `active` mask
+    (a rung), only the active ones: they are gathered bucket by bucket into
+    their own target arrays (CompactPartData, FP64 positions, results), and
+    only buckets holding an active particle keep their lists. Sources are
+    always every particle."""
 
     def __init__(self, w, precise=False, pp_compact=True, n_threads=0, device="cuda:0"):
         self.w = w
@@ -91,18 +112,45 @@ class Evaluation:
         self.lists = host.walk(tr, self.geo, w["theta"], w["n_replicas"],
                                self.L if periodic else 1.0, n_threads)
         self.counts = host.interaction_counts(tr, self.lists, self.active)
+        self._targets()
         self._upload()
+
+    def _targets(self):
+        tr = self.tree
+        if self.active is None:
+            self.t_index = None
+            self.n_targets = self.pos.shape[0]
+            self.t_starts, self.t_sizes = tr["bucket_start"], tr["bucket_size"]
+            self.t_lists = self.lists
+            return
+        nb = tr["n_buckets"]
+        owner = np.repeat(np.arange(nb), tr["bucket_size"])
+        tidx = np.nonzero(self.active)[0]
+        counts = np.bincount(owner[tidx], minlength=nb)
+        keep = np.nonzero(counts > 0)[0]
+        self.t_index = tidx.astype(np.int32)
+        self.n_targets = len(tidx)
+        self.t_sizes = np.ascontiguousarray(counts[keep].astype(np.int32))
+        self.t_starts = np.ascontiguousarray(
+            np.concatenate([[0], np.cumsum(self.t_sizes)[:-1]]).astype(np.int32))
+        self.t_lists = slice_lists(self.lists, keep)
 
     # ------------------------------------------------------------------ device
     def _t(self, a, dtype=None):
         return torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
 
     def _upload(self):
-        tr, L = self.tree, self.lists
-        n = self.pos.shape[0]
+        tr, L = self.tree, self.t_lists
         self.d_parts = self._t(compact_parts(self.pos, self.mass, self.soft))
-        self.d_vars = torch.zeros((n, 5), dtype=torch.float32, device=self.device)
         self.d_pos64 = self._t(self.pos)
+        if self.t_index is None:
+            self.d_targets, self.d_tpos64, self.d_tindex = self.d_parts, self.d_pos64, None
+        else:
+            ti = self.t_index
+            self.d_targets = self._t(compact_parts(self.pos[ti], self.mass[ti], self.soft[ti]))
+            self.d_tpos64 = self._t(self.pos[ti])
+            self.d_tindex = self._t(ti)
+        self.d_vars = torch.zeros((self.n_targets, 5), dtype=torch.float32, device=self.device)
         self.d_mass64 = self._t(self.mass)
         self.d_soft64 = self._t(self.soft)
         self.d_first = self._t(tr["node_first"])
@@ -117,8 +165,8 @@ class Evaluation:
             raise ValueError("cell list exceeds the int32 marker range of one request")
         self.d_cells = self._t(L["cells"])
         self.d_cmarks = self._t(L["cell_marks"].astype(np.int32))
-        self.d_starts = self._t(tr["bucket_start"])
-        self.d_sizes = self._t(tr["bucket_size"])
+        self.d_starts = self._t(self.t_starts)
+        self.d_sizes = self._t(self.t_sizes)
         if self.pp_compact:
             self.d_plist = self._t(L["parts"])
             self.d_pmarks = self._t(L["part_marks"].astype(np.int32))
@@ -161,13 +209,8 @@ class Evaluation:
             setattr(ro.momcRoot, f, float(v))
         L = self.L
         alpha = 2.0 / L
-        if self.active is None:
-            self.ew_markers = None
-            ro.n = self.pos.shape[0]
-        else:
-            idx = np.nonzero(self.active)[0].astype(np.int32)
-            self.ew_markers = self._t(idx)
-            ro.n = len(idx)
+        self.ew_markers = None
+        ro.n = self.n_targets
         ro.nReps = w["n_replicas"]
         ro.nEwReps = int(math.ceil(w["ew_cut"]))
         ro.nEwhLoop = len(self.ewt64)
@@ -197,7 +240,7 @@ class Evaluation:
     # ------------------------------------------------------------------ launches
     def _launch(self, pc):
         a = gpu.ChangaGravityLaunch()
-        a.targets = self.d_parts.data_ptr()
+        a.targets = self.d_targets.data_ptr()
         a.vars = self.d_vars.data_ptr()
         a.moments = self.d_moments.data_ptr()
         a.sources = self.d_parts.data_ptr()
@@ -205,18 +248,18 @@ class Evaluation:
         a.marks = (self.d_cmarks if pc else self.d_pmarks).data_ptr()
         a.starts = self.d_starts.data_ptr()
         a.sizes = self.d_sizes.data_ptr()
-        a.numBuckets = self.tree["n_buckets"]
+        a.numBuckets = len(self.t_sizes)
         a.listKind = 0 if pc else (1 if self.pp_compact else 0)
         a.fperiod = self.L
         a.variant = self.pc_variant if pc else 0
         if self.precise:
-            a.tpos64 = self.d_pos64.data_ptr()
+            a.tpos64 = self.d_tpos64.data_ptr()
             a.spos64 = self.d_pos64.data_ptr()
             a.cm64 = self.d_cm64.data_ptr()
         return a
 
     def zero(self):
-        gpu.check(gpu.lib().changaZeroVars(_dp(self.d_vars), self.pos.shape[0], self._sp()),
+        gpu.check(gpu.lib().changaZeroVars(_dp(self.d_vars), self.n_targets, self._sp()),
                   "changaZeroVars")
 
     def run_pc(self):
@@ -230,9 +273,9 @@ class Evaluation:
     def run_ewald(self):
         if not self.w["periodic"]:
             return
-        pos64 = _dp(self.d_pos64) if self.precise else None
+        pos64 = _dp(self.d_tpos64) if self.precise else None
         cm = self.root_cm64.ctypes.data_as(ctypes.c_void_p) if self.precise else None
-        rc = gpu.lib().changaEwald(_dp(self.d_parts), _dp(self.d_vars),
+        rc = gpu.lib().changaEwald(_dp(self.d_targets), _dp(self.d_vars),
                                    _dp(self.ew_markers) if self.ew_markers is not None else None,
                                    0, self.ewald_ro.n, ctypes.byref(self.ewald_ro), _dp(self.d_ewt),
                                    pos64, cm, self._sp())
@@ -281,28 +324,31 @@ class Evaluation:
     # ------------------------------------------------------------------ adjoint form
     def adjoint(self, w):
         """dL/dx and dL/dm (tree order, [N, 4] FP64 on the device) of
-        L = sum_i w_i . a_i for this evaluation's lists (precision form), with
-        the moments of the last build. w: [N, 3] in tree order (array or
-        tensor)."""
+        L = sum_i w_i . a_i over the targets, for this evaluation's lists
+        (precision form), with the moments of the last build. w: [n_targets, 3]
+        in target order (tree order when every particle is a target)."""
         if not self.precise:
             raise ValueError("the adjoint runs on the precision form (FP64 positions)")
         n = self.pos.shape[0]
         wt = torch.as_tensor(w, dtype=torch.float32).to(self.device).contiguous()
-        if tuple(wt.shape) != (n, 3):
-            raise ValueError(f"w must be [{n}, 3]")
+        if tuple(wt.shape) != (self.n_targets, 3):
+            raise ValueError(f"w must be [{self.n_targets}, 3]")
         tr = self.tree
         grad = torch.empty((n, 4), dtype=torch.float64, device=self.device)
-        entries = int(self.lists["cell_marks"][-1])
+        entries = int(self.t_lists["cell_marks"][-1])
         if getattr(self, "d_node_work", None) is None:
-            self.d_node_work = torch.empty((tr["n_nodes"], 80), dtype=torch.float64,
+            self.d_node_work = torch.empty((tr["n_nodes"], 112), dtype=torch.float64,
                                            device=self.device)
             ints = gpu.lib().changaGravityAdjointWorkInts(tr["n_nodes"], entries)
+            if ints == 0:
+                raise RuntimeError("changaGravityAdjointWorkInts failed")
             self.d_list_work = torch.empty(ints, dtype=torch.int32, device=self.device)
         lo = np.ascontiguousarray(tr["level_offsets"], dtype=np.int32)
         a = gpu.ChangaAdjointLaunch()
         a.cells = self._launch(True)
         a.parts = self._launch(False)
         a.w = wt.data_ptr()
+        a.targetIndex = self.d_tindex.data_ptr() if self.d_tindex is not None else None
         a.grad = grad.data_ptr()
         a.nodeWork = self.d_node_work.data_ptr()
         a.listWork = self.d_list_work.data_ptr()
@@ -336,9 +382,14 @@ class Evaluation:
             pass
 
     def results(self):
-        """(acc[N,3], pot[N], dtgrav[N]) in tree order, FP32 as the wire."""
-        v = self.d_vars.cpu().numpy()
-        return v[:, 0:3].astype(np.float64), v[:, 3].astype(np.float64), v[:, 4].astype(np.float64)
+        """(acc[N,3], pot[N], dtgrav[N]) in tree order, FP32 as the wire;
+        zero for particles that are not targets."""
+        v = self.d_vars.cpu().numpy().astype(np.float64)
+        if self.t_index is not None:
+            full = np.zeros((self.pos.shape[0], 5))
+            full[self.t_index] = v
+            v = full
+        return v[:, 0:3], v[:, 3], v[:, 4]
 
     def node64(self):
         return self.d_node64.cpu().numpy()

@@ -76,7 +76,7 @@ class ChangaBatchOptions(ctypes.Structure):
 
 class ChangaAdjointLaunch(ctypes.Structure):
     _fields_ = [("cells", ChangaGravityLaunch), ("parts", ChangaGravityLaunch), ("w", c_void_p),
-                ("grad", c_void_p), ("nodeWork", c_void_p), ("listWork", c_void_p),
+                ("targetIndex", c_void_p), ("grad", c_void_p), ("nodeWork", c_void_p), ("listWork", c_void_p),
                 ("listWorkInts", c_size_t), ("numCellEntries", c_int), ("nNodes", c_int),
                 ("nDepths", c_int), ("nParticles", c_int), ("levelOffsets", c_void_p),
                 ("first", c_void_p), ("count", c_void_p), ("child", c_void_p),

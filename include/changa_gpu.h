@@ -212,9 +212,11 @@ int changaBatchPlanDestroy(ChangaBatchPlan *plan);
 typedef struct {
   ChangaGravityLaunch cells;      /* particle-cell launch (listKind 0, tpos64 and cm64 set) */
   ChangaGravityLaunch parts;      /* particle-particle launch (spos64 set), same buckets */
-  const float *w;                 /* dL/da, 3 per particle, tree order (device) */
+  const float *w;                 /* dL/da, 3 per target, in target order (device) */
+  const int *targetIndex;         /* particle index of each target (device), or NULL when the
+                                     targets are the particles themselves */
   double *grad;                   /* out, overwritten: dL/dx (3), dL/dm per particle (device) */
-  double *nodeWork;               /* scratch: 80 doubles per node (device) */
+  double *nodeWork;               /* scratch: 112 doubles per node (device) */
   int *listWork;                  /* scratch: changaGravityAdjointWorkInts(nNodes,
                                      numCellEntries) ints (device) */
   size_t listWorkInts;            /* its size in ints */

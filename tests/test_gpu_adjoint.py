@@ -62,15 +62,15 @@ def test_gpu_adjoint_matches_oracle(device, case):
 
 
 def test_gpu_adjoint_repeatable(device):
-    """The transposed cell lists are filled through atomics, so the order of
-    a cell's users, and with it the grouping of FP32 partial sums, varies
-    between runs: results agree to FP32 summation error."""
+    """The transposed cell lists come from a stable sort (a cell's users in
+    bucket order), so every FP32 partial sum is the same from run to run;
+    only the order of FP64 atomic additions varies."""
     w = configs.plummer(n=8192)
     ev = engine.Evaluation(w, precise=True, device=device)
     wv = np.random.default_rng(3).standard_normal(ev.pos.shape)
     g1 = ev.adjoint(wv).cpu().numpy()
     g2 = ev.adjoint(wv).cpu().numpy()
-    assert np.abs(g1 - g2).max() <= 1e-6 * np.abs(g1).max()
+    assert np.abs(g1 - g2).max() <= 1e-12 * np.abs(g1).max()
 
 
 def _oracle_acc(tree, lists, pos, mass, soft):
