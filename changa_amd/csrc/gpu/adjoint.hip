@@ -36,7 +36,9 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
+#include "ewald_body.h"
 #include "gravity_body.h"
+#include "stf.h"
 
 namespace cgpu {
 
@@ -767,6 +769,389 @@ __global__ void adjoint_particles_kernel(int n_nodes, const int* child, const in
   }
 }
 
+// ---------------------------------------------------------------------------
+// (6) Ewald term of L (periodic boxes). L_E = sum_t w_t . a_E(x_t - cm_root),
+// a_E the forward kernel's acceleration (ewald.hip: replicas with g_0..g_5,
+// then the h-table). Per target:
+//   - target side: the gradient of w . a_E, by dual numbers through
+//     ewald_term with dual radial kernels (D_w g_m = -g_(m+1) (r.w) for the
+//     closed form, 2 B_m (r.w) for the two-term series g_m = A_m + B_m r^2),
+//     and sum_h k (k.w)(hC cos + hS sin) for the k-terms;
+//   - source side, summed over targets: G_n = dL/dQ_n of the root moments,
+//     (1/n!) [g_n sum_j alpha_j w_j r^(alpha-e_j) - (w.r) g_(n+1) r^alpha] per
+//     replica; A_h = sum (k.w) sin, B_h = -sum (k.w) cos per h-entry (the
+//     h-table is linear in M, Q_2, Q_3, Q_4); V = sum of the target-side terms.
+// The root kernel turns these into the root's polynomial as for a cell, with
+// H(s) = G_0 + sum_n STF(G_n) . s^n (the Ewald kernels are not harmonic, so G
+// is projected) and U = (sum_k m_k grad H(s_k) + V) / M.
+// ---------------------------------------------------------------------------
+template <typename S>
+struct Dual {
+  S v, d;
+};
+template <typename S>
+__device__ __forceinline__ Dual<S> operator+(Dual<S> a, Dual<S> b) { return {a.v + b.v, a.d + b.d}; }
+template <typename S>
+__device__ __forceinline__ Dual<S> operator-(Dual<S> a, Dual<S> b) { return {a.v - b.v, a.d - b.d}; }
+template <typename S>
+__device__ __forceinline__ Dual<S> operator*(Dual<S> a, Dual<S> b) {
+  return {a.v * b.v, a.v * b.d + a.d * b.v};
+}
+template <typename S>
+__device__ __forceinline__ Dual<S> operator*(S s, Dual<S> a) { return {s * a.v, s * a.d}; }
+template <typename S>
+__device__ __forceinline__ Dual<S> operator*(Dual<S> a, S s) { return {s * a.v, s * a.d}; }
+
+__device__ __forceinline__ float erf_s(float x) { return erff(x); }
+__device__ __forceinline__ double erf_s(double x) { return erf(x); }
+__device__ __forceinline__ float erfc_s(float x) { return erfcf(x); }
+__device__ __forceinline__ double erfc_s(double x) { return erfc(x); }
+__device__ __forceinline__ float exp_s(float x) { return expf(x); }
+__device__ __forceinline__ double exp_s(double x) { return exp(x); }
+__device__ __forceinline__ float sqrt_s(float x) { return sqrtf(x); }
+__device__ __forceinline__ double sqrt_s(double x) { return sqrt(x); }
+
+// g_0..g_6 as in the forward kernel's radial32 / radial64, and gam_0..gam_5
+// with grad g_m = gam_m r: -g_(m+1) for the closed form, 2 B_m for the series
+// g_m = A_m + B_m r^2
+template <typename S>
+__device__ __forceinline__ bool radial_dual(S r2, bool hole, S alpha, S alpha2, S ka, S inner2,
+                                            S* g, S* gam) {
+  if (r2 < inner2) {
+    S sc = ka;
+    const S x = alpha2 * r2;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      g[k] = sc * (x / S(2 * k + 3) - S(1) / S(2 * k + 1));
+      if (k < 6) gam[k] = S(2) * sc * alpha2 / S(2 * k + 3);
+      sc *= S(2) * alpha2;
+    }
+    return true;
+  }
+  const S r = sqrt_s(r2), ir2 = S(1) / r2;
+  const S e = ka * exp_s(-alpha2 * r2) * ir2;
+  g[0] = (hole ? -erf_s(alpha * r) : erfc_s(alpha * r)) / r;
+  S sc = S(1);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    g[k + 1] = S(2 * k + 1) * g[k] * ir2 + sc * e;
+    sc *= S(2) * alpha2;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) gam[k] = -g[k + 1];
+  return false;
+}
+
+// one replica: target side (grad of w . a) into t, root source tensors into G.
+// The directional derivative of a along w (dual numbers) is the gradient of
+// w . a where a is a gradient field (the closed form); the two-term series is
+// not one, and there the antisymmetric rest
+// sum_n (1/(n-1)!) (gam_n + g_(n+1)) [(V_n . w) r - (r . w) V_n] is added.
+template <typename S>
+__device__ __forceinline__ void ewald_adj_term(const Mom22<S>& Q, S rx, S ry, S rz, S wx, S wy,
+                                               S wz, bool hole, const EwaldParams& p, S* t,
+                                               S* G) {
+  const S r2 = rx * rx + ry * ry + rz * rz;
+  const S rw = rx * wx + ry * wy + rz * wz;
+  S g[7], gam[6];
+  const bool series =
+      radial_dual<S>(r2, hole, (S)p.alpha, (S)p.alpha2, (S)p.ka, (S)p.inner2, g, gam);
+  Dual<S> gd[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) gd[k] = {g[k], gam[k] * rw};
+  Dual<S> pot = {S(0), S(0)}, ax = pot, ay = pot, az = pot;
+  ewald_term<S, Dual<S>>(Q, Dual<S>{rx, wx}, Dual<S>{ry, wy}, Dual<S>{rz, wz}, gd, pot, ax, ay, az);
+  t[0] += ax.d;
+  t[1] += ay.d;
+  t[2] += az.d;
+  if (series) {
+    const EwaldVP<S> v = ewald_vp<S, S>(Q, rx, ry, rz);
+    const S c2 = gam[2] + g[3], c3 = (gam[3] + g[4]) * S(0.5), c4 = (gam[4] + g[5]) / S(6);
+    const S vx[3] = {v.v2x, v.v3x, v.v4x}, vy[3] = {v.v2y, v.v3y, v.v4y},
+            vz[3] = {v.v2z, v.v3z, v.v4z}, cf[3] = {c2, c3, c4};
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const S vw = vx[n] * wx + vy[n] * wy + vz[n] * wz;
+      t[0] += cf[n] * (vw * rx - rw * vx[n]);
+      t[1] += cf[n] * (vw * ry - rw * vy[n]);
+      t[2] += cf[n] * (vw * rz - rw * vz[n]);
+    }
+  }
+  S pw[3][5];
+  pw[0][0] = pw[1][0] = pw[2][0] = S(1);
+#pragma unroll
+  for (int e = 1; e < 5; ++e) {
+    pw[0][e] = pw[0][e - 1] * rx;
+    pw[1][e] = pw[1][e - 1] * ry;
+    pw[2][e] = pw[2][e - 1] * rz;
+  }
+#pragma unroll
+  for (int n = 0; n <= 4; ++n) {
+    if (n == 1) continue;
+#pragma unroll
+    for (int a = n; a >= 0; --a)
+#pragma unroll
+      for (int b = n - a; b >= 0; --b) {
+        const int c = n - a - b;
+        S sw = S(0);
+        if (a) sw += S(a) * wx * pw[0][a - 1] * pw[1][b] * pw[2][c];
+        if (b) sw += S(b) * wy * pw[0][a] * pw[1][b - 1] * pw[2][c];
+        if (c) sw += S(c) * wz * pw[0][a] * pw[1][b] * pw[2][c - 1];
+        G[mono(a, b, c)] += S(inv_fact(n)) * (g[n] * sw - rw * g[n + 1] * pw[0][a] * pw[1][b] *
+                                                             pw[2][c]);
+      }
+  }
+}
+
+constexpr int kEwAdjBlock = 256;
+// monomial (a, b, c) of each reduced moment, in FMOMR order
+__device__ constexpr int kRedAbc[22][3] = {
+    {0, 0, 0}, {2, 0, 0}, {0, 2, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {3, 0, 0}, {1, 2, 0},
+    {2, 1, 0}, {0, 3, 0}, {2, 0, 1}, {0, 2, 1}, {1, 1, 1}, {4, 0, 0}, {1, 3, 0}, {3, 1, 0},
+    {0, 4, 0}, {3, 0, 1}, {0, 3, 1}, {2, 2, 0}, {2, 1, 1}, {1, 2, 1}};
+constexpr int kEwAdjFixed = 38;  // G[35] (orders 0, 2, 3, 4 used), V[3]; then A[nH], B[nH]
+
+struct EwAdjArgs {
+  const double* pos64;
+  const float* w;
+  const int* tindex;
+  double* grad;
+  const EwtData* ewt;
+  double* part;  // per block: kEwAdjFixed + 2 nH doubles
+  int n;
+  EwaldParams p;
+};
+
+template <typename T>
+__device__ __forceinline__ T wave_sum_t(T v) {
+#pragma unroll
+  for (int m = kWave / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+  return v;
+}
+
+__global__ __launch_bounds__(kEwAdjBlock) void adj_ewald_kernel(EwAdjArgs a) {
+  extern __shared__ double dyn[];  // red[4][nvals], then the table (2 float4 per entry)
+  __shared__ float4 cand[kEwAdjBlock / kWave][kEwMaxCand];
+  __shared__ int ncand[kEwAdjBlock / kWave];
+  const EwaldParams& p = a.p;
+  const int nvals = kEwAdjFixed + 2 * p.nH;
+  double* red = dyn;
+  float4* tab = reinterpret_cast<float4*>(dyn + 4 * (size_t)nvals);
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  for (int i = tid; i < p.nH; i += kEwAdjBlock) {
+    const EwtData e = a.ewt[i];
+    tab[2 * i] = make_float4(e.hx, e.hy, e.hz, e.hCfac);
+    tab[2 * i + 1] = make_float4(0.0f, 0.0f, 0.0f, e.hSfac);
+  }
+  const int gi = blockIdx.x * kEwAdjBlock + tid;
+  const bool live = gi < a.n;
+  const int id = live ? gi : a.n - 1;
+  const double dx = a.pos64[3 * (size_t)id] - p.cmx, dy = a.pos64[3 * (size_t)id + 1] - p.cmy,
+               dz = a.pos64[3 * (size_t)id + 2] - p.cmz;
+  const float wx = live ? a.w[3 * (size_t)id] : 0.0f, wy = live ? a.w[3 * (size_t)id + 1] : 0.0f,
+              wz = live ? a.w[3 * (size_t)id + 2] : 0.0f;
+  // lattice vectors some particle of the wavefront needs (as the forward kernel)
+  float lo[3] = {(float)dx, (float)dy, (float)dz}, hi[3] = {lo[0], lo[1], lo[2]};
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fminf(lo[k], __shfl_xor(lo[k], m, kWave));
+      hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], m, kWave));
+    }
+  const int W = 2 * p.nEwReps + 1, NC = W * W * W;
+  const float Lf = (float)p.L, cut2f = (float)p.cut2;
+  if (lane == 0) ncand[wv] = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < NC; c0 += kWave) {
+    const int c = c0 + lane;
+    bool need = false;
+    int ix = 0, iy = 0, iz = 0;
+    bool hole = false;
+    if (c < NC) {
+      ix = c / (W * W) - p.nEwReps;
+      iy = (c / W) % W - p.nEwReps;
+      iz = c % W - p.nEwReps;
+      hole = abs(ix) <= p.nReps && abs(iy) <= p.nReps && abs(iz) <= p.nReps;
+      const bool centre = ix == 0 && iy == 0 && iz == 0;
+      const int iv[3] = {ix, iy, iz};
+      float d2 = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float l = lo[k] + iv[k] * Lf, h = hi[k] + iv[k] * Lf;
+        const float e = l > 0.f ? l : (h < 0.f ? -h : 0.f);
+        d2 += e * e;
+      }
+      need = !centre && (hole || d2 <= cut2f * 1.0001f + 1e-6f * Lf * Lf);
+    }
+    const unsigned long long mask = __ballot(need);
+    const int before = __popcll(mask & ((1ull << lane) - 1ull));
+    if (need) cand[wv][ncand[wv] + before] = make_float4(ix * Lf, iy * Lf, iz * Lf, hole ? 1.0f : 0.0f);
+    __syncthreads();
+    if (lane == 0) ncand[wv] += __popcll(mask);
+    __syncthreads();
+  }
+  const int nc = ncand[wv];
+
+  Mom22<double> Qd;
+  Mom22<float> Qf;
+  {
+    double* qd = &Qd.m;
+    float* qf = &Qf.m;
+#pragma unroll
+    for (int k = 0; k < 22; ++k) {
+      qd[k] = p.q[k];
+      qf[k] = (float)p.q[k];
+    }
+  }
+  // replicas other than the central one in FP32, then the central one in FP64
+  double t64[3] = {0.0, 0.0, 0.0}, G64[35];
+  {
+    float t32[3] = {0.f, 0.f, 0.f}, G32[35];
+#pragma unroll
+    for (int q = 0; q < 35; ++q) G32[q] = 0.0f;
+    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+    for (int k = 0; live && k < nc; ++k) {
+      const float4 cv = cand[wv][k];
+      const bool hole = cv.w != 0.0f;
+      const float x = fx + cv.x, y = fy + cv.y, z = fz + cv.z;
+      if (x * x + y * y + z * z > cut2f && !hole) continue;
+      ewald_adj_term<float>(Qf, x, y, z, wx, wy, wz, hole, p, t32, G32);
+    }
+#pragma unroll
+    for (int q = 0; q < 35; ++q) G64[q] = (double)G32[q];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t64[k] = (double)t32[k];
+  }
+  if (live)
+    ewald_adj_term<double>(Qd, dx, dy, dz, (double)wx, (double)wy, (double)wz, p.nReps >= 0, p,
+                           t64, G64);
+  // k space (the forward's phase form): t += k (k.w)(hC cos + hS sin)
+  const float iL = (float)(1.0 / p.L), kscale = (float)(p.L / (2.0 * M_PI));
+  const float ux = (float)dx * iL, uy = (float)dy * iL, uz = (float)dz * iL;
+  __syncthreads();
+  for (int i = 0; i < p.nH; ++i) {
+    const float4 e0 = tab[2 * i];
+    const float hs = tab[2 * i + 1].w;
+    float ph = rintf(e0.x * kscale) * ux + rintf(e0.y * kscale) * uy + rintf(e0.z * kscale) * uz;
+    ph -= rintf(ph);
+    float sn, cs;
+    sincospif(2.0f * ph, &sn, &cs);
+    const float kw = e0.x * wx + e0.y * wy + e0.z * wz;
+    const float f = kw * (e0.w * cs + hs * sn);
+    t64[0] += (double)(e0.x * f);
+    t64[1] += (double)(e0.y * f);
+    t64[2] += (double)(e0.z * f);
+    const float A = wave_sum_t(live ? kw * sn : 0.0f);
+    const float B = wave_sum_t(live ? -kw * cs : 0.0f);
+    if (lane == 0) {
+      red[wv * nvals + kEwAdjFixed + i] = (double)A;
+      red[wv * nvals + kEwAdjFixed + p.nH + i] = (double)B;
+    }
+  }
+  if (live) {  // one target per particle: a plain update
+    double* g = a.grad + 4 * (size_t)(a.tindex ? a.tindex[id] : id);
+    g[0] += t64[0];
+    g[1] += t64[1];
+    g[2] += t64[2];
+  }
+#pragma unroll
+  for (int q = 0; q < 35; ++q) {
+    const double s = wave_sum_t(G64[q]);
+    if (lane == 0) red[wv * nvals + q] = s;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double s = wave_sum_t(live ? t64[k] : 0.0);
+    if (lane == 0) red[wv * nvals + 35 + k] = s;
+  }
+  __syncthreads();
+  for (int v = tid; v < nvals; v += kEwAdjBlock) {
+    double s = 0.0;
+    for (int k = 0; k < kEwAdjBlock / kWave; ++k) s += red[k * nvals + v];
+    a.part[(size_t)blockIdx.x * nvals + v] = s;
+  }
+}
+
+// totals over blocks (deterministic order)
+__global__ void adj_ewald_sum_kernel(const double* part, int nblocks, int nvals, double* total) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvals) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * nvals + v];
+  total[v] = s;
+}
+
+// Hhat of the Ewald term about the root's centre of mass, added to node 0
+__global__ void adj_ewald_root_kernel(const double* total, const EwtData* ewt, EwaldParams p,
+                                      double* node) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double G[35];
+  for (int q = 0; q < 35; ++q) G[q] = total[q];
+  const double* A = total + kEwAdjFixed;
+  const double* B = A + p.nH;
+  const double V3 = 1.0 / (p.L * p.L * p.L);
+  // h-table: hC = -w_h (M - Q2.kk/2 + Q4.k^4/24), hS = w_h Q3.k^3/6
+  for (int i = 0; i < p.nH; ++i) {
+    const double k[3] = {ewt[i].hx, ewt[i].hy, ewt[i].hz};
+    const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+    const double wh = 4.0 * M_PI * V3 * exp(-k2 / (4.0 * p.alpha2)) / k2;
+    G[0] -= A[i] * wh;
+    for (int n = 2; n <= 4; ++n)
+      for (int a = n; a >= 0; --a)
+        for (int b = n - a; b >= 0; --b) {
+          const int c = n - a - b;
+          double kp = 1.0;
+          for (int e = 0; e < a; ++e) kp *= k[0];
+          for (int e = 0; e < b; ++e) kp *= k[1];
+          for (int e = 0; e < c; ++e) kp *= k[2];
+          const double f = n == 2 ? 0.5 * A[i] : n == 3 ? B[i] / 6.0 : -A[i] / 24.0;
+          G[mono(a, b, c)] += f * wh * kp;
+        }
+  }
+  // H: h_0 = G_0; h_al = mult(al) STF(G)_al (orders 2..4)
+  double T[35];
+  for (int q = 0; q < 35; ++q) T[q] = G[q];
+  stf_project(T);
+  double H[35];
+  for (int q = 0; q < 35; ++q) H[q] = 0.0;
+  H[0] = G[0];
+  const double fct[5] = {1.0, 1.0, 2.0, 6.0, 24.0};
+  for (int n = 2; n <= 4; ++n)
+    for (int a = n; a >= 0; --a)
+      for (int b = n - a; b >= 0; --b) {
+        const int c = n - a - b;
+        H[mono(a, b, c)] = fct[n] / (fct[a] * fct[b] * fct[c]) * T[mono(a, b, c)];
+      }
+  // root moments about cm (unit scale), completed
+  double C[35];
+  for (int q = 0; q < 35; ++q) C[q] = 0.0;
+  for (int k = 0; k < 22; ++k) C[mono(kRedAbc[k][0], kRedAbc[k][1], kRedAbc[k][2])] = p.q[k];
+  complete_traceless<2, 4>(C);
+  const double M = p.q[0];
+  double Hj[3];
+  for (int j = 0; j < 3; ++j) {
+    double S = 0.0;
+    for (int n = 3; n <= 4; ++n)
+      for (int a = n; a >= 0; --a)
+        for (int b = n - a; b >= 0; --b) {
+          const int c = n - a - b;
+          const int e[3] = {a, b, c};
+          if (e[j] == 0) continue;
+          S += H[mono(a, b, c)] * e[j] *
+               C[mono(a - (j == 0), b - (j == 1), c - (j == 2))];
+        }
+    Hj[j] = -(S + total[35 + j]) / M;
+  }
+  H[mono(1, 0, 0)] = Hj[0];
+  H[mono(0, 1, 0)] = Hj[1];
+  H[mono(0, 0, 1)] = Hj[2];
+  for (int q = 0; q < 35; ++q) {
+    node[kAdjHhat + q] += H[q];
+    node[kAdjPoly + q] += H[q];
+  }
+}
+
 __global__ void fill_zero_kernel(double* p, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
@@ -831,6 +1216,55 @@ extern "C" size_t changaGravityAdjointWorkInts(int nNodes, int numCellEntries) {
   WorkLayout L;
   if (work_layout(nNodes, numCellEntries, L) != hipSuccess) return 0;
   return L.total;
+}
+
+static size_t ewald_adj_vals(int nH) { return (size_t)kEwAdjFixed + 2 * (size_t)nH; }
+static int ewald_adj_blocks(int n) { return (n + kEwAdjBlock - 1) / kEwAdjBlock; }
+
+// target side into grad, root source side into node 0's Hhat and polynomial
+static int launch_ewald_adjoint(const ChangaAdjointLaunch* a, hipStream_t s) {
+  const EwaldParams p = make_params(a->ewaldRO, a->rootCm64);
+  const int n = a->nTargets;
+  if (n <= 0 || !a->ewt || !a->ewaldWork || !a->cells.tpos64)
+    return fail("changaGravityAdjoint: Ewald term needs nTargets, ewt, ewaldWork, tpos64",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  if (p.nEwReps > 3 || p.nH < 0)
+    return fail("changaGravityAdjoint: Ewald nEwReps > 3", hipErrorInvalidValue, __FILE__,
+                __LINE__);
+  const size_t nvals = ewald_adj_vals(p.nH);
+  const int nb = ewald_adj_blocks(n);
+  if (a->ewaldWorkDoubles < ((size_t)nb + 1) * nvals)
+    return fail("changaGravityAdjoint: ewaldWork too small", hipErrorInvalidValue, __FILE__,
+                __LINE__);
+  // reduction rows of the four wavefronts, then the h-table
+  const size_t shm = 4 * nvals * sizeof(double) + 2 * sizeof(float4) * (size_t)p.nH;
+  if (shm + sizeof(float4) * (kEwAdjBlock / kWave) * kEwMaxCand + 256 > 65536)
+    return fail("changaGravityAdjoint: Ewald h-table too large", hipErrorInvalidValue, __FILE__,
+                __LINE__);
+  EwAdjArgs ea;
+  ea.pos64 = a->cells.tpos64;
+  ea.w = a->w;
+  ea.tindex = a->targetIndex;
+  ea.grad = a->grad;
+  ea.ewt = a->ewt;
+  ea.part = a->ewaldWork;
+  ea.n = n;
+  ea.p = p;
+  double* total = a->ewaldWork + (size_t)nb * nvals;
+  hipLaunchKernelGGL(adj_ewald_kernel, dim3(nb), dim3(kEwAdjBlock), shm, s, ea);
+  CGPU_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(adj_ewald_sum_kernel, dim3((unsigned)((nvals + 127) / 128)), dim3(128), 0, s,
+                     a->ewaldWork, nb, (int)nvals, total);
+  CGPU_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(adj_ewald_root_kernel, dim3(1), dim3(kWave), 0, s, total, a->ewt, p,
+                     a->nodeWork);
+  CGPU_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" size_t changaGravityAdjointEwaldWorkDoubles(int nTargets, int nH) {
+  if (nTargets <= 0 || nH < 0) return 0;
+  return ((size_t)ewald_adj_blocks(nTargets) + 1) * ewald_adj_vals(nH);
 }
 
 extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) {
@@ -909,10 +1343,12 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
       hipLaunchKernelGGL(adj_targets_kernel<0>, dim3(nb), dim3(kWave), 0, s, args);
     CGPU_CHECK(hipGetLastError());
   }
-  // (4) Hhat per node, then top-down
+  // (4) Hhat per node (plus the Ewald term's at the root), then top-down
   hipLaunchKernelGGL(adj_hhat_kernel, dim3((nn + 127) / 128), dim3(128), 0, s, nn, a->nodeWork,
                      a->node64);
   CGPU_CHECK(hipGetLastError());
+  if (a->ewaldRO)
+    if (int e = launch_ewald_adjoint(a, s)) return e;
   for (int d = 0; d < a->nDepths; ++d) {
     const int lo = a->levelOffsets[d], hi = a->levelOffsets[d + 1];
     if (hi <= lo) continue;

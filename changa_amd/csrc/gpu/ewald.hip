@@ -26,110 +26,9 @@
 // keeps only the lattice vectors any of them can need (compacted into LDS),
 // then every lane applies the exact per-particle test. The h-loop table is
 // staged in LDS (dynamic size; no 80-entry limit).
-#include <cmath>
-
-#include "common.h"
+#include "ewald_body.h"
 
 namespace cgpu {
-
-struct EwaldParams {
-  double cmx, cmy, cmz;  // root centre of mass
-  double mk1;            // M * k1, the k = 0 term
-  double q[22];          // root reduced STF moments at unit scale (FMOMR order)
-  double L, alpha, alpha2, ka, cut2, inner2;
-  int nReps, nEwReps, nH, pad;
-};
-
-template <typename T>
-struct Mom22 {
-  T m, xx, yy, xy, xz, yz, xxx, xyy, xxy, yyy, xxz, yyz, xyz, xxxx, xyyy, xxxy, yyyy, xxxz, yyyz,
-      xxyy, xxyz, xyyz;
-};
-
-// Ewald contribution of one replica vector r with radial kernels g[0..5].
-template <typename T>
-__device__ __forceinline__ void ewald_term(const Mom22<T>& Q, T x, T y, T z, const T* g, T& pot,
-                                           T& ax, T& ay, T& az) {
-  T x2 = x * x, y2 = y * y, z2 = z * z;
-  T fxx = x2 - z2, fyy = y2 - z2;
-  T fxy = T(2) * x * y, fxz = T(2) * x * z, fyz = T(2) * y * z;
-  T z3 = T(3) * z2;
-  T fxxx = x * (x2 - z3), fyyy = y * (y2 - z3);
-  T fxyy = T(3) * x * fyy, fxxy = T(3) * y * fxx;
-  T fxxz = z * (T(3) * x2 - z2), fyyz = z * (T(3) * y2 - z2);
-  T fxyz = T(3) * fxy * z;
-  T v2x = Q.xx * x + Q.xy * y + Q.xz * z;
-  T v2y = Q.xy * x + Q.yy * y + Q.yz * z;
-  T v2z = Q.xz * x + Q.yz * y - (Q.xx + Q.yy) * z;
-  T v3x = Q.xxx * fxx + Q.xyy * fyy + Q.xxy * fxy + Q.xxz * fxz + Q.xyz * fyz;
-  T v3y = Q.xyy * fxy + Q.xxy * fxx + Q.yyy * fyy + Q.yyz * fyz + Q.xyz * fxz;
-  T v3z = Q.xxz * fxx + Q.yyz * fyy + Q.xyz * fxy - (Q.xxx + Q.xyy) * fxz - (Q.xxy + Q.yyy) * fyz;
-  T v4x = Q.xxxx * fxxx + Q.xxyy * fxyy + Q.xxxy * fxxy + Q.xyyy * fyyy + Q.xxxz * fxxz +
-          Q.xyyz * fyyz + Q.xxyz * fxyz;
-  T v4y = Q.yyyy * fyyy + Q.xxyy * fxxy + Q.xxxy * fxxx + Q.xyyy * fxyy + Q.xyyz * fxyz +
-          Q.xxyz * fxxz + Q.yyyz * fyyz;
-  T v4z = Q.xxxz * fxxx + Q.xyyz * fxyy + Q.xxyz * fxxy + Q.yyyz * fyyy - Q.xxxx * fxxz -
-          Q.yyyy * fyyz - Q.xxyy * (fxxz + fyyz) - (Q.xxxy + Q.xyyy) * fxyz;
-  T p2 = v2x * x + v2y * y + v2z * z;
-  T p3 = v3x * x + v3y * y + v3z * z;
-  T p4 = v4x * x + v4y * y + v4z * z;
-  const T h = T(0.5), s = T(1) / T(6), f = T(1) / T(24);
-  pot -= g[0] * Q.m + g[2] * h * p2 + g[3] * s * p3 + g[4] * f * p4;
-  T ta = g[1] * Q.m + g[3] * h * p2 + g[4] * s * p3 + g[5] * f * p4;
-  T c3 = g[3] * h, c4 = g[4] * s;
-  ax += g[2] * v2x + c3 * v3x + c4 * v4x - ta * x;
-  ay += g[2] * v2y + c3 * v3y + c4 * v4y - ta * y;
-  az += g[2] * v2z + c3 * v3z + c4 * v4z - ta * z;
-}
-
-// g_0..g_5: series about r = 0 or closed form (hole: -erf, else erfc)
-__device__ __forceinline__ void radial64(double r2, bool hole, const EwaldParams& p, double* g) {
-  if (r2 < p.inner2) {
-    double s = p.ka, x = p.alpha2 * r2;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      g[k] = s * (x / (2 * k + 3) - 1.0 / (2 * k + 1));
-      s *= 2 * p.alpha2;
-    }
-  } else {
-    double r = sqrt(r2), ir2 = 1.0 / r2;
-    double e = p.ka * exp(-p.alpha2 * r2) * ir2;
-    g[0] = (hole ? -erf(p.alpha * r) : erfc(p.alpha * r)) / r;
-    double s = 1.0;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      g[k + 1] = (2 * k + 1) * g[k] * ir2 + s * e;
-      s *= 2 * p.alpha2;
-    }
-  }
-}
-
-__device__ __forceinline__ void radial32(float r2, bool hole, float alpha, float alpha2, float ka,
-                                         float inner2, float* g) {
-  if (r2 < inner2) {
-    const float inv_odd[7] = {1.0f, 1.0f / 3.0f, 1.0f / 5.0f, 1.0f / 7.0f,
-                              1.0f / 9.0f, 1.0f / 11.0f, 1.0f / 13.0f};
-    float s = ka, x = alpha2 * r2;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      g[k] = s * (x * inv_odd[k + 1] - inv_odd[k]);
-      s *= 2 * alpha2;
-    }
-  } else {
-    float dir = rsqrtf(r2), r = r2 * dir, ir2 = dir * dir;
-    float e = ka * expf(-alpha2 * r2) * ir2;
-    g[0] = (hole ? -erff(alpha * r) : erfcf(alpha * r)) * dir;
-    float s = 1.0f;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      g[k + 1] = (2 * k + 1) * g[k] * ir2 + s * e;
-      s *= 2 * alpha2;
-    }
-  }
-}
-
-constexpr int kEwBlock = 256;
-constexpr int kEwMaxCand = 7 * 7 * 7;  // nEwReps <= 3 (encodeOffset replica range)
 
 __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* parts,
                                                          VariablePartData* vars,
@@ -142,13 +41,27 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
   __shared__ float4 cand[kEwBlock / kWave][kEwMaxCand];
   __shared__ int ncand[kEwBlock / kWave];
 
+  // the same table as weights on the cube |h_i| <= kEwCube (zero off the table)
+  __shared__ float2 cube[kEwCubeN];
+  __shared__ int n_outside;
+
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   const float kscale = (float)(p.L / (2.0 * M_PI));  // k -> integer h
+  for (int i = tid; i < kEwCubeN; i += kEwBlock) cube[i] = make_float2(0.0f, 0.0f);
+  if (tid == 0) n_outside = 0;
+  __syncthreads();
   for (int i = tid; i < p.nH; i += kEwBlock) {
     EwtData e = ewt[i];
+    const float hx = rintf(e.hx * kscale), hy = rintf(e.hy * kscale), hz = rintf(e.hz * kscale);
     sh[2 * i] = make_float4(e.hx, e.hy, e.hz, e.hCfac);
-    sh[2 * i + 1] = make_float4(rintf(e.hx * kscale), rintf(e.hy * kscale), rintf(e.hz * kscale),
-                                e.hSfac);
+    sh[2 * i + 1] = make_float4(hx, hy, hz, e.hSfac);
+    if (fabsf(hx) <= kEwCube && fabsf(hy) <= kEwCube && fabsf(hz) <= kEwCube) {
+      const int ci = ((int)hx + kEwCube) * kEwCubeW * kEwCubeW + ((int)hy + kEwCube) * kEwCubeW +
+                     (int)hz + kEwCube;
+      cube[ci] = make_float2(e.hCfac, e.hSfac);
+    } else {
+      atomicAdd(&n_outside, 1);
+    }
   }
 
   const int gi = blockIdx.x * kEwBlock + tid;
@@ -245,23 +158,64 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
     ay += ty;
     az += tz;
   }
-  // k space: phase h.d / L in revolutions (h integer), reduced exactly, then
-  // sin / cos of pi times twice that
+  // k space. On the cube |h_i| <= kEwCube: e^(2 pi i h.u) (u = d / L) as the
+  // product of per-axis powers of e^(2 pi i u_x) etc., which are formed once
+  // per particle; the plane hx is advanced by one complex multiplication.
+  // Sums in FP32 within a plane, in FP64 across planes.
   const float iL = (float)(1.0 / p.L);
   const float ux = fx * iL, uy = fy * iL, uz = fz * iL;
-  for (int i = 0; i < p.nH; ++i) {
-    const float4 e0 = sh[2 * i];
-    const float4 e1 = sh[2 * i + 1];
-    const float hs = e1.w;
-    float ph = e1.x * ux + e1.y * uy + e1.z * uz;
-    ph -= rintf(ph);
-    float s, c;
-    sincospif(2.0f * ph, &s, &c);
-    pot += (double)(e0.w * c + hs * s);
-    float t = e0.w * s - hs * c;
-    ax += (double)(e0.x * t);
-    ay += (double)(e0.y * t);
-    az += (double)(e0.z * t);
+  {
+    float2 ey[kEwCubeW], ez[kEwCubeW];
+    float2 e1x;
+    sincospif(2.0f * ux, &e1x.y, &e1x.x);
+    axis_powers(uy, ey);
+    axis_powers(uz, ez);
+    float2 ex = cpow_neg(e1x);  // e^(-2 pi i kEwCube u_x)
+    const float kf = (float)(2.0 * M_PI / p.L);
+    for (int hx = -kEwCube; hx <= kEwCube; ++hx) {
+      float sp = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+      const float2* cw = cube + (hx + kEwCube) * kEwCubeW * kEwCubeW;
+#pragma unroll
+      for (int hy = -kEwCube; hy <= kEwCube; ++hy) {
+        const float2 exy = cmul(ex, ey[hy + kEwCube]);
+#pragma unroll
+        for (int hz = -kEwCube; hz <= kEwCube; ++hz) {
+          const float2 wgt = cw[(hy + kEwCube) * kEwCubeW + hz + kEwCube];
+          if (wgt.x != 0.0f || wgt.y != 0.0f) {
+            const float2 e = cmul(exy, ez[hz + kEwCube]);
+            sp += wgt.x * e.x + wgt.y * e.y;
+            const float t = wgt.x * e.y - wgt.y * e.x;
+            sx += (float)hx * t;
+            sy += (float)hy * t;
+            sz += (float)hz * t;
+          }
+        }
+      }
+      pot += (double)sp;
+      ax += (double)(kf * sx);
+      ay += (double)(kf * sy);
+      az += (double)(kf * sz);
+      ex = cmul(ex, e1x);
+    }
+  }
+  // entries off the cube (a table cut beyond |h_i| = kEwCube): phase h.d / L
+  // in revolutions (h integer), reduced exactly, then sin / cos of 2 pi that
+  if (n_outside > 0) {
+    for (int i = 0; i < p.nH; ++i) {
+      const float4 e0 = sh[2 * i];
+      const float4 e1 = sh[2 * i + 1];
+      if (fabsf(e1.x) <= kEwCube && fabsf(e1.y) <= kEwCube && fabsf(e1.z) <= kEwCube) continue;
+      const float hs = e1.w;
+      float ph = e1.x * ux + e1.y * uy + e1.z * uz;
+      ph -= rintf(ph);
+      float s, c;
+      sincospif(2.0f * ph, &s, &c);
+      pot += (double)(e0.w * c + hs * s);
+      float t = e0.w * s - hs * c;
+      ax += (double)(e0.x * t);
+      ay += (double)(e0.y * t);
+      az += (double)(e0.z * t);
+    }
   }
   if (live) {
     VariablePartData v = vars[id];
@@ -274,30 +228,6 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
 }
 
 // Builds the kernel parameters from the reference's read-only record.
-static EwaldParams make_params(const EwaldReadOnlyData* ro, const double* rootCm64) {
-  EwaldParams p;
-  const MomcData& mc = ro->momcRoot;
-  const float* m = &mc.m;
-  for (int k = 0; k < 22; ++k) p.q[k] = m[k];
-  p.cmx = rootCm64 ? rootCm64[0] : (double)ro->mm.cmx;
-  p.cmy = rootCm64 ? rootCm64[1] : (double)ro->mm.cmy;
-  p.cmz = rootCm64 ? rootCm64[2] : (double)ro->mm.cmz;
-  double L = ro->L;
-  double alpha = 2.0 / L;
-  p.L = L;
-  p.alpha = alpha;
-  p.alpha2 = alpha * alpha;
-  p.ka = 2.0 * alpha / sqrt(M_PI);
-  p.mk1 = (double)ro->mm.totalMass * M_PI / (alpha * alpha * L * L * L);
-  p.cut2 = (double)ro->fEwCut * (double)ro->fEwCut * L * L;
-  p.inner2 = 1.2e-3 * L * L;
-  p.nReps = ro->nReps;
-  p.nEwReps = ro->nEwReps > ro->nReps ? ro->nEwReps : ro->nReps;
-  p.nH = ro->nEwhLoop;
-  p.pad = 0;
-  return p;
-}
-
 int launch_ewald(const CompactPartData* parts, VariablePartData* vars, const int* markers,
                  int first, int n, const EwaldReadOnlyData* ro, const EwtData* d_ewt,
                  const double* pos64, const double* rootCm64, hipStream_t s) {

@@ -322,11 +322,12 @@ class Evaluation:
         self.run_ewald()
 
     # ------------------------------------------------------------------ adjoint form
-    def adjoint(self, w):
+    def adjoint(self, w, ewald=None):
         """dL/dx and dL/dm (tree order, [N, 4] FP64 on the device) of
         L = sum_i w_i . a_i over the targets, for this evaluation's lists
         (precision form), with the moments of the last build. w: [n_targets, 3]
-        in target order (tree order when every particle is a target)."""
+        in target order (tree order when every particle is a target).
+        ewald: include the Ewald correction in a (default: when periodic)."""
         if not self.precise:
             raise ValueError("the adjoint runs on the precision form (FP64 positions)")
         n = self.pos.shape[0]
@@ -364,6 +365,22 @@ class Evaluation:
         a.node64 = self.d_node64.data_ptr()
         a.pos64 = self.d_pos64.data_ptr()
         a.mass64 = self.d_mass64.data_ptr()
+        if ewald is None:
+            ewald = bool(self.w["periodic"])
+        if ewald:
+            if not self.w["periodic"]:
+                raise ValueError("the Ewald term needs a periodic box")
+            nd = gpu.lib().changaGravityAdjointEwaldWorkDoubles(self.n_targets,
+                                                                 self.ewald_ro.nEwhLoop)
+            work = getattr(self, "d_ewald_work", None)
+            if work is None or work.numel() < nd:
+                self.d_ewald_work = torch.empty(nd, dtype=torch.float64, device=self.device)
+            a.ewaldRO = ctypes.addressof(self.ewald_ro)
+            a.ewt = self.d_ewt.data_ptr()
+            a.rootCm64 = self.root_cm64.ctypes.data
+            a.nTargets = self.n_targets
+            a.ewaldWork = self.d_ewald_work.data_ptr()
+            a.ewaldWorkDoubles = self.d_ewald_work.numel()
         gpu.check(gpu.lib().changaGravityAdjoint(ctypes.byref(a), self._sp()),
                   "changaGravityAdjoint")
         return grad

@@ -80,7 +80,9 @@ class ChangaAdjointLaunch(ctypes.Structure):
                 ("listWorkInts", c_size_t), ("numCellEntries", c_int), ("nNodes", c_int),
                 ("nDepths", c_int), ("nParticles", c_int), ("levelOffsets", c_void_p),
                 ("first", c_void_p), ("count", c_void_p), ("child", c_void_p),
-                ("node64", c_void_p), ("pos64", c_void_p), ("mass64", c_void_p)]
+                ("node64", c_void_p), ("pos64", c_void_p), ("mass64", c_void_p),
+                ("ewaldRO", c_void_p), ("ewt", c_void_p), ("rootCm64", c_void_p),
+                ("nTargets", c_int), ("ewaldWork", c_void_p), ("ewaldWorkDoubles", c_size_t)]
 
 
 CALLBACK = ctypes.CFUNCTYPE(None, c_void_p)
@@ -97,7 +99,7 @@ EXPORTS = [
     "changaGpuReleasePools", "TreePiecePartListDataTransferLocalCompact", "changaNodeGravity",
     "changaPartGravity", "changaZeroVars", "changaEwald", "changaBuildMoments",
     "changaBatchPlanCreate", "changaBatchPlanLaunch", "changaBatchPlanRun", "changaBatchPlanDestroy",
-    "changaGravityAdjoint", "changaGravityAdjointWorkInts",
+    "changaGravityAdjoint", "changaGravityAdjointWorkInts", "changaGravityAdjointEwaldWorkDoubles",
 ]
 
 _lib = None
@@ -157,6 +159,8 @@ def lib():
     L.changaGravityAdjoint.restype = c_int
     L.changaGravityAdjointWorkInts.argtypes = [c_int, c_int]
     L.changaGravityAdjointWorkInts.restype = c_size_t
+    L.changaGravityAdjointEwaldWorkDoubles.argtypes = [c_int, c_int]
+    L.changaGravityAdjointEwaldWorkDoubles.restype = c_size_t
     _lib = L
     return L
 

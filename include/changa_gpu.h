@@ -207,7 +207,9 @@ int changaBatchPlanDestroy(ChangaBatchPlan *plan);
  * every particle on the same lists: target-side tidal terms, particle-source
  * terms, and cell-source terms carried down the tree as polynomials about
  * each node's centre of mass. Softening lengths are constants. Periodic
- * replicas in the lists are handled; the Ewald correction is not part of L.
+ * replicas in the lists are handled. With ewaldRO set, L also holds the
+ * Ewald correction of changaEwald at the targets (its target-side terms, and
+ * its dependence on the root's centre of mass, moments and h-table).
  * ====================================================================== */
 typedef struct {
   ChangaGravityLaunch cells;      /* particle-cell launch (listKind 0, tpos64 and cm64 set) */
@@ -226,11 +228,21 @@ typedef struct {
   const int *first, *count, *child;  /* device, as for changaBuildMoments */
   const double *node64;           /* device: node64 of changaBuildMoments */
   const double *pos64, *mass64;   /* device: particles in tree order */
+  /* Ewald term (optional; all NULL / 0 for none) */
+  const EwaldReadOnlyData *ewaldRO;  /* host: the record given to changaEwald */
+  const EwtData *ewt;                /* device: its h-table */
+  const double *rootCm64;            /* host: root centre of mass (FP64), as for changaEwald */
+  int nTargets;                      /* targets (rows of w), positions cells.tpos64 */
+  double *ewaldWork;                 /* scratch: changaGravityAdjointEwaldWorkDoubles(nTargets,
+                                        ewaldRO->nEwhLoop) doubles (device) */
+  size_t ewaldWorkDoubles;           /* its size in doubles */
 } ChangaAdjointLaunch;
 
 /* Ints of listWork for a tree of nNodes nodes and a cell list of
    numCellEntries entries (the cell list transposed per cell). */
 size_t changaGravityAdjointWorkInts(int nNodes, int numCellEntries);
+/* Doubles of ewaldWork for nTargets targets and an h-table of nH entries. */
+size_t changaGravityAdjointEwaldWorkDoubles(int nTargets, int nH);
 int changaGravityAdjoint(const ChangaAdjointLaunch *a, void *stream);
 
 #ifdef __cplusplus

@@ -1008,7 +1008,7 @@ void orc_adjoint_lists(int n_nodes, const int32_t *node_child, int n_buckets,
                        const double *pos, const double *mass, const double *soft,
                        const int32_t *cells, const int64_t *cell_marks,
                        const int32_t *parts, const int64_t *part_marks, double period,
-                       const double *w, int n_threads, double *grad) {
+                       const double *w, const double *root_extra, int n_threads, double *grad) {
     mono_init();
     double *acc = (double *)calloc((size_t)n_nodes * ORC_ADJ, sizeof(double));
 #ifdef _OPENMP
@@ -1154,6 +1154,8 @@ void orc_adjoint_lists(int n_nodes, const int32_t *node_child, int n_buckets,
         Pn[mono_idx[1][0][0]] = -U[0];
         Pn[mono_idx[0][1][0]] = -U[1];
         Pn[mono_idx[0][0][1]] = -U[2];
+        if (n == 0 && root_extra)
+            for (int q = 0; q < 35; ++q) Pn[q] += root_extra[q];
         const int p = parent[n];
         if (p >= 0) {
             const double *gp = geo + (size_t)p * 12;
@@ -1227,5 +1229,207 @@ void orc_direct_grad(int n, const double *pos, const double *mass, const double 
         }
         for (int q = 0; q < 3; ++q) grad[4 * k + q] = gx[q];
         grad[4 * k + 3] = gm;
+    }
+}
+
+/* -------------------------------------------------------------------------
+ * Adjoint of the Ewald correction (no reference counterpart).
+ *
+ * L_E = sum_t w_t . a_E(x_t - cm_root) with a_E the BucketEwald acceleration
+ * of orc_bucket_ewald and the h-table of orc_ewald_table built from the root
+ * moments. Per target t (d = x_t - cm):
+ *   - target side: the gradient of w . a for every real-space replica term
+ *     a = sum_n (1/n!) [n V_n g_n - P_n g_(n+1) r] (V_n = Q_n.r^(n-1),
+ *     P_n = Q_n.r^n), with grad g_m = gam_m r, gam_m = -g_(m+1) for the
+ *     closed form and 2 B_m for the two-term series g_m = A_m + B_m r^2 (the
+ *     series' a is not an exact gradient field, so this is the transposed
+ *     Jacobian applied to w, not its derivative along w); and of the k-terms,
+ *     sum_h k (k.w)(hC cos + hS sin);
+ *   - source side: G_n = dL/dQ_n (symmetric tensors, dL = sum_n G_n . dQ_n):
+ *     real space (1/n!) [g_n sum_j alpha_j w_j r^(alpha-e_j) - (w.r) g_(n+1)
+ *     r^alpha]; k space through hC, hS (linear in M, Q_2, Q_3, Q_4).
+ * Then, as for a cell, H(s) = G_0 + sum_n STF(G_n) . s^n, U = (sum_k m_k
+ * grad H(s_k) + sum_t (target-side terms)) / M, Hhat = H - U . s about the
+ * root's centre of mass; hroot receives Hhat's 35 monomial coefficients.
+ * ------------------------------------------------------------------------- */
+
+/* W = Q_n . (r^(n-2) w): W_i = sum_{j, |be| = n-2} mult(be) C(e_i+e_j+be) w_j r^be */
+static void contract_w(const cmom *C, int n, const double r[3], const double w[3], double W[3]) {
+    W[0] = W[1] = W[2] = 0.0;
+    if (n < 2) return;
+    int m = n - 2;
+    for (int a = m; a >= 0; --a)
+        for (int b = m - a; b >= 0; --b) {
+            int c = m - a - b;
+            double f = multinom(a, b, c) * ipow(r[0], a) * ipow(r[1], b) * ipow(r[2], c);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    int e[3] = {a, b, c};
+                    e[i]++;
+                    e[j]++;
+                    W[i] += f * w[j] * C->v[e[0]][e[1]][e[2]];
+                }
+        }
+}
+
+void orc_adjoint_ewald(int n, const double *pos, const double *w, const double root_cm[3],
+                       double root_mass, const double q_root[ORC_NMOM], double u_root,
+                       const double *ewt, int n_ewt, double L, int n_reps, double ew_cut,
+                       int n_threads, double *grad, double hroot[35]) {
+    mono_init();
+    cmom C;
+    to_complete(q_root, u_root, &C);
+    C.v[0][0][0] = q_root[0];
+    int nEwReps = (int)ceil(ew_cut);
+    if (nEwReps < n_reps) nEwReps = n_reps;
+    const double cut2 = ew_cut * ew_cut * L * L, inner2 = 1.2e-3 * L * L;
+    const double alpha = 2.0 / L, alpha2 = alpha * alpha, ka = 2.0 * alpha / sqrt(M_PI);
+    double G[35], V[3] = {0, 0, 0};
+    double *AB = (double *)calloc(2 * (size_t)(n_ewt > 0 ? n_ewt : 1), sizeof(double));
+    memset(G, 0, sizeof(G));
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel
+#endif
+    {
+        double Gl[35], Vl[3] = {0, 0, 0};
+        double *ABl = (double *)calloc(2 * (size_t)(n_ewt > 0 ? n_ewt : 1), sizeof(double));
+        memset(Gl, 0, sizeof(Gl));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+        for (int t = 0; t < n; ++t) {
+            const double *wt = w + 3 * t;
+            if (wt[0] == 0.0 && wt[1] == 0.0 && wt[2] == 0.0) continue;
+            double d[3] = {pos[3 * t] - root_cm[0], pos[3 * t + 1] - root_cm[1],
+                           pos[3 * t + 2] - root_cm[2]};
+            double tt[3] = {0, 0, 0};
+            for (int ix = -nEwReps; ix <= nEwReps; ++ix)
+                for (int iy = -nEwReps; iy <= nEwReps; ++iy)
+                    for (int iz = -nEwReps; iz <= nEwReps; ++iz) {
+                        int hole = abs(ix) <= n_reps && abs(iy) <= n_reps && abs(iz) <= n_reps;
+                        double r[3] = {d[0] + ix * L, d[1] + iy * L, d[2] + iz * L};
+                        double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                        if (r2 > cut2 && !hole) continue;
+                        double rw = r[0] * wt[0] + r[1] * wt[1] + r[2] * wt[2];
+                        /* g_m and gam_m = (1/r) dg_m/dr, so grad g_m = gam_m r */
+                        double g[7], gam[6], dg[6];
+                        if (r2 < inner2) {
+                            double s = ka, x = alpha2 * r2;
+                            for (int k = 0; k < 7; ++k) {
+                                g[k] = s * (x / (2 * k + 3) - 1.0 / (2 * k + 1));
+                                if (k < 6) gam[k] = 2.0 * s * alpha2 / (2 * k + 3);
+                                s *= 2 * alpha2;
+                            }
+                        } else {
+                            double r1 = sqrt(r2), ir2 = 1.0 / r2;
+                            double e = ka * exp(-alpha2 * r2) * ir2;
+                            g[0] = (hole ? -erf(alpha * r1) : erfc(alpha * r1)) / r1;
+                            double s = 1.0;
+                            for (int k = 0; k < 6; ++k) {
+                                g[k + 1] = (2 * k + 1) * g[k] * ir2 + s * e;
+                                s *= 2 * alpha2;
+                            }
+                            for (int k = 0; k < 6; ++k) gam[k] = -g[k + 1];
+                        }
+                        for (int k = 0; k < 6; ++k) dg[k] = gam[k] * rw;
+                        for (int nn = 0; nn <= 4; ++nn) {
+                            if (nn == 1) continue;
+                            double P, Vn[3], W[3], inv = 1.0 / fact(nn);
+                            contract(&C, nn, r, &P, Vn);
+                            contract_w(&C, nn, r, wt, W);
+                            double Vw = Vn[0] * wt[0] + Vn[1] * wt[1] + Vn[2] * wt[2];
+                            for (int k = 0; k < 3; ++k)
+                                tt[k] += inv * (nn * (nn - 1) * W[k] * g[nn] + nn * gam[nn] * Vw * r[k] -
+                                                nn * Vn[k] * g[nn + 1] * rw - P * dg[nn + 1] * r[k] -
+                                                P * g[nn + 1] * wt[k]);
+                            for (int q = 0; q < 35; ++q) {
+                                int a = mono_abc[q][0], b = mono_abc[q][1], c = mono_abc[q][2];
+                                if (a + b + c != nn) continue;
+                                double s = 0.0;
+                                if (a) s += a * wt[0] * ipow(r[0], a - 1) * ipow(r[1], b) * ipow(r[2], c);
+                                if (b) s += b * wt[1] * ipow(r[0], a) * ipow(r[1], b - 1) * ipow(r[2], c);
+                                if (c) s += c * wt[2] * ipow(r[0], a) * ipow(r[1], b) * ipow(r[2], c - 1);
+                                Gl[q] += inv * (g[nn] * s - rw * g[nn + 1] * ipow(r[0], a) *
+                                                                  ipow(r[1], b) * ipow(r[2], c));
+                            }
+                        }
+                    }
+            for (int i = 0; i < n_ewt; ++i) {
+                const double *e = ewt + 5 * i;
+                double th = e[0] * d[0] + e[1] * d[1] + e[2] * d[2];
+                double c = cos(th), s = sin(th);
+                double kw = e[0] * wt[0] + e[1] * wt[1] + e[2] * wt[2];
+                double f = kw * (e[3] * c + e[4] * s);
+                for (int k = 0; k < 3; ++k) tt[k] += e[k] * f;
+                ABl[2 * i] += kw * s;
+                ABl[2 * i + 1] -= kw * c;
+            }
+            for (int k = 0; k < 3; ++k) {
+                grad[4 * t + k] += tt[k];
+                Vl[k] += tt[k];
+            }
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            for (int q = 0; q < 35; ++q) G[q] += Gl[q];
+            for (int k = 0; k < 3; ++k) V[k] += Vl[k];
+            for (int i = 0; i < 2 * n_ewt; ++i) AB[i] += ABl[i];
+        }
+        free(ABl);
+    }
+    /* k space: hC = -w_h (M - Q2.kk/2 + Q4.k^4/24), hS = w_h Q3.k^3/6 */
+    for (int i = 0; i < n_ewt; ++i) {
+        const double *e = ewt + 5 * i;
+        double k2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+        double wh = 4 * M_PI / (L * L * L) * exp(-k2 / (4 * alpha2)) / k2;
+        double A = AB[2 * i], B = AB[2 * i + 1];
+        for (int q = 0; q < 35; ++q) {
+            int a = mono_abc[q][0], b = mono_abc[q][1], c = mono_abc[q][2];
+            int o = a + b + c;
+            double kp = ipow(e[0], a) * ipow(e[1], b) * ipow(e[2], c);
+            if (o == 0) G[q] += -A * wh;
+            else if (o == 2) G[q] += A * wh * 0.5 * kp;
+            else if (o == 3) G[q] += B * wh / 6.0 * kp;
+            else if (o == 4) G[q] += -A * wh / 24.0 * kp;
+        }
+    }
+    free(AB);
+    /* H coefficients: h_0 = G_0, h_al = mult(al) STF(G_n)_al */
+    memset(hroot, 0, sizeof(double) * 35);
+    hroot[0] = G[0];
+    for (int nn = 2; nn <= 4; ++nn) {
+        double T[81], S[81];
+        int dgt[4];
+        for (int i = 0; i < p3(nn); ++i) {
+            digits(i, nn, dgt);
+            int cnt[3] = {0, 0, 0};
+            for (int p = 0; p < nn; ++p) cnt[dgt[p]]++;
+            T[i] = G[mono_idx[cnt[0]][cnt[1]][cnt[2]]];
+        }
+        stf_project(T, nn, S);
+        for (int q = 0; q < 35; ++q) {
+            int a = mono_abc[q][0], b = mono_abc[q][1], c = mono_abc[q][2];
+            if (a + b + c != nn) continue;
+            int dd[4], p = 0;
+            for (int i = 0; i < a; ++i) dd[p++] = 0;
+            for (int i = 0; i < b; ++i) dd[p++] = 1;
+            for (int i = 0; i < c; ++i) dd[p++] = 2;
+            hroot[q] = multinom(a, b, c) * S[flat(dd, nn)];
+        }
+    }
+    /* U = (sum_k m_k grad H(s_k) + V) / M; sum_k m_k d_j H = sum_al h_al al_j C(al - e_j) */
+    for (int j = 0; j < 3; ++j) {
+        double S = 0.0;
+        for (int q = 0; q < 35; ++q) {
+            int a = mono_abc[q][0], b = mono_abc[q][1], c = mono_abc[q][2];
+            int e[3] = {a, b, c};
+            if (e[j] == 0 || a + b + c < 3) continue;
+            e[j]--;
+            S += hroot[q] * (j == 0 ? a : j == 1 ? b : c) * C.v[e[0]][e[1]][e[2]];
+        }
+        hroot[mono_idx[j == 0][j == 1][j == 2]] = -(S + V[j]) / root_mass;
     }
 }

@@ -132,7 +132,15 @@ void orc_adjoint_lists(int n_nodes, const int32_t *node_child, int n_buckets,
                        const double *pos, const double *mass, const double *soft,
                        const int32_t *cells, const int64_t *cell_marks,
                        const int32_t *parts, const int64_t *part_marks, double period,
-                       const double *w, int n_threads, double *grad);
+                       const double *w, const double *root_extra, int n_threads, double *grad);
+/* Adjoint of orc_bucket_ewald over all n particles (w: 3 per particle, zero
+   for non-targets): grad[4t..4t+2] += the target-side terms; hroot[35]
+   receives the root's polynomial (monomial coefficients about the root's
+   centre of mass), to be passed as root_extra to orc_adjoint_lists. */
+void orc_adjoint_ewald(int n, const double *pos, const double *w, const double root_cm[3],
+                       double root_mass, const double q_root[ORC_NMOM], double u_root,
+                       const double *ewt, int n_ewt, double L, int n_reps, double ew_cut,
+                       int n_threads, double *grad, double hroot[35]);
 /* Exact gradient of L for the softened direct sum over all n particles. */
 void orc_direct_grad(int n, const double *pos, const double *mass, const double *soft,
                      const double *w, int n_threads, double *grad);

@@ -55,7 +55,8 @@ def lib():
     L.orc_direct_sum.argtypes = [_i, _p, _p, _p, _i, _p, _i, _p, _p]
     L.orc_ewald_exact.argtypes = [_i, _p, _p, _p, _i, _p, _d, _d, _d, _i, _i, _p, _p]
     L.orc_adjoint_lists.argtypes = [_i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
-                                    _p, _d, _p, _i, _p]
+                                    _p, _d, _p, _p, _i, _p]
+    L.orc_adjoint_ewald.argtypes = [_i, _p, _p, _p, _d, _p, _d, _p, _i, _d, _i, _d, _i, _p, _p]
     L.orc_direct_grad.argtypes = [_i, _p, _p, _p, _p, _i, _p]
     L.orc_rinv_derivs.argtypes = [_p, _i, _p]
     _lib = L
@@ -239,21 +240,41 @@ def ewald_exact(pos, mass, soft, targets, L=1.0, alpha=None, rcut=None, hmax=Non
     return acc, pot
 
 
-def adjoint_lists(tree, geo, mom, pos, mass, soft, lists, w, period=0.0, n_threads=0):
+def adjoint_lists(tree, geo, mom, pos, mass, soft, lists, w, period=0.0, n_threads=0,
+                  root_extra=None):
     """Exact gradient of L = sum_i w_i . a_i for the list evaluation with
     softened cells as flagged in the lists (forces_lists(..., soft_flags=True)).
-    Returns grad[N, 4]: dL/dx, dL/dm."""
+    root_extra: 35 monomial coefficients added to the root's polynomial (the
+    source side of the Ewald term, adjoint_ewald). Returns grad[N, 4]:
+    dL/dx, dL/dm."""
     n = pos.shape[0]
     grad = np.zeros((n, 4))
     wv = np.ascontiguousarray(w, dtype=np.float64)
+    rx = None if root_extra is None else np.ascontiguousarray(root_extra, dtype=np.float64)
     lib().orc_adjoint_lists(tree["n_nodes"], _ptr(tree["node_child"]), tree["n_buckets"],
                             _ptr(tree["bucket_node"]), _ptr(tree["node_first"]),
                             _ptr(tree["node_count"]), _ptr(geo), _ptr(mom), _ptr(pos),
                             _ptr(mass), _ptr(soft), _ptr(lists["cells"]),
                             _ptr(lists["cell_marks"]), _ptr(lists["parts"]),
-                            _ptr(lists["part_marks"]), float(period), _ptr(wv), n_threads,
-                            _ptr(grad))
+                            _ptr(lists["part_marks"]), float(period), _ptr(wv), _ptr(rx),
+                            n_threads, _ptr(grad))
     return grad
+
+
+def adjoint_ewald(pos, w, geo, mom, ewt, L, n_reps, ew_cut, n_threads=0):
+    """Adjoint of bucket_ewald over every particle (w zero for non-targets).
+    Returns (grad[N, 4] with the target-side terms, hroot[35])."""
+    n = pos.shape[0]
+    grad = np.zeros((n, 4))
+    hroot = np.zeros(35)
+    wv = np.ascontiguousarray(w, dtype=np.float64)
+    e = np.ascontiguousarray(ewt, dtype=np.float64)
+    cm = np.ascontiguousarray(geo[0, 0:3])
+    q = np.ascontiguousarray(mom[0])
+    lib().orc_adjoint_ewald(n, _ptr(pos), _ptr(wv), _ptr(cm), float(geo[0, 5]), _ptr(q),
+                            float(geo[0, 3]), _ptr(e), len(e), float(L), int(n_reps),
+                            float(ew_cut), n_threads, _ptr(grad), _ptr(hroot))
+    return grad, hroot
 
 
 def direct_grad(pos, mass, soft, w, n_threads=0):

@@ -21,10 +21,15 @@ from tests.helpers import summary
 pytestmark = pytest.mark.gpu
 
 
-def _oracle_grad(ev, w):
+def _oracle_grad(ev, w, ewald=False):
     geo, mom = O.build_moments(ev.tree, ev.pos, ev.mass, ev.soft)
-    return O.adjoint_lists(ev.tree, geo, mom, ev.pos, ev.mass, ev.soft, ev.lists, w,
-                           period=ev.L if ev.w["periodic"] else 0.0)
+    ge, hroot = None, None
+    if ewald:
+        ge, hroot = O.adjoint_ewald(ev.pos, w, geo, mom, ev.ewt64, ev.L, ev.w["n_replicas"],
+                                    ev.w["ew_cut"])
+    g = O.adjoint_lists(ev.tree, geo, mom, ev.pos, ev.mass, ev.soft, ev.lists, w,
+                        period=ev.L if ev.w["periodic"] else 0.0, root_extra=hroot)
+    return g if ge is None else g + ge
 
 
 def _errors(g, go):
@@ -49,7 +54,7 @@ def test_gpu_adjoint_matches_oracle(device, case):
     if case == "softened":
         assert ev.counts[1] > 0
     wv = np.random.default_rng(21).standard_normal(ev.pos.shape)
-    g = ev.adjoint(wv).cpu().numpy()
+    g = ev.adjoint(wv, ewald=False).cpu().numpy()
     torch.cuda.synchronize()
     go = _oracle_grad(ev, wv)
     ex, em = _errors(g, go)
@@ -57,6 +62,29 @@ def test_gpu_adjoint_matches_oracle(device, case):
     for e in (ex, em):
         assert np.median(e) <= 2e-5 and np.percentile(e, 99) <= 5e-4, summary(e)
     # translation invariance of the tree force
+    tot = np.abs(g[:, :3].sum(axis=0)).max() / np.abs(g[:, :3]).sum(axis=0).max()
+    assert tot < 1e-5, tot
+
+
+@pytest.mark.parametrize("active", [False, True])
+def test_gpu_adjoint_ewald_matches_oracle(device, active):
+    """Periodic box, L over tree plus Ewald acceleration: the Ewald term's
+    target side, and its source side through the root's moments, centre of
+    mass and h-table, against the oracle's adjoint_ewald added to the list
+    adjoint (root_extra)."""
+    import torch
+    w = configs.zeldovich(ng=16, rms=0.05, active_frac=0.3 if active else None)
+    ev = engine.Evaluation(w, precise=True, device=device)
+    wt = np.random.default_rng(22).standard_normal((ev.n_targets, 3))
+    g = ev.adjoint(wt).cpu().numpy()
+    torch.cuda.synchronize()
+    wv = np.zeros_like(ev.pos)
+    wv[ev.t_index if ev.t_index is not None else slice(None)] = wt
+    go = _oracle_grad(ev, wv, ewald=True)
+    ex, em = _errors(g, go)
+    print(f"adjoint ewald active={active}: dL/dx {summary(ex)} dL/dm {summary(em)}", flush=True)
+    for e in (ex, em):
+        assert np.median(e) <= 2e-5 and np.percentile(e, 99) <= 5e-4, summary(e)
     tot = np.abs(g[:, :3].sum(axis=0)).max() / np.abs(g[:, :3]).sum(axis=0).max()
     assert tot < 1e-5, tot
 

@@ -192,3 +192,90 @@ def test_adjoint_converges_to_direct_gradient():
     assert errs[2][0] < errs[1][0] < errs[0][0], errs
     assert errs[2][1] < errs[1][1] < errs[0][1], errs
     assert errs[2][0] < 1e-4 and errs[2][1] < 1e-4, errs
+
+
+def _periodic_parts(w, pos, mass, soft, tr):
+    geo, mom = O.build_moments(tr, pos, mass, soft)
+    ewt = O.ewald_table(mom[0], geo[0, 3], w["L"], w["ewh_cut"])
+    return geo, mom, ewt
+
+
+def _loss_periodic(w, tr, pos, mass, soft, lists, wv):
+    geo, mom, ewt = _periodic_parts(w, pos, mass, soft, tr)
+    acc, _, _, _ = O.forces_lists(tr, geo, mom, pos, mass, soft, lists, period=w["L"],
+                                  soft_flags=True)
+    ea, _, _ = O.bucket_ewald(np.arange(pos.shape[0]), pos, geo[0, 0:3], geo[0, 5], mom[0],
+                              geo[0, 3], ewt, w["L"], w["n_replicas"], w["ew_cut"])
+    return float((wv * (acc + ea)).sum())
+
+
+def _grad_periodic(w, tr, pos, mass, soft, lists, wv):
+    geo, mom, ewt = _periodic_parts(w, pos, mass, soft, tr)
+    ge, hroot = O.adjoint_ewald(pos, wv, geo, mom, ewt, w["L"], w["n_replicas"], w["ew_cut"])
+    g = O.adjoint_lists(tr, geo, mom, pos, mass, soft, lists, wv, period=w["L"],
+                        root_extra=hroot)
+    return g + ge
+
+
+def test_adjoint_with_ewald_finite_differences():
+    """Tree with replicas plus the Ewald correction: moments, the h-table and
+    every term rebuilt from the perturbed particles."""
+    w = configs.zeldovich(ng=12, rms=0.05)
+    tr, pos, mass, soft, lists, _ = _setup(w)
+    rng = np.random.default_rng(9)
+    wv = rng.standard_normal(pos.shape)
+    g = _grad_periodic(w, tr, pos, mass, soft, lists, wv)
+    sx, sm = np.abs(g[:, :3]).max(), np.abs(g[:, 3]).max()
+    for k in rng.choice(pos.shape[0], size=8, replace=False):
+        fd = np.zeros(4)
+        for c in range(3):
+            p1, p2 = pos.copy(), pos.copy()
+            p1[k, c] += 1e-5
+            p2[k, c] -= 1e-5
+            fd[c] = (_loss_periodic(w, tr, p1, mass, soft, lists, wv) -
+                     _loss_periodic(w, tr, p2, mass, soft, lists, wv)) / 2e-5
+        h = 1e-4 * mass[k]
+        m1, m2 = mass.copy(), mass.copy()
+        m1[k] += h
+        m2[k] -= h
+        fd[3] = (_loss_periodic(w, tr, pos, m1, soft, lists, wv) -
+                 _loss_periodic(w, tr, pos, m2, soft, lists, wv)) / (2 * h)
+        ex = np.abs(fd[:3] - g[k, :3]).max() / sx
+        em = abs(fd[3] - g[k, 3]) / sm
+        assert ex < 2e-7 and em < 2e-7, (k, fd, g[k], ex, em)
+
+
+def test_adjoint_with_ewald_translation_invariance():
+    w = configs.zeldovich(ng=12, rms=0.05)
+    tr, pos, mass, soft, lists, _ = _setup(w)
+    wv = np.random.default_rng(10).standard_normal(pos.shape)
+    g = _grad_periodic(w, tr, pos, mass, soft, lists, wv)
+    tot = g[:, :3].sum(axis=0)
+    assert np.abs(tot).max() < 1e-11 * np.abs(g[:, :3]).sum(axis=0).max(), tot
+
+
+def test_adjoint_with_ewald_series_region():
+    """A target inside the Ewald series region (|x - cm_root|^2 < 1.2e-3 L^2,
+    where the central replica uses the two-term series): its derivative
+    follows the series' own dependence on r^2."""
+    w = dict(configs.zeldovich(ng=12, rms=0.05))
+    pos = w["pos"].copy()
+    others = np.delete(np.arange(pos.shape[0]), 0)
+    cm = pos[others].mean(axis=0)
+    pos[0] = cm + np.array([0.012, -0.018, 0.01]) * w["L"]
+    w["pos"] = pos
+    tr, tpos, mass, soft, lists, _ = _setup(w)
+    geo, mom = O.build_moments(tr, tpos, mass, soft)
+    d2 = ((tpos - geo[0, 0:3]) ** 2).sum(axis=1)
+    k = int(d2.argmin())
+    assert d2[k] < 1.2e-3 * w["L"] ** 2
+    wv = np.random.default_rng(11).standard_normal(tpos.shape)
+    g = _grad_periodic(w, tr, tpos, mass, soft, lists, wv)
+    sx = np.abs(g[:, :3]).max()
+    for c in range(3):
+        p1, p2 = tpos.copy(), tpos.copy()
+        p1[k, c] += 1e-6
+        p2[k, c] -= 1e-6
+        fd = (_loss_periodic(w, tr, p1, mass, soft, lists, wv) -
+              _loss_periodic(w, tr, p2, mass, soft, lists, wv)) / 2e-6
+        assert abs(fd - g[k, c]) / sx < 2e-7, (c, fd, g[k, c])
