@@ -11,10 +11,20 @@ timed and reported.
 Workloads: --config 2 (default), the 128^3 periodic Zel'dovich box
 (2,097,152 particles, Ewald nReplicas = 1, fEwCut = 2.6, |h|^2 <= 10,
 theta = 0.7, bucket 16); --config 3, 2^22 particles in 512 Hernquist halos
-plus a 20% uniform background (theta = 0.6, bucket 32, isolated); --config 5,
-the 256^3 box (rms 0.15) with a rung of 10% active particles, the only
-targets. The metric is G interactions/s, an interaction being one (active
-target, source cell / particle / Ewald term) pair as ChaNGa counts them.
+plus a 20% uniform background (theta = 0.6, bucket 32, isolated); --config 4,
+a Plummer sphere of 2^20 particles (eps 0.01, theta 0.7, bucket 16);
+--config 5, the 256^3 box (rms 0.15) with a rung of 10% active particles,
+the only targets. The metric is G interactions/s, an interaction being one
+(active target, source cell / particle / Ewald term) pair as ChaNGa counts
+them.
+
+Adjoint (configs 4 and 5, or --adjoint): the time of changaGravityAdjoint
+for L = sum w . a (w ~ N(0,1)^3 per target) on the same lists; --kdk adds one
+kick-drift-kick step of forward forces and adjoint with only the positions
+kept between the passes (trees, lists and moments rebuilt in the backward
+pass); --grad-accuracy adds the gradient error against the FP64 oracle's
+adjoint on the same lists (and, for config 4, against the FP64 direct-sum
+gradient of a 4,096-particle Plummer sphere).
 
 Prints one JSON line. --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -41,7 +51,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--form", choices=["precise", "plain"], default="precise")
     ap.add_argument("--path", choices=["batched", "per-bucket"], default="batched")
-    ap.add_argument("--config", type=int, choices=[2, 3, 5], default=2)
+    ap.add_argument("--config", type=int, choices=[2, 3, 4, 5], default=2)
     ap.add_argument("--ng", type=int, default=128, help="lattice size of the periodic box")
     ap.add_argument("--n", type=int, default=1 << 22, help="particles of the halo workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -51,6 +61,12 @@ def parse():
                          "(default 1 for config 2, 1/4 for 3, 1/10 for 5: about 10-30 s of "
                          "CPU work)")
     ap.add_argument("--samples", type=int, default=256)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time the adjoint (default for configs 4 and 5)")
+    ap.add_argument("--kdk", action="store_true",
+                    help="one kick-drift-kick step of forward + adjoint, checkpointed")
+    ap.add_argument("--grad-accuracy", action="store_true",
+                    help="gradient error against the FP64 oracle (minutes of CPU work)")
     return ap.parse_args()
 
 
@@ -80,6 +96,12 @@ def main():
                     "nReplicas=1 fEwCut=2.6 |h|^2<=10, theta=0.7, bucket 16, hexadecapole")
         data = "synthetic (seeded Zel'dovich lattice and Bernoulli(0.1) rung, numpy PCG64 seed 1005)"
         tag = f"zeldovich{ng}_active"
+    elif args.config == 4:
+        w = configs.workload(4)
+        workload = ("Plummer sphere, 1048576 equal-mass particles (a=1, total mass 1), eps=0.01 "
+                    "spline softening, theta=0.7, bucket 16, hexadecapole, isolated")
+        data = "synthetic (seeded Plummer sphere, numpy PCG64 seed 1004)"
+        tag = "plummer1048576"
     else:
         w = configs.halos(n=args.n, cfg=3)
         workload = (f"{args.n} particles: 512 Hernquist halos (dN/dM~M^-1.9 over 3 decades, "
@@ -186,10 +208,191 @@ def main():
     if not args.no_accuracy:
         out["accuracy"] = accuracy(ev, ev.step_batched if args.path == "batched" else ev.step,
                                    args.samples)
+    if args.adjoint or args.kdk or args.grad_accuracy or args.config in (4, 5):
+        out["adjoint"] = adjoint_leg(ev, w, args)
     if not args.no_cpu:
-        frac = args.cpu_frac if args.cpu_frac is not None else {2: 1.0, 3: 0.25, 5: 0.1}[args.config]
+        frac = args.cpu_frac if args.cpu_frac is not None else {2: 1.0, 3: 0.25, 4: 0.25,
+                                                                5: 0.1}[args.config]
         out["cpu_baseline"] = cpu_baseline(ev, frac)
     print(json.dumps(out))
+
+
+def _events_ms(fn, steps):
+    import torch
+    stream = torch.cuda.current_stream()
+    fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def _target_original_index(ev):
+    """Original (workload) index of each target, in target order."""
+    perm = ev.tree["perm"]
+    return perm if ev.t_index is None else perm[ev.t_index]
+
+
+def adjoint_leg(ev, w, args):
+    """Adjoint time on the evaluation's lists, optionally one checkpointed
+    kick-drift-kick step and the gradient error against the FP64 oracle."""
+    import torch
+
+    rng = np.random.default_rng(12)
+    wt = rng.standard_normal((ev.n_targets, 3))
+    wt_dev = torch.tensor(wt, dtype=torch.float32, device=ev.device)
+    steps = max(1, min(args.steps, 10))
+    fwd_ms = _events_ms(ev.step_batched, steps)
+    adj_ms = _events_ms(lambda: ev.adjoint(wt_dev), steps)
+    out = {"loss": "L = sum_targets w . a, w ~ N(0,1)^3 (seed 12)",
+           "forward_ms": round(fwd_ms, 4), "adjoint_ms": round(adj_ms, 4),
+           "adjoint_over_forward": round(adj_ms / fwd_ms, 2),
+           "includes_ewald": bool(w["periodic"])}
+    if w["periodic"]:
+        out["adjoint_tree_only_ms"] = round(
+            _events_ms(lambda: ev.adjoint(wt_dev, ewald=False), steps), 4)
+    if args.grad_accuracy:
+        out["grad_accuracy"] = grad_accuracy(ev, wt, w)
+    if args.kdk:
+        out["kdk"] = kdk_step(w)
+    return out
+
+
+def kdk_step(w, dt=0.01, return_grad=False):
+    """One kick-drift-kick step x0 -> x1 (active particles kicked, every
+    particle drifted; periodic positions wrapped), L = sum_active w . v1,
+    forward then backward. Between the passes only the positions are kept:
+    the backward pass rebuilds each tree, its lists and moments (checkpoint
+    and recompute), then runs the adjoint. GPU time by HIP events, host time
+    (tree, lists, upload) by the wall clock."""
+    import torch
+
+    from changa_amd import engine
+
+    n = w["pos"].shape[0]
+    act = w.get("active")
+    aidx = np.arange(n) if act is None else np.nonzero(act)[0]
+    rng = np.random.default_rng(14)
+    v0 = 0.01 * rng.standard_normal((n, 3))
+    wv = rng.standard_normal((len(aidx), 3))
+    host_s, gpu_ms = 0.0, 0.0
+    stream = torch.cuda.current_stream()
+
+    def timed_gpu(fn):
+        nonlocal gpu_ms
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        r = fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        gpu_ms += a.elapsed_time(b)
+        return r
+
+    def build(x):
+        nonlocal host_s
+        t = time.time()
+        ev = engine.Evaluation(dict(w, pos=x), precise=True)
+        torch.cuda.synchronize()
+        host_s += time.time() - t
+        return ev
+
+    def forces(x):
+        ev = build(x)
+        timed_gpu(ev.step)
+        acc, _, _ = ev.results()
+        a = np.empty((n, 3))
+        a[ev.tree["perm"]] = acc
+        del ev
+        return a[aidx]
+
+    def adjoint(x, w_act):
+        ev = build(x)
+        # w_act is per active particle (original order); targets in target order
+        orig = _target_original_index(ev)
+        pos_in_act = np.empty(n, dtype=np.int64)
+        pos_in_act[aidx] = np.arange(len(aidx))
+        wt = w_act[pos_in_act[orig]]
+        g = timed_gpu(lambda: ev.adjoint(wt)).cpu().numpy()
+        out = np.empty((n, 4))
+        out[ev.tree["perm"]] = g
+        del ev
+        return out
+
+    x0 = w["pos"]
+    box = float(w["L"]) if w["periodic"] else None
+    a0 = forces(x0)
+    vh = v0.copy()
+    vh[aidx] += 0.5 * dt * a0
+    x1 = x0 + dt * vh
+    if box is not None:
+        x1 = np.mod(x1, box)
+        x1[x1 >= box] = 0.0
+    a1 = forces(x1)
+    # backward: L = sum wv . (vh + dt/2 a1)
+    g1 = adjoint(x1, 0.5 * dt * wv)
+    dl_da0 = 0.5 * dt * wv + 0.5 * dt * dt * g1[aidx, :3]
+    g0 = adjoint(x0, dl_da0)
+    grad = g0 + g1
+    info = {"steps": "forward F(x0), F(x1); backward adjoint at x1, then at x0",
+            "dt": dt, "gpu_ms": round(gpu_ms, 3), "host_s": round(host_s, 2),
+            "grad_norm_x": float(np.linalg.norm(grad[:, :3])),
+            "grad_norm_m": float(np.linalg.norm(grad[:, 3]))}
+    return (info, grad) if return_grad else info
+
+
+def grad_accuracy(ev, wt, w, n_samples=4096):
+    """GPU gradient (L = sum w . a on this evaluation's lists) against the
+    FP64 oracle's exact adjoint on the same lists, on seeded particles; for
+    the isolated Plummer sphere also the tree adjoint of a 4,096-particle
+    sphere against the FP64 direct-sum gradient."""
+    import torch
+
+    from changa_amd import configs, engine
+    from oracle import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    g = ev.adjoint(torch.tensor(wt, dtype=torch.float32, device=ev.device)).cpu().numpy()
+    wfull = np.zeros_like(ev.pos)
+    wfull[ev.t_index if ev.t_index is not None else slice(None)] = wt
+    t = time.time()
+    geo, mom = O.build_moments(ev.tree, ev.pos, ev.mass, ev.soft)
+    hroot, ge = None, None
+    if w["periodic"]:
+        ge, hroot = O.adjoint_ewald(ev.pos, wfull, geo, mom, ev.ewt64, ev.L, w["n_replicas"],
+                                    w["ew_cut"], n_threads=threads)
+    go = O.adjoint_lists(ev.tree, geo, mom, ev.pos, ev.mass, ev.soft, ev.lists, wfull,
+                         period=ev.L, n_threads=threads, root_extra=hroot)
+    if ge is not None:
+        go = go + ge
+    oracle_s = time.time() - t
+    samp = np.random.default_rng(15).choice(ev.pos.shape[0], size=min(n_samples, ev.pos.shape[0]),
+                                            replace=False)
+    ex = (np.linalg.norm(g[samp, :3] - go[samp, :3], axis=1) /
+          np.linalg.norm(go[samp, :3], axis=1))
+    em = np.abs(g[samp, 3] - go[samp, 3]) / np.abs(go[samp, 3])
+
+    def st(e):
+        return {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)),
+                "max": float(e.max())}
+
+    out = {"vs_oracle_same_lists": {"samples": int(len(samp)), "dL_dx": st(ex), "dL_dm": st(em),
+                                    "oracle_s": round(oracle_s, 1), "cores": threads}}
+    if not w["periodic"]:
+        ws = configs.plummer(n=4096, cfg=4)
+        es = engine.Evaluation(ws, precise=True, device=ev.device)
+        wv = np.random.default_rng(16).standard_normal((4096, 3))
+        gs = es.adjoint(wv).cpu().numpy()
+        gd = O.direct_grad(es.pos, es.mass, es.soft, wv, n_threads=threads)
+        ex = np.linalg.norm(gs[:, :3] - gd[:, :3], axis=1) / np.linalg.norm(gd[:, :3], axis=1)
+        em = np.abs(gs[:, 3] - gd[:, 3]) / np.abs(gd[:, 3])
+        out["vs_fp64_direct_4096"] = {"dL_dx": st(ex), "dL_dm": st(em)}
+    return out
 
 
 def traffic_from_profile(tag, kernel):
