@@ -34,6 +34,7 @@
 // (one wave per child, one lane per polynomial coefficient); (5) evaluation
 // at the particles.
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_run_length_encode.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include "ewald_body.h"
@@ -47,7 +48,7 @@ constexpr int kAdjStride = 112;  // doubles per node: D[36], Vs[3], pad, Hhat[35
 constexpr int kAdjVs = 36;
 constexpr int kAdjHhat = 40;
 constexpr int kAdjPoly = 75;
-constexpr int kUsersPerItem = 128;
+constexpr int kUsersPerItem = 512;
 constexpr int kAdjPcTile = 64;
 
 // ---------------------------------------------------------------------------
@@ -254,6 +255,19 @@ struct Transpose {
   int* item;
 };
 
+// Transposed particle lists (ILPart runs), scratch: value[R] (the runs as
+// sorted by their first source particle: bucket << 32 | replica bits (31..22)
+// | run length), key[nruns] (distinct first source particles), start[nruns +
+// 1] (users of key k at value[start[k], start[k + 1])), nruns (device),
+// item[nruns + 1] (work items of up to kRunUsersPerItem users).
+struct PartTranspose {
+  const unsigned long long* value;
+  const int* key;
+  const int* start;
+  const int* nruns;
+  const int* item;  // first work item of key k (exclusive scan of its chunks)
+};
+
 struct AdjArgs {
   ChangaGravityLaunch pc, pp;
   const float* w;
@@ -261,6 +275,7 @@ struct AdjArgs {
   double* grad;
   double* node;  // kAdjStride doubles per node
   Transpose tr;
+  PartTranspose ptr;
   int nNodes;
   int entries;
 };
@@ -393,6 +408,151 @@ __global__ __launch_bounds__(64) void adj_cells_kernel(AdjArgs a) {
     }
     if (lane < 39 && mine != 0.0f) atomicAdd(nd + lane, (double)mine);
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (2b) particle sources, from the particle lists transposed by source run:
+// one wavefront per work item (up to kRunUsersPerItem users of one run
+// start, a source bucket); the users' targets are staged in LDS 64 at a
+// time; lanes form groups of P >= run length lanes, one source particle per
+// lane, each group taking every G-th staged target. Per source particle and
+// item: the sum over targets of m_s (b w - c d (d.w)) and b (d.w),
+// d = x_s - x_t, added into grad (FP64 atomics; one per particle and item).
+// ---------------------------------------------------------------------------
+constexpr unsigned kRunLenMask = 0x3fffffu;
+// A source bucket with at most nMinParticleNode particles is opened by every
+// walk, so its run has every bucket as a user: users are split into items.
+constexpr int kRunUsersPerItem = 256;
+
+__global__ void adj_pchunks_kernel(const int* count, const int* nruns, int n, int* chunks) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) chunks[i] = i < *nruns ? (count[i] + kRunUsersPerItem - 1) / kRunUsersPerItem : 0;
+}
+
+__global__ __launch_bounds__(64) void adj_pkeys_kernel(ChangaGravityLaunch pp, unsigned* key,
+                                                       unsigned long long* value) {
+  const int b = blockIdx.x;
+  const ILPart* lst = static_cast<const ILPart*>(pp.list);
+  for (int r = pp.marks[b] + (int)threadIdx.x; r < pp.marks[b + 1]; r += kWave) {
+    const ILPart q = lst[r];
+    key[r] = (unsigned)q.index;
+    value[r] = ((unsigned long long)(unsigned)b << 32) | ((unsigned)q.off & ~kRunLenMask) |
+               ((unsigned)q.num & kRunLenMask);
+  }
+}
+
+__global__ __launch_bounds__(64) void adj_pp_sources_kernel(AdjArgs a) {
+  __shared__ float4 st_pos[kWave];  // staged target: x, y, z relative to the run's origin, soft
+  __shared__ float4 st_w[kWave];    // w, run length of its user (as int bits)
+  __shared__ int u_end[kWave];
+  __shared__ int u_first[kWave];
+  __shared__ unsigned long long u_val[kWave];
+  const int lane = threadIdx.x;
+  const int nruns = *a.ptr.nruns;
+  const int n_items = a.ptr.item[nruns];
+  const double L = a.pp.fperiod;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    // the run of this item: last r with item[r] <= it
+    int lo = 0, hi = nruns - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.ptr.item[mid] <= it) lo = mid;
+      else hi = mid - 1;
+    }
+    const int k = a.ptr.key[lo];
+    const int v0 = a.ptr.start[lo] + (it - a.ptr.item[lo]) * kRunUsersPerItem;
+    const int v1 = min(v0 + kRunUsersPerItem, a.ptr.start[lo + 1]);
+    // longest run among the users (runs sharing a start normally have one length)
+    int maxlen = 0;
+    for (int u = v0 + lane; u < v1; u += kWave)
+      maxlen = max(maxlen, (int)(a.ptr.value[u] & kRunLenMask));
+#pragma unroll
+    for (int m = kWave / 2; m >= 1; m >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, m, kWave));
+    int P = 1;
+    while (P < maxlen && P < kWave) P <<= 1;
+    const int G = kWave / P, grp = lane / P, sl = lane % P;
+    const double ox = a.pp.spos64[3 * (size_t)k], oy = a.pp.spos64[3 * (size_t)k + 1],
+                 oz = a.pp.spos64[3 * (size_t)k + 2];
+    for (int s0 = 0; s0 < maxlen; s0 += P) {
+      const int si = s0 + sl;  // source index within the run
+      const bool src_live = si < maxlen;
+      const int p = k + (src_live ? si : 0);
+      const CompactPartData q = a.pp.sources[p];
+      const float sx = (float)(a.pp.spos64[3 * (size_t)p] - ox);
+      const float sy = (float)(a.pp.spos64[3 * (size_t)p + 1] - oy);
+      const float sz = (float)(a.pp.spos64[3 * (size_t)p + 2] - oz);
+      const float sm = q.mass, ss = q.soft;
+      float gx = 0.f, gy = 0.f, gz = 0.f, gm = 0.f;
+      for (int ub = v0; ub < v1; ub += kWave) {
+        const int nu = min(kWave, v1 - ub);
+        int sz_u = 0;
+        if (lane < nu) {
+          const unsigned long long val = a.ptr.value[ub + lane];
+          const int bk = (int)(val >> 32);
+          sz_u = a.pc.sizes[bk];
+          u_first[lane] = a.pc.starts[bk];
+          u_val[lane] = val;
+        }
+        const int incl = wave_incl_scan(sz_u, lane);
+        u_end[lane] = incl;
+        const int total = __shfl(incl, kWave - 1, kWave);
+        __syncthreads();
+        for (int q0 = 0; q0 < total; q0 += kWave) {
+          const int qq = q0 + lane;
+          if (qq < total) {
+            const int uk = find_run(u_end, nu, qq);
+            const int t = u_first[uk] + qq - (uk ? u_end[uk - 1] : 0);
+            const unsigned long long val = u_val[uk];
+            const int oid = (int)(unsigned)(val & 0xffffffffull);
+            // d = x_s + off L - x_t  =>  target relative to the run's origin: x_t - off L - o
+            st_pos[lane] = make_float4(
+                (float)(a.pc.tpos64[3 * (size_t)t] - off_x(oid) * L - ox),
+                (float)(a.pc.tpos64[3 * (size_t)t + 1] - off_y(oid) * L - oy),
+                (float)(a.pc.tpos64[3 * (size_t)t + 2] - off_z(oid) * L - oz),
+                a.pc.targets[t].soft);
+            st_w[lane] = make_float4(a.w[3 * (size_t)t], a.w[3 * (size_t)t + 1],
+                                     a.w[3 * (size_t)t + 2],
+                                     __int_as_float((int)(val & kRunLenMask)));
+          }
+          __syncthreads();
+          const int ns = min(kWave, total - q0);
+          if (src_live) {
+            for (int j = grp; j < ns; j += G) {
+              const float4 tq = st_pos[j], wv = st_w[j];
+              if (si >= __float_as_int(wv.w)) continue;
+              const float dx = sx - tq.x, dy = sy - tq.y, dz = sz - tq.z;
+              const float r2 = dx * dx + dy * dy + dz * dz;
+              if (r2 == 0.0f) continue;
+              float bb, cc;
+              spline_bc(r2, tq.w + ss, bb, cc);
+              const float dw = dx * wv.x + dy * wv.y + dz * wv.z;
+              gx += sm * (bb * wv.x - cc * dx * dw);
+              gy += sm * (bb * wv.y - cc * dy * dw);
+              gz += sm * (bb * wv.z - cc * dz * dw);
+              gm += bb * dw;
+            }
+          }
+          __syncthreads();
+        }
+      }
+      // sum the G groups (lanes sl, sl + P, ...), then one update per source
+#pragma unroll
+      for (int m = kWave / 2; m >= 1; m >>= 1) {
+        if (m < P) break;
+        gx += __shfl_xor(gx, m, kWave);
+        gy += __shfl_xor(gy, m, kWave);
+        gz += __shfl_xor(gz, m, kWave);
+        gm += __shfl_xor(gm, m, kWave);
+      }
+      if (grp == 0 && src_live) {
+        double* g = a.grad + 4 * (size_t)p;
+        atomicAdd(g, (double)gx);
+        atomicAdd(g + 1, (double)gy);
+        atomicAdd(g + 2, (double)gz);
+        atomicAdd(g + 3, (double)gm);
+      }
+    }
   }
 }
 
@@ -536,6 +696,8 @@ __global__ __launch_bounds__(64) void adj_targets_kernel(AdjArgs a) {
       atomicAdd(g + 2, (double)gz);
       atomicAdd(g + 3, (double)gm);
     };
+    // (run lists: the source side is adj_pp_sources_kernel's, over the
+    // transposed lists)
     if (KIND == 0) {
       const ILCell* lst = static_cast<const ILCell*>(a.pp.list);
       for (int base = pps; base < ppe; base += kWave) {
@@ -543,34 +705,6 @@ __global__ __launch_bounds__(64) void adj_targets_kernel(AdjArgs a) {
         ILCell c = {0, 0};
         if (live) c = lst[base + lane];
         source_pass(c.index, c.offsetID, live);
-      }
-    } else {
-      const ILPart* lst = static_cast<const ILPart*>(a.pp.list);
-      for (int rb = pps; rb < ppe; rb += kWave) {
-        const int nr = min(kWave, ppe - rb);
-        int num = 0;
-        if (lane < nr) {
-          ILPart q = lst[rb + lane];
-          ps.r_idx[lane] = q.index;
-          ps.r_oid[lane] = q.off;
-          num = q.num;
-        }
-        const int incl = wave_incl_scan(num, lane);
-        ps.r_end[lane] = incl;
-        const int total = __shfl(incl, kWave - 1, kWave);
-        __syncthreads();
-        for (int sb = 0; sb < total; sb += kWave) {
-          const int q = sb + lane;
-          const bool live = q < total;
-          int p = 0, oid = 0;
-          if (live) {
-            const int k = find_run(ps.r_end, nr, q);
-            p = ps.r_idx[k] + (q - (k ? ps.r_end[k - 1] : 0));
-            oid = ps.r_oid[k];
-          }
-          source_pass(p, oid, live);
-        }
-        __syncthreads();
       }
     }
 
@@ -609,15 +743,6 @@ __host__ __device__ constexpr int mono(int a, int b, int c) {
 __host__ __device__ constexpr double inv_fact(int n) {
   return n <= 1 ? 1.0 : n == 2 ? 0.5 : n == 3 ? 1.0 / 6.0 : n == 4 ? 1.0 / 24.0 : 1.0 / 120.0;
 }
-
-__constant__ int kMonoAbc[35][3] = {
-    {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {1, 1, 0}, {1, 0, 1},
-    {0, 2, 0}, {0, 1, 1}, {0, 0, 2}, {3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0},
-    {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}, {4, 0, 0},
-    {3, 1, 0}, {3, 0, 1}, {2, 2, 0}, {2, 1, 1}, {2, 0, 2}, {1, 3, 0}, {1, 2, 1},
-    {1, 1, 2}, {1, 0, 3}, {0, 4, 0}, {0, 3, 1}, {0, 2, 2}, {0, 1, 3}, {0, 0, 4}};
-__constant__ double kBinom[5][5] = {{1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0},
-                                    {1, 3, 3, 1, 0}, {1, 4, 6, 4, 1}};
 
 // trace-free completion over monomial slots of orders 2..NMAX:
 // X(a,b,c>=2) = -X(a+2,b,c-2) - X(a,b+2,c-2)
@@ -695,34 +820,50 @@ __global__ void adj_hhat_kernel(int n_nodes, double* node, const double* node64)
     for (int q = 0; q < 35; ++q) nd[kAdjPoly + q] = H[q];
 }
 
-__device__ __forceinline__ double ipow_small(double x, int e) {
-  double r = 1.0;
-  for (int i = 0; i < e; ++i) r *= x;
-  return r;
+// P(s) -> P(s + d) in place, for a polynomial of degree <= 4 in monomial
+// slots: one Taylor shift per axis (repeated synthetic division along each
+// line of the other two exponents), 35 FMAs per axis.
+template <int AXIS>
+__device__ __forceinline__ void taylor_shift_axis(double* P, double d) {
+#pragma unroll
+  for (int u = 0; u <= 4; ++u)
+#pragma unroll
+    for (int v = 0; v <= 4 - u; ++v) {
+      const int n = 4 - u - v;  // degree along AXIS on this line
+#pragma unroll
+      for (int k = 0; k < n; ++k)
+#pragma unroll
+        for (int e = n - 1; e >= k; --e) {
+          const int lo = AXIS == 0 ? mono(e, u, v) : AXIS == 1 ? mono(u, e, v) : mono(u, v, e);
+          const int hi = AXIS == 0 ? mono(e + 1, u, v)
+                                   : AXIS == 1 ? mono(u, e + 1, v) : mono(u, v, e + 1);
+          P[lo] = fma(d, P[hi], P[lo]);
+        }
+    }
 }
 
-// children of the nodes [lo, lo + gridDim.x / 2): one wave per (parent,
-// child slot), lane q computes coefficient q of
+// children of the nodes [lo, hi): one thread per (parent, child slot),
 // P_child(s) = Hhat_child(s) + P_parent(s + cm_child - cm_parent)
-__global__ __launch_bounds__(64) void adj_shift_kernel(int lo, const int* child, double* node,
-                                                       const double* node64) {
-  const int parent = lo + (int)(blockIdx.x >> 1);
-  const int c = child[2 * parent + (blockIdx.x & 1)];
-  const int q = threadIdx.x;
-  if (c < 0 || q >= 35) return;
+__global__ __launch_bounds__(128) void adj_shift_kernel(int lo, int hi, const int* child,
+                                                        double* node, const double* node64) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int parent = lo + (i >> 1);
+  if (parent >= hi) return;
+  const int c = child[2 * parent + (i & 1)];
+  if (c < 0) return;
   const double* gp = node64 + (size_t)parent * 34;
   const double* gc = node64 + (size_t)c * 34;
-  const double d0 = gc[0] - gp[0], d1 = gc[1] - gp[1], d2 = gc[2] - gp[2];
-  const int i = kMonoAbc[q][0], j = kMonoAbc[q][1], k = kMonoAbc[q][2];
   const double* Pp = node + (size_t)parent * kAdjStride + kAdjPoly;
-  double sum = node[(size_t)c * kAdjStride + kAdjHhat + q];
-  for (int t = q; t < 35; ++t) {
-    const int a = kMonoAbc[t][0], b = kMonoAbc[t][1], cc = kMonoAbc[t][2];
-    if (a < i || b < j || cc < k) continue;
-    sum += Pp[t] * kBinom[a][i] * kBinom[b][j] * kBinom[cc][k] * ipow_small(d0, a - i) *
-           ipow_small(d1, b - j) * ipow_small(d2, cc - k);
-  }
-  node[(size_t)c * kAdjStride + kAdjPoly + q] = sum;
+  double P[35];
+#pragma unroll
+  for (int q = 0; q < 35; ++q) P[q] = Pp[q];
+  taylor_shift_axis<0>(P, gc[0] - gp[0]);
+  taylor_shift_axis<1>(P, gc[1] - gp[1]);
+  taylor_shift_axis<2>(P, gc[2] - gp[2]);
+  const double* Hc = node + (size_t)c * kAdjStride + kAdjHhat;
+  double* Pc = node + (size_t)c * kAdjStride + kAdjPoly;
+#pragma unroll
+  for (int q = 0; q < 35; ++q) Pc[q] = P[q] + Hc[q];
 }
 
 // bucket particles: dL/dm += P(s), dL/dx += m grad P(s)
@@ -1073,13 +1214,24 @@ __global__ __launch_bounds__(kEwAdjBlock) void adj_ewald_kernel(EwAdjArgs a) {
   }
 }
 
-// totals over blocks (deterministic order)
-__global__ void adj_ewald_sum_kernel(const double* part, int nblocks, int nvals, double* total) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nvals) return;
+// totals over blocks, one workgroup per value: fixed strided partial sums,
+// then a fixed-order tree in LDS (deterministic)
+constexpr int kEwSumBlock = 256;
+__global__ __launch_bounds__(kEwSumBlock) void adj_ewald_sum_kernel(const double* part,
+                                                                    int nblocks, int nvals,
+                                                                    double* total) {
+  __shared__ double sh[kEwSumBlock];
+  const int v = blockIdx.x, tid = threadIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * nvals + v];
-  total[v] = s;
+  for (int b = tid; b < nblocks; b += kEwSumBlock) s += part[(size_t)b * nvals + v];
+  sh[tid] = s;
+  __syncthreads();
+#pragma unroll
+  for (int m = kEwSumBlock / 2; m >= 1; m >>= 1) {
+    if (tid < m) sh[tid] += sh[tid + m];
+    __syncthreads();
+  }
+  if (tid == 0) total[v] = sh[0];
 }
 
 // Hhat of the Ewald term about the root's centre of mass, added to node 0
@@ -1152,6 +1304,11 @@ __global__ void adj_ewald_root_kernel(const double* total, const EwtData* ewt, E
   }
 }
 
+__global__ void fill_zero_int_kernel(int* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 __global__ void fill_zero_kernel(double* p, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
@@ -1166,17 +1323,27 @@ static int zero(double* p, size_t n, hipStream_t s) {
 
 // Scratch layout of listWork (offsets in ints, 256-byte aligned segments).
 struct WorkLayout {
-  size_t key, key_out, val, val_out, start, chunks, item, temp, total;
+  size_t key, key_out, val, val_out, start, chunks, item;
+  size_t pkey, pkey_out, pval, pval_out, puniq, pcount, pstart, pnruns, pchunks, pitem;
+  size_t temp, total;
   size_t temp_bytes;
-  unsigned end_bit;
+  unsigned end_bit, p_end_bit;
 };
 
 static size_t align64(size_t x) { return (x + 63) & ~(size_t)63; }
 
-static hipError_t work_layout(int n_nodes, int entries, WorkLayout& L) {
-  L.end_bit = 1;
-  while ((1u << L.end_bit) < (unsigned)n_nodes && L.end_bit < 32) ++L.end_bit;
-  size_t sort_bytes = 0, scan_bytes = 0;
+static unsigned bits_for(long long n) {
+  unsigned b = 1;
+  while ((1ll << b) < n && b < 32) ++b;
+  return b;
+}
+
+// part_entries: runs of the particle list when it is an ILPart list, else 0
+static hipError_t work_layout(int n_nodes, int entries, int part_entries, int n_particles,
+                              WorkLayout& L) {
+  L.end_bit = bits_for(n_nodes);
+  L.p_end_bit = bits_for(n_particles);
+  size_t sort_bytes = 0, scan_bytes = 0, psort_bytes = 0, rle_bytes = 0, pscan_bytes = 0;
   hipError_t e = rocprim::radix_sort_pairs(
       nullptr, sort_bytes, (const unsigned*)nullptr, (unsigned*)nullptr,
       (const unsigned long long*)nullptr, (unsigned long long*)nullptr, (size_t)entries, 0,
@@ -1185,24 +1352,49 @@ static hipError_t work_layout(int n_nodes, int entries, WorkLayout& L) {
   e = rocprim::exclusive_scan(nullptr, scan_bytes, (const int*)nullptr, (int*)nullptr, 0,
                               (size_t)n_nodes + 1, rocprim::plus<int>());
   if (e != hipSuccess) return e;
-  L.temp_bytes = sort_bytes > scan_bytes ? sort_bytes : scan_bytes;
+  if (part_entries > 0) {
+    e = rocprim::radix_sort_pairs(nullptr, psort_bytes, (const unsigned*)nullptr,
+                                  (unsigned*)nullptr,
+                                  (const unsigned long long*)nullptr,
+                                  (unsigned long long*)nullptr, (size_t)part_entries, 0,
+                                  L.p_end_bit);
+    if (e != hipSuccess) return e;
+    e = rocprim::run_length_encode(nullptr, rle_bytes, (const unsigned*)nullptr,
+                                   (size_t)part_entries, (unsigned*)nullptr, (int*)nullptr,
+                                   (int*)nullptr);
+    if (e != hipSuccess) return e;
+    e = rocprim::exclusive_scan(nullptr, pscan_bytes, (const int*)nullptr, (int*)nullptr, 0,
+                                (size_t)part_entries + 1, rocprim::plus<int>());
+    if (e != hipSuccess) return e;
+  }
+  size_t tb = sort_bytes;
+  for (size_t x : {scan_bytes, psort_bytes, rle_bytes, pscan_bytes}) tb = x > tb ? x : tb;
+  L.temp_bytes = tb;
+  const size_t E = (size_t)entries, R = (size_t)part_entries;
   size_t o = 0;
-  L.key = o;
-  o = align64(o + (size_t)entries);
-  L.key_out = o;
-  o = align64(o + (size_t)entries);
-  L.val = o;
-  o = align64(o + 2 * (size_t)entries);
-  L.val_out = o;
-  o = align64(o + 2 * (size_t)entries);
-  L.start = o;
-  o = align64(o + (size_t)n_nodes + 1);
-  L.chunks = o;
-  o = align64(o + (size_t)n_nodes + 1);
-  L.item = o;
-  o = align64(o + (size_t)n_nodes + 1);
-  L.temp = o;
-  o = align64(o + (L.temp_bytes + 3) / 4);
+  auto take = [&](size_t ints) {
+    const size_t at = o;
+    o = align64(o + ints);
+    return at;
+  };
+  L.key = take(E);
+  L.key_out = take(E);
+  L.val = take(2 * E);
+  L.val_out = take(2 * E);
+  L.start = take((size_t)n_nodes + 1);
+  L.chunks = take((size_t)n_nodes + 1);
+  L.item = take((size_t)n_nodes + 1);
+  L.pkey = take(R);
+  L.pkey_out = take(R);
+  L.pval = take(2 * R);
+  L.pval_out = take(2 * R);
+  L.puniq = take(R);
+  L.pcount = take(R + 1);
+  L.pstart = take(R + 1);
+  L.pnruns = take(1);
+  L.pchunks = take(R + 1);
+  L.pitem = take(R + 1);
+  L.temp = take((L.temp_bytes + 3) / 4);
   L.total = o;
   return hipSuccess;
 }
@@ -1211,10 +1403,11 @@ static hipError_t work_layout(int n_nodes, int entries, WorkLayout& L) {
 
 using namespace cgpu;
 
-extern "C" size_t changaGravityAdjointWorkInts(int nNodes, int numCellEntries) {
-  if (nNodes <= 0 || numCellEntries < 0) return 0;
+extern "C" size_t changaGravityAdjointWorkInts(int nNodes, int numCellEntries,
+                                               int numPartEntries, int nParticles) {
+  if (nNodes <= 0 || numCellEntries < 0 || numPartEntries < 0 || nParticles <= 0) return 0;
   WorkLayout L;
-  if (work_layout(nNodes, numCellEntries, L) != hipSuccess) return 0;
+  if (work_layout(nNodes, numCellEntries, numPartEntries, nParticles, L) != hipSuccess) return 0;
   return L.total;
 }
 
@@ -1253,7 +1446,7 @@ static int launch_ewald_adjoint(const ChangaAdjointLaunch* a, hipStream_t s) {
   double* total = a->ewaldWork + (size_t)nb * nvals;
   hipLaunchKernelGGL(adj_ewald_kernel, dim3(nb), dim3(kEwAdjBlock), shm, s, ea);
   CGPU_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(adj_ewald_sum_kernel, dim3((unsigned)((nvals + 127) / 128)), dim3(128), 0, s,
+  hipLaunchKernelGGL(adj_ewald_sum_kernel, dim3((unsigned)nvals), dim3(kEwSumBlock), 0, s,
                      a->ewaldWork, nb, (int)nvals, total);
   CGPU_CHECK(hipGetLastError());
   hipLaunchKernelGGL(adj_ewald_root_kernel, dim3(1), dim3(kWave), 0, s, total, a->ewt, p,
@@ -1270,7 +1463,7 @@ extern "C" size_t changaGravityAdjointEwaldWorkDoubles(int nTargets, int nH) {
 extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) {
   if (!a || !a->w || !a->grad || !a->nodeWork || !a->listWork || !a->levelOffsets || !a->first ||
       !a->count || !a->child || !a->node64 || !a->pos64 || !a->mass64 || a->nNodes <= 0 ||
-      a->nDepths <= 0 || a->nParticles <= 0 || a->numCellEntries < 0)
+      a->nDepths <= 0 || a->nParticles <= 0 || a->numCellEntries < 0 || a->numPartEntries < 0)
     return fail("changaGravityAdjoint: missing argument", hipErrorInvalidValue, __FILE__, __LINE__);
   const ChangaGravityLaunch& pc = a->cells;
   const ChangaGravityLaunch& pp = a->parts;
@@ -1280,8 +1473,12 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
       pc.fperiod != pp.fperiod || pc.listKind != 0 || (pp.listKind != 0 && pp.listKind != 1))
     return fail("changaGravityAdjoint: cell/particle launches (precision form, same buckets)",
                 hipErrorInvalidValue, __FILE__, __LINE__);
+  // the ILPart particle list is transposed by source run (its source side
+  // needs no atomics per pair); per-particle ILCell lists keep the per-bucket
+  // source pass
+  const int R = pp.listKind == 1 ? a->numPartEntries : 0;
   WorkLayout W;
-  CGPU_CHECK(work_layout(a->nNodes, a->numCellEntries, W));
+  CGPU_CHECK(work_layout(a->nNodes, a->numCellEntries, R, a->nParticles, W));
   if (a->listWorkInts < W.total)
     return fail("changaGravityAdjoint: listWork too small", hipErrorInvalidValue, __FILE__,
                 __LINE__);
@@ -1308,6 +1505,11 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
   args.tr.start = iw + W.start;
   args.tr.chunks = iw + W.chunks;
   args.tr.item = iw + W.item;
+  args.ptr.value = reinterpret_cast<const unsigned long long*>(iw + W.pval_out);
+  args.ptr.key = iw + W.puniq;
+  args.ptr.start = iw + W.pstart;
+  args.ptr.nruns = iw + W.pnruns;
+  args.ptr.item = iw + W.pitem;
   // (1) transpose: entries sorted by cell, stable (bucket order within a cell)
   if (nb > 0) {
     hipLaunchKernelGGL(adj_keys_kernel, dim3(nb), dim3(kWave), 0, s, pc, key, val);
@@ -1330,6 +1532,36 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
     CGPU_CHECK(rocprim::exclusive_scan(iw + W.temp, tb, args.tr.chunks, args.tr.item, 0,
                                        (size_t)nn + 1, rocprim::plus<int>(), s));
   }
+  // (1b) runs sorted by first source particle, stable; distinct starts and
+  // their user ranges
+  if (R > 0) {
+    unsigned* pkey = reinterpret_cast<unsigned*>(iw + W.pkey);
+    unsigned* pkey_out = reinterpret_cast<unsigned*>(iw + W.pkey_out);
+    unsigned long long* pval = reinterpret_cast<unsigned long long*>(iw + W.pval);
+    unsigned long long* pval_out = reinterpret_cast<unsigned long long*>(iw + W.pval_out);
+    int* pcount = iw + W.pcount;
+    hipLaunchKernelGGL(adj_pkeys_kernel, dim3(nb), dim3(kWave), 0, s, pp, pkey, pval);
+    CGPU_CHECK(hipGetLastError());
+    size_t tb = W.temp_bytes;
+    CGPU_CHECK(rocprim::radix_sort_pairs(iw + W.temp, tb, pkey, pkey_out, pval, pval_out,
+                                         (size_t)R, 0, W.p_end_bit, s));
+    hipLaunchKernelGGL(fill_zero_int_kernel, dim3((unsigned)((R + 1 + 255) / 256)), dim3(256), 0,
+                       s, pcount, (size_t)R + 1);
+    CGPU_CHECK(hipGetLastError());
+    tb = W.temp_bytes;
+    CGPU_CHECK(rocprim::run_length_encode(iw + W.temp, tb, pkey_out, (size_t)R,
+                                          reinterpret_cast<unsigned*>(iw + W.puniq), pcount,
+                                          iw + W.pnruns, s));
+    tb = W.temp_bytes;
+    CGPU_CHECK(rocprim::exclusive_scan(iw + W.temp, tb, pcount, iw + W.pstart, 0, (size_t)R + 1,
+                                       rocprim::plus<int>(), s));
+    hipLaunchKernelGGL(adj_pchunks_kernel, dim3((unsigned)((R + 1 + 255) / 256)), dim3(256), 0, s,
+                       pcount, iw + W.pnruns, R + 1, iw + W.pchunks);
+    CGPU_CHECK(hipGetLastError());
+    tb = W.temp_bytes;
+    CGPU_CHECK(rocprim::exclusive_scan(iw + W.temp, tb, iw + W.pchunks, iw + W.pitem, 0,
+                                       (size_t)R + 1, rocprim::plus<int>(), s));
+  }
   // (2) cell sources, (3) target side and particle pairs
   if (nb > 0) {
     int dev = 0, ncu = 0;
@@ -1342,6 +1574,10 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
     else
       hipLaunchKernelGGL(adj_targets_kernel<0>, dim3(nb), dim3(kWave), 0, s, args);
     CGPU_CHECK(hipGetLastError());
+    if (R > 0) {
+      hipLaunchKernelGGL(adj_pp_sources_kernel, dim3(ncu * 16), dim3(kWave), 0, s, args);
+      CGPU_CHECK(hipGetLastError());
+    }
   }
   // (4) Hhat per node (plus the Ewald term's at the root), then top-down
   hipLaunchKernelGGL(adj_hhat_kernel, dim3((nn + 127) / 128), dim3(128), 0, s, nn, a->nodeWork,
@@ -1352,8 +1588,8 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
   for (int d = 0; d < a->nDepths; ++d) {
     const int lo = a->levelOffsets[d], hi = a->levelOffsets[d + 1];
     if (hi <= lo) continue;
-    hipLaunchKernelGGL(adj_shift_kernel, dim3(2 * (hi - lo)), dim3(kWave), 0, s, lo, a->child,
-                       a->nodeWork, a->node64);
+    hipLaunchKernelGGL(adj_shift_kernel, dim3((2 * (hi - lo) + 127) / 128), dim3(128), 0, s, lo,
+                       hi, a->child, a->nodeWork, a->node64);
     CGPU_CHECK(hipGetLastError());
   }
   // (5) particles

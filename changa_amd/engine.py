@@ -337,10 +337,11 @@ class Evaluation:
         tr = self.tree
         grad = torch.empty((n, 4), dtype=torch.float64, device=self.device)
         entries = int(self.t_lists["cell_marks"][-1])
+        part_entries = int(self.t_lists["part_marks"][-1])
         if getattr(self, "d_node_work", None) is None:
             self.d_node_work = torch.empty((tr["n_nodes"], 112), dtype=torch.float64,
                                            device=self.device)
-            ints = gpu.lib().changaGravityAdjointWorkInts(tr["n_nodes"], entries)
+            ints = gpu.lib().changaGravityAdjointWorkInts(tr["n_nodes"], entries, part_entries, n)
             if ints == 0:
                 raise RuntimeError("changaGravityAdjointWorkInts failed")
             self.d_list_work = torch.empty(ints, dtype=torch.int32, device=self.device)
@@ -355,6 +356,7 @@ class Evaluation:
         a.listWork = self.d_list_work.data_ptr()
         a.listWorkInts = self.d_list_work.numel()
         a.numCellEntries = entries
+        a.numPartEntries = part_entries
         a.nNodes = tr["n_nodes"]
         a.nDepths = tr["n_depths"]
         a.nParticles = n

@@ -77,7 +77,8 @@ class ChangaBatchOptions(ctypes.Structure):
 class ChangaAdjointLaunch(ctypes.Structure):
     _fields_ = [("cells", ChangaGravityLaunch), ("parts", ChangaGravityLaunch), ("w", c_void_p),
                 ("targetIndex", c_void_p), ("grad", c_void_p), ("nodeWork", c_void_p), ("listWork", c_void_p),
-                ("listWorkInts", c_size_t), ("numCellEntries", c_int), ("nNodes", c_int),
+                ("listWorkInts", c_size_t), ("numCellEntries", c_int), ("numPartEntries", c_int),
+                ("nNodes", c_int),
                 ("nDepths", c_int), ("nParticles", c_int), ("levelOffsets", c_void_p),
                 ("first", c_void_p), ("count", c_void_p), ("child", c_void_p),
                 ("node64", c_void_p), ("pos64", c_void_p), ("mass64", c_void_p),
@@ -157,7 +158,7 @@ def lib():
     L.changaBatchPlanDestroy.restype = c_int
     L.changaGravityAdjoint.argtypes = [ctypes.POINTER(ChangaAdjointLaunch), P]
     L.changaGravityAdjoint.restype = c_int
-    L.changaGravityAdjointWorkInts.argtypes = [c_int, c_int]
+    L.changaGravityAdjointWorkInts.argtypes = [c_int, c_int, c_int, c_int]
     L.changaGravityAdjointWorkInts.restype = c_size_t
     L.changaGravityAdjointEwaldWorkDoubles.argtypes = [c_int, c_int]
     L.changaGravityAdjointEwaldWorkDoubles.restype = c_size_t

@@ -220,9 +220,10 @@ typedef struct {
   double *grad;                   /* out, overwritten: dL/dx (3), dL/dm per particle (device) */
   double *nodeWork;               /* scratch: 112 doubles per node (device) */
   int *listWork;                  /* scratch: changaGravityAdjointWorkInts(nNodes,
-                                     numCellEntries) ints (device) */
+                                     numCellEntries, numPartEntries, nParticles) ints (device) */
   size_t listWorkInts;            /* its size in ints */
   int numCellEntries;             /* entries of the cell list (cells.marks[numBuckets]) */
+  int numPartEntries;             /* entries of the particle list (parts.marks[numBuckets]) */
   int nNodes, nDepths, nParticles;
   const int *levelOffsets;        /* host, nDepths + 1 (as for changaBuildMoments) */
   const int *first, *count, *child;  /* device, as for changaBuildMoments */
@@ -238,9 +239,12 @@ typedef struct {
   size_t ewaldWorkDoubles;           /* its size in doubles */
 } ChangaAdjointLaunch;
 
-/* Ints of listWork for a tree of nNodes nodes and a cell list of
-   numCellEntries entries (the cell list transposed per cell). */
-size_t changaGravityAdjointWorkInts(int nNodes, int numCellEntries);
+/* Ints of listWork for a tree of nNodes nodes, a cell list of numCellEntries
+   entries and a particle list of numPartEntries entries over nParticles
+   source particles (the cell list transposed per cell; an ILPart particle
+   list transposed per source run). */
+size_t changaGravityAdjointWorkInts(int nNodes, int numCellEntries, int numPartEntries,
+                                    int nParticles);
 /* Doubles of ewaldWork for nTargets targets and an h-table of nH entries. */
 size_t changaGravityAdjointEwaldWorkDoubles(int nTargets, int nH);
 int changaGravityAdjoint(const ChangaAdjointLaunch *a, void *stream);
