@@ -243,16 +243,18 @@ __device__ __forceinline__ int find_run(const int* ends, int n, int q) {
 }
 
 // Transposed cell lists (scratch):
-//   key[E], value[E] (the entries as sorted: cell, then {bucket, offsetID}
-//   packed as bucket << 32 | offsetID), start[nNodes + 1] (users of cell c at
-//   [start[c], start[c+1])), chunks[nNodes + 1], item[nNodes + 1] (work
-//   items per cell, exclusive prefix; item[nNodes] is their number).
+//   key[E], value[E] (the entries as sorted: cell, then {bucket, replica and
+//   softened bits of offsetID} packed in 32 bits), start[nNodes + 1] (users
+//   of cell c at [start[c], start[c+1])), chunks[nNodes + 1], item[nNodes + 1]
+//   (work items per cell, exclusive prefix; item[nNodes] is their number),
+//   item_cell[items] (the cell of each item).
 struct Transpose {
   const unsigned* key;
-  const unsigned long long* value;
+  const unsigned* value;  // bucket << kBucketShift | offsetID >> 22
   int* start;
   int* chunks;
   int* item;
+  int* item_cell;  // cell of each work item
 };
 
 // Transposed particle lists (ILPart runs), scratch: value[R] (the runs as
@@ -265,7 +267,8 @@ struct PartTranspose {
   const int* key;
   const int* start;
   const int* nruns;
-  const int* item;  // first work item of key k (exclusive scan of its chunks)
+  const int* item;      // first work item of key k (exclusive scan of its chunks)
+  const int* item_run;  // key index of each work item
 };
 
 struct AdjArgs {
@@ -283,14 +286,19 @@ struct AdjArgs {
 // ---------------------------------------------------------------------------
 // (1) transpose
 // ---------------------------------------------------------------------------
+// value: bucket << 10 | offsetID bits 31..22 (replica and softened flag;
+// bits 21..0 carry no information for a cell entry)
+constexpr int kBucketShift = 10;
+constexpr int kMaxAdjBuckets = 1 << 22;
+
 __global__ __launch_bounds__(64) void adj_keys_kernel(ChangaGravityLaunch pc, unsigned* key,
-                                                      unsigned long long* value) {
+                                                      unsigned* value) {
   const int b = blockIdx.x;
   const ILCell* cells = static_cast<const ILCell*>(pc.list);
   for (int e = pc.marks[b] + threadIdx.x; e < pc.marks[b + 1]; e += kWave) {
     const ILCell c = cells[e];
     key[e] = (unsigned)c.index;
-    value[e] = ((unsigned long long)(unsigned)b << 32) | (unsigned)c.offsetID;
+    value[e] = ((unsigned)b << kBucketShift) | ((unsigned)c.offsetID >> 22);
   }
 }
 
@@ -315,6 +323,15 @@ __global__ void adj_chunks_kernel(const int* start, int n_nodes, int* chunks) {
   if (c == n_nodes) chunks[c] = 0;
 }
 
+// owner of each work item: the owner's index at its first item (items of
+// owner c: item[c] .. item[c + 1] - 1), then a running maximum fills the rest
+__global__ void adj_item_heads_kernel(const int* item, const int* n_owners_dev, int n_owners,
+                                      int* owner) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = n_owners_dev ? *n_owners_dev : n_owners;
+  if (c < n && item[c + 1] > item[c]) owner[item[c]] = c;
+}
+
 // ---------------------------------------------------------------------------
 // (2) cell sources: one wavefront per work item (up to kUsersPerItem users of
 // one cell); lanes walk the users' targets.
@@ -328,14 +345,7 @@ __global__ __launch_bounds__(64) void adj_cells_kernel(AdjArgs a) {
   const float* wire = reinterpret_cast<const float*>(a.pc.moments);
   const double L = a.pc.fperiod;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    // the cell of this item: last c with item[c] <= it
-    int lo = 0, hi = a.nNodes - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (a.tr.item[mid] <= it) lo = mid;
-      else hi = mid - 1;
-    }
-    const int c = lo;
+    const int c = a.tr.item_cell[it];
     const int u0 = a.tr.start[c] + (it - a.tr.item[c]) * kUsersPerItem;
     const int nu = min(kUsersPerItem, a.tr.start[c + 1] - u0);
     const double cx = a.pc.cm64[3 * (size_t)c], cy = a.pc.cm64[3 * (size_t)c + 1],
@@ -347,11 +357,11 @@ __global__ __launch_bounds__(64) void adj_cells_kernel(AdjArgs a) {
       const int k = k0 + lane;
       int sz = 0;
       if (k < nu) {
-        const unsigned long long u = a.tr.value[u0 + k];
-        const int bk = (int)(u >> 32);
+        const unsigned u = a.tr.value[u0 + k];
+        const int bk = (int)(u >> kBucketShift);
         sz = a.pc.sizes[bk];
         u_start[k] = a.pc.starts[bk];
-        u_oid[k] = (int)(unsigned)(u & 0xffffffffull);
+        u_oid[k] = (int)((u & ((1u << kBucketShift) - 1u)) << 22);
       }
       const int incl = wave_incl_scan(sz, lane);
       if (k < nu) u_end[k] = base + incl;
@@ -453,13 +463,7 @@ __global__ __launch_bounds__(64) void adj_pp_sources_kernel(AdjArgs a) {
   const int n_items = a.ptr.item[nruns];
   const double L = a.pp.fperiod;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    // the run of this item: last r with item[r] <= it
-    int lo = 0, hi = nruns - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (a.ptr.item[mid] <= it) lo = mid;
-      else hi = mid - 1;
-    }
+    const int lo = a.ptr.item_run[it];
     const int k = a.ptr.key[lo];
     const int v0 = a.ptr.start[lo] + (it - a.ptr.item[lo]) * kRunUsersPerItem;
     const int v1 = min(v0 + kRunUsersPerItem, a.ptr.start[lo + 1]);
@@ -1321,10 +1325,26 @@ static int zero(double* p, size_t n, hipStream_t s) {
   return 0;
 }
 
+// owner[0, cap) of the work items whose per-owner first items are item[0 ..
+// n_owners] (n_owners read on the device when n_owners_dev is set)
+static int item_owners(const int* item, const int* n_owners_dev, int n_owners, int* heads,
+                       int* owner, size_t cap, void* temp, size_t temp_bytes, hipStream_t s) {
+  hipLaunchKernelGGL(fill_zero_int_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s,
+                     heads, cap);
+  CGPU_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(adj_item_heads_kernel, dim3((unsigned)((n_owners + 255) / 256)), dim3(256), 0,
+                     s, item, n_owners_dev, n_owners, heads);
+  CGPU_CHECK(hipGetLastError());
+  size_t tb = temp_bytes;
+  CGPU_CHECK(rocprim::inclusive_scan(temp, tb, heads, owner, cap, rocprim::maximum<int>(), s));
+  return 0;
+}
+
 // Scratch layout of listWork (offsets in ints, 256-byte aligned segments).
 struct WorkLayout {
-  size_t key, key_out, val, val_out, start, chunks, item;
-  size_t pkey, pkey_out, pval, pval_out, puniq, pcount, pstart, pnruns, pchunks, pitem;
+  size_t key, key_out, val, val_out, start, chunks, item, item_cell;
+  size_t pkey, pkey_out, pval, pval_out, puniq, pcount, pstart, pnruns, pchunks, pitem, pitem_run;
+  size_t item_cap, pitem_cap, item_heads, pitem_heads;
   size_t temp, total;
   size_t temp_bytes;
   unsigned end_bit, p_end_bit;
@@ -1344,10 +1364,9 @@ static hipError_t work_layout(int n_nodes, int entries, int part_entries, int n_
   L.end_bit = bits_for(n_nodes);
   L.p_end_bit = bits_for(n_particles);
   size_t sort_bytes = 0, scan_bytes = 0, psort_bytes = 0, rle_bytes = 0, pscan_bytes = 0;
-  hipError_t e = rocprim::radix_sort_pairs(
-      nullptr, sort_bytes, (const unsigned*)nullptr, (unsigned*)nullptr,
-      (const unsigned long long*)nullptr, (unsigned long long*)nullptr, (size_t)entries, 0,
-      L.end_bit);
+  hipError_t e = rocprim::radix_sort_pairs(nullptr, sort_bytes, (const unsigned*)nullptr,
+                                           (unsigned*)nullptr, (const unsigned*)nullptr,
+                                           (unsigned*)nullptr, (size_t)entries, 0, L.end_bit);
   if (e != hipSuccess) return e;
   e = rocprim::exclusive_scan(nullptr, scan_bytes, (const int*)nullptr, (int*)nullptr, 0,
                               (size_t)n_nodes + 1, rocprim::plus<int>());
@@ -1367,8 +1386,20 @@ static hipError_t work_layout(int n_nodes, int entries, int part_entries, int n_
                                 (size_t)part_entries + 1, rocprim::plus<int>());
     if (e != hipSuccess) return e;
   }
+  size_t own_bytes = 0, pown_bytes = 0;
+  const size_t item_cap = (size_t)n_nodes + (size_t)entries / kUsersPerItem + 1;
+  const size_t pitem_cap = (size_t)part_entries + (size_t)part_entries / kRunUsersPerItem + 1;
+  e = rocprim::inclusive_scan(nullptr, own_bytes, (const int*)nullptr, (int*)nullptr, item_cap,
+                              rocprim::maximum<int>());
+  if (e != hipSuccess) return e;
+  if (part_entries > 0) {
+    e = rocprim::inclusive_scan(nullptr, pown_bytes, (const int*)nullptr, (int*)nullptr,
+                                pitem_cap, rocprim::maximum<int>());
+    if (e != hipSuccess) return e;
+  }
   size_t tb = sort_bytes;
-  for (size_t x : {scan_bytes, psort_bytes, rle_bytes, pscan_bytes}) tb = x > tb ? x : tb;
+  for (size_t x : {scan_bytes, psort_bytes, rle_bytes, pscan_bytes, own_bytes, pown_bytes})
+    tb = x > tb ? x : tb;
   L.temp_bytes = tb;
   const size_t E = (size_t)entries, R = (size_t)part_entries;
   size_t o = 0;
@@ -1379,11 +1410,14 @@ static hipError_t work_layout(int n_nodes, int entries, int part_entries, int n_
   };
   L.key = take(E);
   L.key_out = take(E);
-  L.val = take(2 * E);
-  L.val_out = take(2 * E);
+  L.val = take(E);
+  L.val_out = take(E);
   L.start = take((size_t)n_nodes + 1);
   L.chunks = take((size_t)n_nodes + 1);
   L.item = take((size_t)n_nodes + 1);
+  L.item_cap = (size_t)n_nodes + E / kUsersPerItem + 1;
+  L.item_cell = take(L.item_cap);
+  L.item_heads = take(L.item_cap);
   L.pkey = take(R);
   L.pkey_out = take(R);
   L.pval = take(2 * R);
@@ -1394,6 +1428,9 @@ static hipError_t work_layout(int n_nodes, int entries, int part_entries, int n_
   L.pnruns = take(1);
   L.pchunks = take(R + 1);
   L.pitem = take(R + 1);
+  L.pitem_cap = R + R / kRunUsersPerItem + 1;
+  L.pitem_run = take(L.pitem_cap);
+  L.pitem_heads = take(L.pitem_cap);
   L.temp = take((L.temp_bytes + 3) / 4);
   L.total = o;
   return hipSuccess;
@@ -1473,6 +1510,9 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
       pc.fperiod != pp.fperiod || pc.listKind != 0 || (pp.listKind != 0 && pp.listKind != 1))
     return fail("changaGravityAdjoint: cell/particle launches (precision form, same buckets)",
                 hipErrorInvalidValue, __FILE__, __LINE__);
+  if (pc.numBuckets >= kMaxAdjBuckets)
+    return fail("changaGravityAdjoint: more than 2^22 buckets", hipErrorInvalidValue, __FILE__,
+                __LINE__);
   // the ILPart particle list is transposed by source run (its source side
   // needs no atomics per pair); per-particle ILCell lists keep the per-bucket
   // source pass
@@ -1489,8 +1529,8 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
   int* iw = a->listWork;
   unsigned* key = reinterpret_cast<unsigned*>(iw + W.key);
   unsigned* key_out = reinterpret_cast<unsigned*>(iw + W.key_out);
-  unsigned long long* val = reinterpret_cast<unsigned long long*>(iw + W.val);
-  unsigned long long* val_out = reinterpret_cast<unsigned long long*>(iw + W.val_out);
+  unsigned* val = reinterpret_cast<unsigned*>(iw + W.val);
+  unsigned* val_out = reinterpret_cast<unsigned*>(iw + W.val_out);
   AdjArgs args;
   args.pc = pc;
   args.pp = pp;
@@ -1505,11 +1545,13 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
   args.tr.start = iw + W.start;
   args.tr.chunks = iw + W.chunks;
   args.tr.item = iw + W.item;
+  args.tr.item_cell = iw + W.item_cell;
   args.ptr.value = reinterpret_cast<const unsigned long long*>(iw + W.pval_out);
   args.ptr.key = iw + W.puniq;
   args.ptr.start = iw + W.pstart;
   args.ptr.nruns = iw + W.pnruns;
   args.ptr.item = iw + W.pitem;
+  args.ptr.item_run = iw + W.pitem_run;
   // (1) transpose: entries sorted by cell, stable (bucket order within a cell)
   if (nb > 0) {
     hipLaunchKernelGGL(adj_keys_kernel, dim3(nb), dim3(kWave), 0, s, pc, key, val);
@@ -1531,6 +1573,9 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
     size_t tb = W.temp_bytes;
     CGPU_CHECK(rocprim::exclusive_scan(iw + W.temp, tb, args.tr.chunks, args.tr.item, 0,
                                        (size_t)nn + 1, rocprim::plus<int>(), s));
+    if (int e = item_owners(args.tr.item, nullptr, nn, iw + W.item_heads, args.tr.item_cell,
+                            W.item_cap, iw + W.temp, W.temp_bytes, s))
+      return e;
   }
   // (1b) runs sorted by first source particle, stable; distinct starts and
   // their user ranges
@@ -1561,6 +1606,9 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
     tb = W.temp_bytes;
     CGPU_CHECK(rocprim::exclusive_scan(iw + W.temp, tb, iw + W.pchunks, iw + W.pitem, 0,
                                        (size_t)R + 1, rocprim::plus<int>(), s));
+    if (int e = item_owners(iw + W.pitem, iw + W.pnruns, R, iw + W.pitem_heads, iw + W.pitem_run,
+                            W.pitem_cap, iw + W.temp, W.temp_bytes, s))
+      return e;
   }
   // (2) cell sources, (3) target side and particle pairs
   if (nb > 0) {
