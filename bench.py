@@ -44,6 +44,11 @@ PEAK_FP32_NONPACKED = 78.6e12
 FLOP = dict(pc=183, soft=23, pp=23, ewald_real=333, ewald_k=58)
 
 
+def log(msg):
+    """Progress of the long legs, on stderr (stdout carries the one JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +116,7 @@ def main():
         tag = f"halos{args.n}"
     ev = engine.Evaluation(w, precise=(args.form == "precise"))
     setup_s = time.time() - t0
+    log(f"workload set up in {setup_s:.1f} s")
     counts = ev.n_interactions()
     n_int = sum(counts.values())
     flops = sum(FLOP[k] * v for k, v in counts.items())
@@ -205,15 +211,19 @@ def main():
         },
     }
 
+    log(f"timed: {value:.1f} G interactions/s")
     if not args.no_accuracy:
         out["accuracy"] = accuracy(ev, ev.step_batched if args.path == "batched" else ev.step,
                                    args.samples)
+        log("accuracy leg done")
     if args.adjoint or args.kdk or args.grad_accuracy or args.config in (4, 5):
         out["adjoint"] = adjoint_leg(ev, w, args)
+        log("adjoint leg done")
     if not args.no_cpu:
         frac = args.cpu_frac if args.cpu_frac is not None else {2: 1.0, 3: 0.25, 4: 0.25,
                                                                 5: 0.1}[args.config]
         out["cpu_baseline"] = cpu_baseline(ev, frac)
+        log("cpu baseline done")
     print(json.dumps(out))
 
 
@@ -258,8 +268,10 @@ def adjoint_leg(ev, w, args):
             _events_ms(lambda: ev.adjoint(wt_dev, ewald=False), steps), 4)
     if args.grad_accuracy:
         out["grad_accuracy"] = grad_accuracy(ev, wt, w)
+        log("gradient accuracy done")
     if args.kdk:
         out["kdk"] = kdk_step(w)
+        log("kdk step done")
     return out
 
 
@@ -300,6 +312,7 @@ def kdk_step(w, dt=0.01, return_grad=False):
         ev = engine.Evaluation(dict(w, pos=x), precise=True)
         torch.cuda.synchronize()
         host_s += time.time() - t
+        log(f"kdk: tree, lists and moments in {time.time() - t:.1f} s")
         return ev
 
     def forces(x):
@@ -361,11 +374,13 @@ def grad_accuracy(ev, wt, w, n_samples=4096):
     wfull = np.zeros_like(ev.pos)
     wfull[ev.t_index if ev.t_index is not None else slice(None)] = wt
     t = time.time()
+    log("gradient accuracy: FP64 oracle adjoint")
     geo, mom = O.build_moments(ev.tree, ev.pos, ev.mass, ev.soft)
     hroot, ge = None, None
     if w["periodic"]:
         ge, hroot = O.adjoint_ewald(ev.pos, wfull, geo, mom, ev.ewt64, ev.L, w["n_replicas"],
                                     w["ew_cut"], n_threads=threads)
+        log(f"gradient accuracy: oracle Ewald adjoint done ({time.time() - t:.1f} s)")
     go = O.adjoint_lists(ev.tree, geo, mom, ev.pos, ev.mass, ev.soft, ev.lists, wfull,
                          period=ev.L, n_threads=threads, root_extra=hroot)
     if ge is not None:
@@ -381,6 +396,11 @@ def grad_accuracy(ev, wt, w, n_samples=4096):
         return {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)),
                 "max": float(e.max())}
 
+    def st_rms(d, ref):
+        """|error| relative to the rms of the reference over the samples
+        (the gradient has zeros, where a per-particle ratio is undefined)"""
+        return st(np.abs(d) / np.sqrt(np.mean(ref ** 2)))
+
     out = {"vs_oracle_same_lists": {"samples": int(len(samp)), "dL_dx": st(ex), "dL_dm": st(em),
                                     "oracle_s": round(oracle_s, 1), "cores": threads}}
     if not w["periodic"]:
@@ -391,7 +411,11 @@ def grad_accuracy(ev, wt, w, n_samples=4096):
         gd = O.direct_grad(es.pos, es.mass, es.soft, wv, n_threads=threads)
         ex = np.linalg.norm(gs[:, :3] - gd[:, :3], axis=1) / np.linalg.norm(gd[:, :3], axis=1)
         em = np.abs(gs[:, 3] - gd[:, 3]) / np.abs(gd[:, 3])
-        out["vs_fp64_direct_4096"] = {"dL_dx": st(ex), "dL_dm": st(em)}
+        ex_r = st_rms(np.linalg.norm(gs[:, :3] - gd[:, :3], axis=1),
+                      np.linalg.norm(gd[:, :3], axis=1))
+        em_r = st_rms(gs[:, 3] - gd[:, 3], gd[:, 3])
+        out["vs_fp64_direct_4096"] = {"theta": ws["theta"], "dL_dx": st(ex), "dL_dm": st(em),
+                                      "dL_dx_over_rms": ex_r, "dL_dm_over_rms": em_r}
     return out
 
 

@@ -1018,6 +1018,11 @@ void orc_adjoint_lists(int n_nodes, const int32_t *node_child, int n_buckets,
     for (int bk = 0; bk < n_buckets; ++bk) {
         const int bn = bucket_node[bk];
         const int first = node_first[bn], count = node_count[bn];
+        /* every term is linear in w: targets with w = 0 contribute nothing */
+        int live = 0;
+        for (int i = first; i < first + count && !live; ++i)
+            live = w[3 * i] != 0.0 || w[3 * i + 1] != 0.0 || w[3 * i + 2] != 0.0;
+        if (!live) continue;
         for (int64_t e = cell_marks[bk]; e < cell_marks[bk + 1]; ++e) {
             const int cn = cells[2 * e];
             double off[3];
@@ -1054,6 +1059,7 @@ void orc_adjoint_lists(int n_nodes, const int32_t *node_child, int n_buckets,
             memset(Dl, 0, sizeof(Dl));
             for (int i = first; i < first + count; ++i) {
                 const double *wi = w + 3 * i;
+                if (wi[0] == 0.0 && wi[1] == 0.0 && wi[2] == 0.0) continue;
                 double r[3] = {pos[3 * i] - cx, pos[3 * i + 1] - cy, pos[3 * i + 2] - cz};
                 orc_tder T;
                 rinv_derivs(r, 6, T);
