@@ -207,7 +207,8 @@ def main():
             "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s",
             "frac": round(achieved * 1e12 / PEAK_FP32, 4),
-            "traffic": traffic_from_profile(tag, dom),
+            "traffic": traffic_from_profile(
+                tag, f"batch_kernel<{opts.pcTile},{opts.ppTile}>" if dom == "batch_kernel" else dom),
         },
     }
 
@@ -420,7 +421,8 @@ def grad_accuracy(ev, wt, w, n_samples=4096):
 
 
 def traffic_from_profile(tag, kernel):
-    """HBM bytes per launch of `kernel`, from the committed rocprofv3 --pmc
+    """HBM bytes per launch of `kernel` (for the batched kernel, of its
+    variant, e.g. "batch_kernel<64,256>"), from the committed rocprofv3 --pmc
     summary of this workload (FETCH_SIZE and WRITE_SIZE, copy-calibrated)."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.md")
     if not os.path.exists(path):

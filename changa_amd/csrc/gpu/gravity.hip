@@ -159,7 +159,7 @@ __global__ __launch_bounds__(64) void pc_kernel_flat(ChangaGravityLaunch a) {
     for (int base = start; base < end; base += kPcTile) {
       const int nc = min(kPcTile, end - base);
       if (lane < nc) {
-        ILCell c = cells[base + lane];
+        const ILCell c = load_stream(cells + base + lane);
         s_idx[lane] = c.index;
         s_oid[lane] = c.offsetID;
       }

@@ -195,11 +195,30 @@ struct WireCell {
   int idx;
 };
 
+// List entries are read once per evaluation: streaming (non-temporal) loads,
+// so that they do not displace the cell records and particles that
+// neighbouring buckets reuse from L2.
+__device__ __forceinline__ ILCell load_stream(const ILCell* p) {
+  const long long v = __builtin_nontemporal_load(reinterpret_cast<const long long*>(p));
+  ILCell c;
+  c.index = (int)(unsigned)(v & 0xffffffffll);
+  c.offsetID = (int)(unsigned)((unsigned long long)v >> 32);
+  return c;
+}
+
+__device__ __forceinline__ ILPart load_stream(const ILPart* p) {
+  ILPart r;
+  r.index = __builtin_nontemporal_load(&p->index);
+  r.off = __builtin_nontemporal_load(&p->off);
+  r.num = __builtin_nontemporal_load(&p->num);
+  return r;
+}
+
 template <bool PRECISE>
 __device__ __forceinline__ void load_cell(const ChangaGravityLaunch& a, const ILCell* cells, int e,
                                           int end, WireCell& w) {
   if (e < end) {
-    ILCell c = cells[e];
+    const ILCell c = load_stream(cells + e);
     w.idx = c.index;
     w.oid = c.offsetID;
     const float* src = reinterpret_cast<const float*>(a.moments) + (size_t)c.index * 27;
@@ -408,7 +427,7 @@ __device__ __forceinline__ void pp_tiles(const ChangaGravityLaunch& a, int start
     for (int base = start; base < end; base += TILE) {
       const int ns = min(TILE, end - base);
       for (int i = lane; i < ns; i += kWave) {
-        ILCell c = lst[base + i];
+        const ILCell c = load_stream(lst + base + i);
         stage(i, c.index, c.offsetID);
       }
       __syncthreads();
@@ -421,7 +440,7 @@ __device__ __forceinline__ void pp_tiles(const ChangaGravityLaunch& a, int start
       const int nr = min(kWave, end - rb);
       int num = 0;
       if (lane < nr) {
-        ILPart p = lst[rb + lane];
+        const ILPart p = load_stream(lst + rb + lane);
         s.r_idx[lane] = p.index;
         s.r_oid[lane] = p.off;
         num = p.num;
