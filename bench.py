@@ -26,6 +26,11 @@ pass); --grad-accuracy adds the gradient error against the FP64 oracle's
 adjoint on the same lists (and, for config 4, against the FP64 direct-sum
 gradient of a 4,096-particle Plummer sphere).
 
+The interaction lists are built once, before the timed steps, on the GPU from
+the moment build's node records (--lists gpu, the default; changaBuildLists*)
+or by the host walk (--lists host); both give the same lists, and
+config.list_build_ms reports the GPU build's time.
+
 Prints one JSON line. --gpus N with N > 1 exits: this build runs on one GPU.
 """
 import argparse
@@ -66,6 +71,8 @@ def parse():
                          "(default 1 for config 2, 1/4 for 3, 1/10 for 5: about 10-30 s of "
                          "CPU work)")
     ap.add_argument("--samples", type=int, default=256)
+    ap.add_argument("--lists", choices=["gpu", "host"], default="gpu",
+                    help="interaction lists built on the GPU (default) or by the host walk")
     ap.add_argument("--adjoint", action="store_true",
                     help="time the adjoint (default for configs 4 and 5)")
     ap.add_argument("--kdk", action="store_true",
@@ -114,8 +121,16 @@ def main():
                     "hexadecapole")
         data = "synthetic (seeded halos, numpy PCG64 seed 1003)"
         tag = f"halos{args.n}"
-    ev = engine.Evaluation(w, precise=(args.form == "precise"))
+    ev = engine.Evaluation(w, precise=(args.form == "precise"), lists=args.lists)
     setup_s = time.time() - t0
+    list_ms = None
+    if args.lists == "gpu":
+        ev.build_lists()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ev.build_lists()
+        torch.cuda.synchronize()
+        list_ms = (time.perf_counter() - t1) * 1e3
     log(f"workload set up in {setup_s:.1f} s")
     counts = ev.n_interactions()
     n_int = sum(counts.values())
@@ -189,6 +204,8 @@ def main():
             "n_buckets": int(ev.tree["n_buckets"]),
             "n_nodes": int(ev.tree["n_nodes"]),
             "host_setup_s": round(setup_s, 2),
+            "lists": args.lists,
+            "list_build_ms": None if list_ms is None else round(list_ms, 2),
         },
         "step_ms": {k: round(v, 4) for k, v in step_ms.items()},
         "batched_options": {"pcTile": opts.pcTile, "ppTile": opts.ppTile,
@@ -310,7 +327,7 @@ def kdk_step(w, dt=0.01, return_grad=False):
     def build(x):
         nonlocal host_s
         t = time.time()
-        ev = engine.Evaluation(dict(w, pos=x), precise=True)
+        ev = engine.Evaluation(dict(w, pos=x), precise=True, lists="gpu")
         torch.cuda.synchronize()
         host_s += time.time() - t
         log(f"kdk: tree, lists and moments in {time.time() - t:.1f} s")

@@ -90,10 +90,16 @@ class Evaluation:
     only buckets holding an active particle keep their lists. Sources are
     always every particle."""
 
-    def __init__(self, w, precise=False, pp_compact=True, n_threads=0, device="cuda:0"):
+    def __init__(self, w, precise=False, pp_compact=True, n_threads=0, device="cuda:0",
+                 lists="host"):
+        if lists not in ("host", "gpu"):
+            raise ValueError("lists: 'host' or 'gpu'")
+        if lists == "gpu" and not pp_compact:
+            raise ValueError("GPU-built lists are ILPart runs (pp_compact=True)")
         self.w = w
         self.precise = precise
         self.pp_compact = pp_compact
+        self.list_source = lists
         self.pc_variant = 0
         self.device = torch.device(device)
         periodic = bool(w["periodic"])
@@ -108,12 +114,23 @@ class Evaluation:
         self.mass = np.ascontiguousarray(w["mass"][perm])
         self.soft = np.ascontiguousarray(w["soft"][perm])
         self.active = None if w.get("active") is None else np.ascontiguousarray(w["active"][perm])
-        self.geo = host.node_geometry(tr, self.pos, self.mass, self.soft)
-        self.lists = host.walk(tr, self.geo, w["theta"], w["n_replicas"],
-                               self.L if periodic else 1.0, n_threads)
-        self.counts = host.interaction_counts(tr, self.lists, self.active)
+        self._lists = None
+        if lists == "host":
+            self.geo = host.node_geometry(tr, self.pos, self.mass, self.soft)
+            self._lists = host.walk(tr, self.geo, w["theta"], w["n_replicas"],
+                                    self.L if periodic else 1.0, n_threads)
+            self.counts = host.interaction_counts(tr, self._lists, self.active)
         self._targets()
         self._upload()
+
+    @property
+    def lists(self):
+        """Interaction lists of every bucket of the tree (host arrays, the
+        host walk's layout); with GPU-built lists, downloaded on first use
+        (buckets without targets have empty lists)."""
+        if self._lists is None:
+            self._lists = self._download_lists()
+        return self._lists
 
     def _targets(self):
         tr = self.tree
@@ -121,7 +138,9 @@ class Evaluation:
             self.t_index = None
             self.n_targets = self.pos.shape[0]
             self.t_starts, self.t_sizes = tr["bucket_start"], tr["bucket_size"]
-            self.t_lists = self.lists
+            self.t_keep = np.arange(tr["n_buckets"])
+            if self._lists is not None:
+                self.t_lists = self._lists
             return
         nb = tr["n_buckets"]
         owner = np.repeat(np.arange(nb), tr["bucket_size"])
@@ -133,14 +152,16 @@ class Evaluation:
         self.t_sizes = np.ascontiguousarray(counts[keep].astype(np.int32))
         self.t_starts = np.ascontiguousarray(
             np.concatenate([[0], np.cumsum(self.t_sizes)[:-1]]).astype(np.int32))
-        self.t_lists = slice_lists(self.lists, keep)
+        self.t_keep = keep
+        if self._lists is not None:
+            self.t_lists = slice_lists(self._lists, keep)
 
     # ------------------------------------------------------------------ device
     def _t(self, a, dtype=None):
         return torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
 
     def _upload(self):
-        tr, L = self.tree, self.t_lists
+        tr = self.tree
         self.d_parts = self._t(compact_parts(self.pos, self.mass, self.soft))
         self.d_pos64 = self._t(self.pos)
         if self.t_index is None:
@@ -161,12 +182,25 @@ class Evaluation:
         self.d_moments = torch.zeros((nn, 27), dtype=torch.float32, device=self.device)
         self.d_node64 = torch.zeros((nn, 34), dtype=torch.float64, device=self.device)
         self.d_cm64 = torch.zeros((nn, 3), dtype=torch.float64, device=self.device)
+        self.d_starts = self._t(self.t_starts)
+        self.d_sizes = self._t(self.t_sizes)
+        self.stream = torch.cuda.current_stream(self.device)
+        if self.list_source == "gpu":
+            self.build_moments()
+            self._gpu_lists()
+        else:
+            self._upload_host_lists()
+            self.build_moments()
+        torch.cuda.synchronize(self.device)
+        if self.w["periodic"]:
+            self._setup_ewald()
+
+    def _upload_host_lists(self):
+        L = self.t_lists
         if L["cell_marks"][-1] >= 2 ** 31:
             raise ValueError("cell list exceeds the int32 marker range of one request")
         self.d_cells = self._t(L["cells"])
         self.d_cmarks = self._t(L["cell_marks"].astype(np.int32))
-        self.d_starts = self._t(self.t_starts)
-        self.d_sizes = self._t(self.t_sizes)
         if self.pp_compact:
             self.d_plist = self._t(L["parts"])
             self.d_pmarks = self._t(L["part_marks"].astype(np.int32))
@@ -176,11 +210,85 @@ class Evaluation:
                 raise ValueError("expanded particle list exceeds the int32 marker range")
             self.d_plist = self._t(cells)
             self.d_pmarks = self._t(marks.astype(np.int32))
-        self.stream = torch.cuda.current_stream(self.device)
-        self.build_moments()
-        torch.cuda.synchronize(self.device)
-        if self.w["periodic"]:
-            self._setup_ewald()
+
+    def build_lists(self):
+        """Rebuild the GPU-built lists from the current node records (after
+        build_moments, e.g. for new positions on the same tree topology)."""
+        if self.list_source != "gpu":
+            raise ValueError("the lists of this evaluation come from the host walk")
+        self._lists = None
+        self._gpu_lists()
+
+    def _gpu_lists(self):
+        """The target buckets' lists built on the GPU from the node records
+        of the moment build (changaBuildListsCount / Fill); the interaction
+        counts come from its per-bucket sizes."""
+        tr, w = self.tree, self.w
+        keep = self.t_keep
+        nb = len(keep)
+        i32 = dict(dtype=torch.int32, device=self.device)
+        self.d_bucket_node = self._t(tr["bucket_node"][keep].astype(np.int32))
+        self.d_cmarks = torch.empty(nb + 1, **i32)
+        self.d_pmarks = torch.empty(nb + 1, **i32)
+        self.d_soft_start = torch.empty(max(nb, 1), **i32)
+        d_src = torch.empty(max(nb, 1), **i32)
+        nbytes = gpu.lib().changaBuildListsWorkBytes(nb)
+        if nbytes == 0:
+            raise RuntimeError("changaBuildListsWorkBytes failed")
+        work = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        b = gpu.ChangaListBuild()
+        b.nBuckets = nb
+        b.bucketNode = self.d_bucket_node.data_ptr()
+        b.first = self.d_first.data_ptr()
+        b.count = self.d_count.data_ptr()
+        b.child = self.d_child.data_ptr()
+        b.node64 = self.d_node64.data_ptr()
+        b.theta = float(w["theta"])
+        b.nReplicas = int(w["n_replicas"]) if w["periodic"] else 0
+        b.period = self.L if w["periodic"] else 0.0
+        b.cellMarks = self.d_cmarks.data_ptr()
+        b.partMarks = self.d_pmarks.data_ptr()
+        b.softStart = self.d_soft_start.data_ptr()
+        b.sourceCounts = d_src.data_ptr()
+        b.work = work.data_ptr()
+        b.workBytes = work.numel()
+        nc, npr = ctypes.c_longlong(0), ctypes.c_longlong(0)
+        gpu.check(gpu.lib().changaBuildListsCount(ctypes.byref(b), ctypes.byref(nc),
+                                                  ctypes.byref(npr), self._sp()),
+                  "changaBuildListsCount")
+        self.d_cells = torch.empty((max(nc.value, 1), 2), **i32)
+        self.d_plist = torch.empty((max(npr.value, 1), 3), **i32)
+        b.cells = self.d_cells.data_ptr()
+        b.parts = self.d_plist.data_ptr()
+        gpu.check(gpu.lib().changaBuildListsFill(ctypes.byref(b), self._sp()),
+                  "changaBuildListsFill")
+        cm = self.d_cmarks.cpu().numpy().astype(np.int64)
+        pm = self.d_pmarks.cpu().numpy().astype(np.int64)
+        ss = self.d_soft_start.cpu().numpy()[:nb].astype(np.int64)
+        src = d_src.cpu().numpy()[:nb].astype(np.int64)
+        self.t_lists = dict(cell_marks=cm, part_marks=pm, soft_start=ss)
+        act = self.t_sizes.astype(np.int64)
+        nhex, nsoft = ss - cm[:-1], cm[1:] - ss
+        self.counts = (int((nhex * act).sum()), int((nsoft * act).sum()), int((src * act).sum()))
+
+    def _download_lists(self):
+        """GPU-built lists as host arrays over every bucket of the tree."""
+        tr = self.tree
+        nbt, keep = tr["n_buckets"], self.t_keep
+        L = self.t_lists
+        cells = self.d_cells.cpu().numpy().reshape(-1)[:2 * int(L["cell_marks"][-1])]
+        parts = self.d_plist.cpu().numpy().reshape(-1)[:3 * int(L["part_marks"][-1])]
+        out = dict(cells=np.ascontiguousarray(cells), parts=np.ascontiguousarray(parts))
+        for key in ("cell_marks", "part_marks"):
+            lens = np.zeros(nbt, dtype=np.int64)
+            lens[keep] = np.diff(L[key])
+            out[key] = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        ss = out["cell_marks"][:-1].copy()
+        ss[keep] += L["soft_start"] - L["cell_marks"][:-1]
+        out["soft_start"] = ss
+        p = parts.reshape(-1, 3)
+        out["n_part_sources"] = int(p[:, 2].astype(np.int64).sum())
+        return out
 
     def _sp(self):
         return ctypes.c_void_p(self.stream.cuda_stream)

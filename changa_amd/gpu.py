@@ -86,6 +86,15 @@ class ChangaAdjointLaunch(ctypes.Structure):
                 ("nTargets", c_int), ("ewaldWork", c_void_p), ("ewaldWorkDoubles", c_size_t)]
 
 
+class ChangaListBuild(ctypes.Structure):
+    _fields_ = [("nBuckets", c_int), ("bucketNode", c_void_p), ("first", c_void_p),
+                ("count", c_void_p), ("child", c_void_p), ("node64", c_void_p),
+                ("theta", ctypes.c_double), ("nReplicas", c_int), ("period", ctypes.c_double),
+                ("cellMarks", c_void_p), ("partMarks", c_void_p), ("softStart", c_void_p),
+                ("sourceCounts", c_void_p), ("cells", c_void_p), ("parts", c_void_p),
+                ("work", c_void_p), ("workBytes", c_size_t)]
+
+
 CALLBACK = ctypes.CFUNCTYPE(None, c_void_p)
 
 # every extern "C" symbol include/changa_gpu.h declares
@@ -101,6 +110,7 @@ EXPORTS = [
     "changaPartGravity", "changaZeroVars", "changaEwald", "changaBuildMoments",
     "changaBatchPlanCreate", "changaBatchPlanLaunch", "changaBatchPlanRun", "changaBatchPlanDestroy",
     "changaGravityAdjoint", "changaGravityAdjointWorkInts", "changaGravityAdjointEwaldWorkDoubles",
+    "changaBuildListsWorkBytes", "changaBuildListsCount", "changaBuildListsFill",
 ]
 
 _lib = None
@@ -162,6 +172,10 @@ def lib():
     L.changaGravityAdjointWorkInts.restype = c_size_t
     L.changaGravityAdjointEwaldWorkDoubles.argtypes = [c_int, c_int]
     L.changaGravityAdjointEwaldWorkDoubles.restype = c_size_t
+    L.changaBuildListsWorkBytes.argtypes = [c_int]
+    L.changaBuildListsWorkBytes.restype = c_size_t
+    L.changaBuildListsCount.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+    L.changaBuildListsFill.argtypes = [c_void_p, c_void_p]
     _lib = L
     return L
 

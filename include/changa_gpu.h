@@ -249,6 +249,41 @@ size_t changaGravityAdjointWorkInts(int nNodes, int numCellEntries, int numPartE
 size_t changaGravityAdjointEwaldWorkDoubles(int nTargets, int nH);
 int changaGravityAdjoint(const ChangaAdjointLaunch *a, void *stream);
 
+/* ======================================================================
+ * Part 5: interaction-list construction on the GPU
+ *
+ * The caller side of the cell and particle lists (ListCompute::doWork /
+ * stateReady, Compute.cpp:690-884, 1543-1922, with openCriterionBucket,
+ * gravity.h:597-635, and openSoftening, gravity.h:251-260) as a GPU pass
+ * over the node records changaBuildMoments writes: for each target bucket,
+ * the ILCell list (hexadecapole cells, then softened cells flagged in bit
+ * 31 of offsetID) and the ILPart list, in the layout the launches above
+ * take. Two calls: Count (per-bucket sizes into cellMarks / partMarks /
+ * softStart, synchronizes the stream and returns the totals), then, once
+ * the caller has allocated cells and parts, Fill.
+ * ====================================================================== */
+typedef struct {
+  int nBuckets;                   /* target buckets */
+  const int *bucketNode;          /* device: node of each target bucket */
+  const int *first, *count, *child;  /* device: topology, as for changaBuildMoments */
+  const double *node64;           /* device: node records of changaBuildMoments */
+  double theta;                   /* opening angle */
+  int nReplicas;                  /* periodic replicas per side (0: isolated; at most 3) */
+  double period;                  /* box size when periodic */
+  int *cellMarks, *partMarks;     /* device out: nBuckets + 1 list offsets each */
+  int *softStart;                 /* device out: first softened cell of each bucket */
+  int *sourceCounts;              /* device out, optional: source particles of each bucket */
+  ILCell *cells;                  /* device out (Fill): the cell list */
+  ILPart *parts;                  /* device out (Fill): the particle runs */
+  void *work;                     /* device scratch: changaBuildListsWorkBytes(nBuckets) */
+  size_t workBytes;
+} ChangaListBuild;
+
+size_t changaBuildListsWorkBytes(int nBuckets);
+int changaBuildListsCount(const ChangaListBuild *b, long long *numCells, long long *numParts,
+                          void *stream);
+int changaBuildListsFill(const ChangaListBuild *b, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
