@@ -10,6 +10,9 @@ reference tree; the outputs are data, the reference stays where it is).
                       big-endian: 36,000 dark particles: mass, pos, vel, eps)
   pkdtest_t0.json     the t = 0 energies of teststep/pkdtest.log (PKDGRAV,
                       theta = 0.7, hexadecapole, nBucket 8)
+  king_soft_head.bin  the first 64 particles of teststep/king_soft.bin as a
+                      tipsy file (header counts set to 64, records byte for
+                      byte)
 
 Usage: python tests/golden/make_golden.py [/root/reference]
 """
@@ -106,8 +109,21 @@ def king_soft(ref):
         break
 
 
+def king_soft_head(ref, n=64):
+    """The first n dark particles of teststep/king_soft.bin as a tipsy file
+    of their own: the file's big-endian header with nbodies = ndark = n, then
+    the n records byte for byte (a reader fixture)."""
+    d = open(os.path.join(ref, "teststep", "king_soft.bin"), "rb").read()
+    head = bytearray(d[:32])
+    head[8:12] = np.array([n], dtype=">i4").tobytes()
+    head[20:24] = np.array([n], dtype=">i4").tobytes()
+    with open(os.path.join(HERE, "king_soft_head.bin"), "wb") as f:
+        f.write(bytes(head) + d[32:32 + 36 * n])
+
+
 if __name__ == "__main__":
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     moments_vectors()
     king_soft(ref)
+    king_soft_head(ref)
     print("fixtures written to", HERE)

@@ -73,3 +73,29 @@ def test_gpu_lists_forces(device):
     print(f"gpu lists periodic acc {summary(ea)} pot {summary(ep)}", flush=True)
     for e in (ea, ep):
         assert np.median(e) <= 5e-6 and np.percentile(e, 99) <= 5e-5, summary(e)
+
+
+def test_tipsy_evaluate(device, tmp_path):
+    """A tipsy file through changa_amd.tipsy.evaluate (GPU lists, precision
+    form): the .acc2 / .pot files hold this build's forces in input order."""
+    from changa_amd import tipsy
+    w = configs.plummer(n=4096)
+    path = tmp_path / "plummer.bin"
+    tipsy.write(path, 0.0, dark={"mass": w["mass"], "pos": w["pos"], "eps": w["soft"]})
+    acc, pot = tipsy.evaluate(path, str(tmp_path / "out"), theta=w["theta"], bucket=w["bucket"],
+                              device=device)
+    # "%.14g" keeps 14 significant digits
+    np.testing.assert_allclose(tipsy.read_array(str(tmp_path / "out.pot")), pot, rtol=1e-13)
+    np.testing.assert_allclose(tipsy.read_array(str(tmp_path / "out.acc2"), vector=True), acc,
+                               rtol=1e-13, atol=1e-13 * np.abs(acc).max())
+    # the same particles (FP32 as the file holds them) through the engine
+    import torch
+    wf = dict(w, pos=w["pos"].astype(np.float32).astype(np.float64),
+              mass=w["mass"].astype(np.float32).astype(np.float64),
+              soft=w["soft"].astype(np.float32).astype(np.float64))
+    ev = engine.Evaluation(wf, precise=True, device=device, lists="gpu")
+    ev.step()
+    torch.cuda.synchronize()
+    a_t, p_t, _ = ev.results()
+    np.testing.assert_array_equal(acc[ev.tree["perm"]], a_t)
+    np.testing.assert_array_equal(pot[ev.tree["perm"]], p_t)
