@@ -26,10 +26,12 @@ pass); --grad-accuracy adds the gradient error against the FP64 oracle's
 adjoint on the same lists (and, for config 4, against the FP64 direct-sum
 gradient of a 4,096-particle Plummer sphere).
 
-The interaction lists are built once, before the timed steps, on the GPU from
-the moment build's node records (--lists gpu, the default; changaBuildLists*)
-or by the host walk (--lists host); both give the same lists, and
-config.list_build_ms reports the GPU build's time.
+The tree and the interaction lists are built once, before the timed steps,
+on the GPU (--tree gpu --lists gpu, the defaults; changaBuildTree,
+changaBuildLists*) or on the host (--tree host, --lists host); both give the
+same tree and lists. config.tree_build_ms (including the positions' upload
+and the download of the topology) and config.list_build_ms report the GPU
+builds' times.
 
 Prints one JSON line. --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -73,6 +75,8 @@ def parse():
     ap.add_argument("--samples", type=int, default=256)
     ap.add_argument("--lists", choices=["gpu", "host"], default="gpu",
                     help="interaction lists built on the GPU (default) or by the host walk")
+    ap.add_argument("--tree", choices=["gpu", "host"], default="gpu",
+                    help="tree built on the GPU (default) or on the host")
     ap.add_argument("--adjoint", action="store_true",
                     help="time the adjoint (default for configs 4 and 5)")
     ap.add_argument("--kdk", action="store_true",
@@ -121,8 +125,15 @@ def main():
                     "hexadecapole")
         data = "synthetic (seeded halos, numpy PCG64 seed 1003)"
         tag = f"halos{args.n}"
-    ev = engine.Evaluation(w, precise=(args.form == "precise"), lists=args.lists)
+    ev = engine.Evaluation(w, precise=(args.form == "precise"), lists=args.lists, tree=args.tree)
     setup_s = time.time() - t0
+    tree_ms = None
+    if args.tree == "gpu":
+        kw = dict(box_lo=np.zeros(3), box_size=w["L"]) if w["periodic"] else {}
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        engine.gpu_build_tree(w["pos"], w["bucket"], **kw)
+        tree_ms = (time.perf_counter() - t1) * 1e3
     list_ms = None
     if args.lists == "gpu":
         ev.build_lists()
@@ -206,6 +217,8 @@ def main():
             "host_setup_s": round(setup_s, 2),
             "lists": args.lists,
             "list_build_ms": None if list_ms is None else round(list_ms, 2),
+            "tree": args.tree,
+            "tree_build_ms": None if tree_ms is None else round(tree_ms, 1),
         },
         "step_ms": {k: round(v, 4) for k, v in step_ms.items()},
         "batched_options": {"pcTile": opts.pcTile, "ppTile": opts.ppTile,
@@ -327,7 +340,7 @@ def kdk_step(w, dt=0.01, return_grad=False):
     def build(x):
         nonlocal host_s
         t = time.time()
-        ev = engine.Evaluation(dict(w, pos=x), precise=True, lists="gpu")
+        ev = engine.Evaluation(dict(w, pos=x), precise=True, lists="gpu", tree="gpu")
         torch.cuda.synchronize()
         host_s += time.time() - t
         log(f"kdk: tree, lists and moments in {time.time() - t:.1f} s")

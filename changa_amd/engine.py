@@ -9,6 +9,7 @@ VariablePartData array (libchanga_gpu.so). PyTorch only provides device memory
 and the stream.
 """
 import ctypes
+import functools
 import math
 
 import numpy as np
@@ -78,6 +79,58 @@ def slice_lists(lists, buckets):
     return out
 
 
+def gpu_build_tree(pos, bucket_size, box_lo=None, box_size=None, device="cuda:0"):
+    """The tree host.build_tree makes, built on the GPU (changaBuildTree);
+    returns the same dict of host arrays."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n = pos.shape[0]
+    if box_lo is None:
+        lo, hi = pos.min(axis=0), pos.max(axis=0)
+        box_size = float((hi - lo).max()) * (1 + 1e-12) + 1e-300
+        box_lo = lo
+    dev = torch.device(device)
+    i32 = dict(dtype=torch.int32, device=dev)
+    d_pos = torch.from_numpy(pos).to(dev)
+    nn_cap = max(2 * n - 1, 1)
+    d_perm = torch.empty(n, **i32)
+    d_first, d_count = torch.empty(nn_cap, **i32), torch.empty(nn_cap, **i32)
+    d_child = torch.empty(2 * nn_cap, **i32)
+    d_boxr = torch.empty(nn_cap, dtype=torch.float64, device=dev)
+    d_bnode = torch.empty(n, **i32)
+    nbytes = gpu.lib().changaBuildTreeWorkBytes(n)
+    if nbytes == 0:
+        raise RuntimeError("changaBuildTreeWorkBytes failed")
+    work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    levels = np.zeros(130, dtype=np.int32)
+    t = gpu.ChangaTreeBuild()
+    t.n = n
+    t.pos = d_pos.data_ptr()
+    for k in range(3):
+        t.boxLo[k] = float(box_lo[k])
+    t.boxSize = float(box_size)
+    t.bucketSize = int(bucket_size)
+    t.perm, t.first, t.count = d_perm.data_ptr(), d_first.data_ptr(), d_count.data_ptr()
+    t.child, t.boxRadius, t.bucketNode = d_child.data_ptr(), d_boxr.data_ptr(), d_bnode.data_ptr()
+    t.maxNodes = nn_cap
+    t.levelOffsets = levels.ctypes.data
+    t.maxDepths = len(levels)
+    t.work, t.workBytes = work.data_ptr(), work.numel()
+    s = torch.cuda.current_stream(dev)
+    gpu.check(gpu.lib().changaBuildTree(ctypes.byref(t), ctypes.c_void_p(s.cuda_stream)),
+              "changaBuildTree")
+    nn, nb, nd = t.nNodes, t.nBuckets, t.nDepths
+    lo_off = levels[:nd + 1].copy()
+    depth = np.repeat(np.arange(nd, dtype=np.int32), np.diff(lo_off))
+    tr = dict(n=n, n_nodes=nn, n_buckets=nb, n_depths=nd,
+              perm=d_perm.cpu().numpy(), node_first=d_first[:nn].cpu().numpy(),
+              node_count=d_count[:nn].cpu().numpy(), node_child=d_child[:2 * nn].cpu().numpy(),
+              node_depth=depth, node_box_radius=d_boxr[:nn].cpu().numpy(),
+              level_offsets=lo_off, bucket_node=d_bnode[:nb].cpu().numpy())
+    tr["bucket_start"] = np.ascontiguousarray(tr["node_first"][tr["bucket_node"]])
+    tr["bucket_size"] = np.ascontiguousarray(tr["node_count"][tr["bucket_node"]])
+    return tr
+
+
 class Evaluation:
     """Tree, lists and device state for one workload (see configs.py).
 
@@ -91,9 +144,9 @@ class Evaluation:
     always every particle."""
 
     def __init__(self, w, precise=False, pp_compact=True, n_threads=0, device="cuda:0",
-                 lists="host"):
-        if lists not in ("host", "gpu"):
-            raise ValueError("lists: 'host' or 'gpu'")
+                 lists="host", tree="host"):
+        if lists not in ("host", "gpu") or tree not in ("host", "gpu"):
+            raise ValueError("lists and tree: 'host' or 'gpu'")
         if lists == "gpu" and not pp_compact:
             raise ValueError("GPU-built lists are ILPart runs (pp_compact=True)")
         self.w = w
@@ -104,10 +157,12 @@ class Evaluation:
         self.device = torch.device(device)
         periodic = bool(w["periodic"])
         self.L = float(w["L"]) if periodic else 0.0
+        build = host.build_tree if tree == "host" else functools.partial(gpu_build_tree,
+                                                                         device=device)
         if periodic:
-            tr = host.build_tree(w["pos"], w["bucket"], box_lo=np.zeros(3), box_size=self.L)
+            tr = build(w["pos"], w["bucket"], box_lo=np.zeros(3), box_size=self.L)
         else:
-            tr = host.build_tree(w["pos"], w["bucket"])
+            tr = build(w["pos"], w["bucket"])
         self.tree = tr
         perm = tr["perm"]
         self.pos = np.ascontiguousarray(w["pos"][perm])

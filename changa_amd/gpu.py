@@ -95,6 +95,16 @@ class ChangaListBuild(ctypes.Structure):
                 ("work", c_void_p), ("workBytes", c_size_t)]
 
 
+class ChangaTreeBuild(ctypes.Structure):
+    _fields_ = [("n", c_int), ("pos", c_void_p), ("boxLo", ctypes.c_double * 3),
+                ("boxSize", ctypes.c_double), ("bucketSize", c_int), ("perm", c_void_p),
+                ("first", c_void_p), ("count", c_void_p), ("child", c_void_p),
+                ("boxRadius", c_void_p), ("bucketNode", c_void_p), ("maxNodes", c_int),
+                ("levelOffsets", c_void_p), ("maxDepths", c_int), ("work", c_void_p),
+                ("workBytes", c_size_t), ("nNodes", c_int), ("nBuckets", c_int),
+                ("nDepths", c_int)]
+
+
 CALLBACK = ctypes.CFUNCTYPE(None, c_void_p)
 
 # every extern "C" symbol include/changa_gpu.h declares
@@ -111,6 +121,7 @@ EXPORTS = [
     "changaBatchPlanCreate", "changaBatchPlanLaunch", "changaBatchPlanRun", "changaBatchPlanDestroy",
     "changaGravityAdjoint", "changaGravityAdjointWorkInts", "changaGravityAdjointEwaldWorkDoubles",
     "changaBuildListsWorkBytes", "changaBuildListsCount", "changaBuildListsFill",
+    "changaBuildTreeWorkBytes", "changaBuildTree",
 ]
 
 _lib = None
@@ -176,6 +187,9 @@ def lib():
     L.changaBuildListsWorkBytes.restype = c_size_t
     L.changaBuildListsCount.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
     L.changaBuildListsFill.argtypes = [c_void_p, c_void_p]
+    L.changaBuildTreeWorkBytes.argtypes = [c_int]
+    L.changaBuildTreeWorkBytes.restype = c_size_t
+    L.changaBuildTree.argtypes = [c_void_p, c_void_p]
     _lib = L
     return L
 

@@ -284,6 +284,40 @@ int changaBuildListsCount(const ChangaListBuild *b, long long *numCells, long lo
                           void *stream);
 int changaBuildListsFill(const ChangaListBuild *b, void *stream);
 
+/* ======================================================================
+ * Part 6: tree build on the GPU
+ *
+ * The caller side's tree (TreePiece::buildTree, TreePiece.cpp:2619, and
+ * LocalTreeBuilder, Compute.cpp:2440-2580): particles in Morton order (21
+ * bits per axis, top bit x, then y, then z: the axis cycle of
+ * makeOctChildren, GenericTreeNode.h:473-575), binary nodes split on the
+ * highest key bit on which their particles differ until they hold at most
+ * bucketSize particles (Compute.cpp:2476), numbered breadth first; buckets
+ * in particle order. The outputs are the topology changaBuildMoments,
+ * changaBuildLists* and changaGravityAdjoint take. Synchronizes the stream
+ * (one count per depth is read back).
+ * ====================================================================== */
+typedef struct {
+  int n;                          /* particles */
+  const double *pos;              /* device: FP64 x, y, z per particle, input order */
+  double boxLo[3], boxSize;       /* the cube the Morton keys cover */
+  int bucketSize;                 /* nBucket */
+  int *perm;                      /* device out, n: input index of each tree-order particle */
+  int *first, *count, *child;     /* device out: node arrays (child: 2 per node, -1 for buckets) */
+  double *boxRadius;              /* device out: half diagonal of each node's Morton cell */
+  int *bucketNode;                /* device out: bucket nodes in particle order (n entries) */
+  int maxNodes;                   /* capacity of the node arrays: at least 2n - 1 */
+  int *levelOffsets;              /* host out: depth d holds nodes [levelOffsets[d],
+                                     levelOffsets[d + 1]) */
+  int maxDepths;                  /* capacity of levelOffsets */
+  void *work;                     /* device scratch: changaBuildTreeWorkBytes(n) bytes */
+  size_t workBytes;
+  int nNodes, nBuckets, nDepths;  /* out */
+} ChangaTreeBuild;
+
+size_t changaBuildTreeWorkBytes(int n);
+int changaBuildTree(ChangaTreeBuild *t, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
