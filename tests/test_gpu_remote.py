@@ -1,0 +1,221 @@
+"""The remote-chunk request variants of the HostCUDA mirror, exercised the way
+they exist for: two processes (TreePieces) share the GPU, each owning half
+of the particles and the nodes inside its half (the nodes spanning both
+halves belong to process 0). Each process uploads its local tree
+(DataManagerTransferLocalTree) and, received from the other process over
+gloo, the other's nodes and particles as a remote chunk
+(DataManagerTransferRemoteChunk). Its buckets' lists are split by ownership
+into TreePieceCellListDataTransfer{Local,Remote,RemoteResume} and
+TreePiecePartListDataTransfer{Local,LocalSmallPhase,Remote,RemoteResume}
+requests (the Resume requests carry their own missed nodes / particles,
+HostCUDA.cu:296-341, 517-572). The gathered accelerations must equal the
+single-process device path up to the FP32 summation order of the split."""
+import ctypes
+import os
+import socket
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 8192
+
+
+def _free_port():
+    with socket.socket() as s:
+        s.bind(("127.0.0.1", 0))
+        return s.getsockname()[1]
+
+
+def _split(ev):
+    """Particle halves (at a bucket boundary) and node ownership."""
+    tr = ev.tree
+    nb = tr["n_buckets"]
+    cut = int(tr["bucket_start"][nb // 2])
+    first, count = tr["node_first"], tr["node_count"]
+    last = first + count
+    owner = np.where(last <= cut, 0, np.where(first >= cut, 1, 0))  # spanning nodes: 0
+    return cut, owner
+
+
+def _worker(rank, port, out_path):
+    os.environ["MASTER_ADDR"] = "127.0.0.1"
+    os.environ["MASTER_PORT"] = str(port)
+    import torch
+    import torch.distributed as dist
+
+    from changa_amd import configs, engine, gpu
+
+    dist.init_process_group("gloo", rank=rank, world_size=2)
+    ev = engine.Evaluation(configs.plummer(n=N, eps=0.05), precise=False, device="cuda:0")
+    torch.cuda.synchronize()
+    tr = ev.tree
+    cut, owner = _split(ev)
+    lo, hi = (0, cut) if rank == 0 else (cut, N)
+    moments = ev.d_moments.cpu().numpy()
+    parts = ev.d_parts.cpu().numpy()
+
+    # local data, and the other process's local data received over gloo
+    mine = np.nonzero(owner == rank)[0]
+    local_mom = np.ascontiguousarray(moments[mine])
+    local_parts = np.ascontiguousarray(parts[lo:hi])
+    sizes = [torch.zeros(2, dtype=torch.int64) for _ in range(2)]
+    dist.all_gather(sizes, torch.tensor([len(mine), hi - lo], dtype=torch.int64))
+    maxm = int(max(s[0] for s in sizes))
+    maxp = int(max(s[1] for s in sizes))
+    mb = torch.zeros((maxm, 27), dtype=torch.float32)
+    mb[:len(mine)] = torch.from_numpy(local_mom)
+    pb = torch.zeros((maxp, 5), dtype=torch.float32)
+    pb[:hi - lo] = torch.from_numpy(local_parts)
+    gm = [torch.zeros_like(mb) for _ in range(2)]
+    gp = [torch.zeros_like(pb) for _ in range(2)]
+    dist.all_gather(gm, mb)
+    dist.all_gather(gp, pb)
+    other = 1 - rank
+    remote_mom = np.ascontiguousarray(gm[other][:int(sizes[other][0])].numpy())
+    remote_parts = np.ascontiguousarray(gp[other][:int(sizes[other][1])].numpy())
+    theirs = np.nonzero(owner == other)[0]
+    o_lo = cut if rank == 0 else 0
+    local_idx = np.full(tr["n_nodes"], -1, dtype=np.int64)
+    local_idx[mine] = np.arange(len(mine))
+    remote_idx = np.full(tr["n_nodes"], -1, dtype=np.int64)
+    remote_idx[theirs] = np.arange(len(theirs))
+
+    # per-bucket lists of this process's buckets, split by ownership
+    L = ev.lists
+    pcells, pmarks = engine.expand_parts(L)
+    buckets = np.nonzero((tr["bucket_start"] >= lo) & (tr["bucket_start"] < hi))[0]
+    half = len(buckets) // 2
+    empty = np.zeros((0, 2), dtype=np.int64)
+    # per request kind, one list per bucket (empty where the request does not
+    # carry that bucket): cells local / remote (first half) / remote resume
+    # (second half); particles local (first half) / local small phase
+    # (second half) / remote (first half) / remote resume (second half)
+    reqs = {k: [] for k in ("cl", "cr", "crr", "pl", "pls", "pr", "prr")}
+    for j, b in enumerate(buckets):
+        c = L["cells"].reshape(-1, 2)[L["cell_marks"][b]:L["cell_marks"][b + 1]].astype(np.int64)
+        is_local = owner[c[:, 0]] == rank
+        cl = c[is_local].copy()
+        cl[:, 0] = local_idx[cl[:, 0]]
+        cr = c[~is_local].copy()
+        cr[:, 0] = remote_idx[cr[:, 0]]
+        p = pcells.reshape(-1, 2)[pmarks[b]:pmarks[b + 1]].astype(np.int64)
+        pin = (p[:, 0] >= lo) & (p[:, 0] < hi)
+        pl = p[pin].copy()
+        pl[:, 0] -= lo
+        pr = p[~pin].copy()
+        pr[:, 0] -= o_lo
+        first = j < half
+        reqs["cl"].append(cl)
+        reqs["cr"].append(cr if first else empty)
+        reqs["crr"].append(empty if first else cr)
+        reqs["pl"].append(pl if first else empty)
+        reqs["pls"].append(empty if first else pl)
+        reqs["pr"].append(pr if first else empty)
+        reqs["prr"].append(empty if first else pr)
+
+    L_ = gpu.lib()
+    fired = []
+    handler = gpu.CALLBACK(lambda cb: fired.append(cb))
+    L_.changaGpuSetCallbackHandler(handler)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    varp = np.zeros((hi - lo, 5), dtype=np.float32)
+    dm, dp, dv = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    L_.DataManagerTransferLocalTree(local_mom.ctypes.data, local_mom.nbytes,
+                                    local_parts.ctypes.data, local_parts.nbytes,
+                                    varp.ctypes.data, varp.nbytes, ctypes.byref(dm),
+                                    ctypes.byref(dp), ctypes.byref(dv), stream, hi - lo,
+                                    ctypes.c_void_p(11))
+    rm, rp = ctypes.c_void_p(), ctypes.c_void_p()
+    L_.DataManagerTransferRemoteChunk(remote_mom.ctypes.data, remote_mom.nbytes,
+                                      remote_parts.ctypes.data, remote_parts.nbytes,
+                                      ctypes.byref(rm), ctypes.byref(rp), stream,
+                                      ctypes.c_void_p(12))
+    keep = []
+    starts_all = (tr["bucket_start"][buckets] - lo).astype(np.int32)
+    sizes_all = tr["bucket_size"][buckets].astype(np.int32)
+
+    def send(fn, lists, sel, cb, missed=None, small=None):
+        lst = [lists[i] for i in sel]
+        marks = np.concatenate([[0], np.cumsum([len(x) for x in lst])]).astype(np.int32)
+        flat = np.ascontiguousarray(np.concatenate(lst).astype(np.int32).reshape(-1)) \
+            if marks[-1] else np.zeros(2, dtype=np.int32)
+        q = gpu.CudaRequest()
+        q.stream = stream
+        q.d_localMoments, q.d_localParts, q.d_localVars = dm, dp, dv
+        q.d_remoteMoments, q.d_remoteParts = rm, rp
+        q.list = flat.ctypes.data
+        q.bucketMarkers = marks.ctypes.data
+        st = np.ascontiguousarray(starts_all[sel])
+        sz = np.ascontiguousarray(sizes_all[sel])
+        q.bucketStarts, q.bucketSizes = st.ctypes.data, sz.ctypes.data
+        q.numInteractions = int(marks[-1])
+        q.numBucketsPlusOne = len(sel) + 1
+        q.fperiod = 0.0
+        q.cb = cb
+        if missed is not None:
+            q.missedNodes = q.missedParts = missed.ctypes.data
+            q.sMissed = missed.nbytes
+        keep.append((q, flat, marks, st, sz, missed))
+        if small is not None:
+            fn(ctypes.byref(q), small.ctypes.data, len(small))
+        else:
+            fn(ctypes.byref(q))
+
+    first_half, second_half = np.arange(half), np.arange(half, len(buckets))
+
+    def missed(lists, sel, pool):
+        """A Resume request's own copy of the remote entries it uses,
+        indexed from 0, and its lists re-indexed into that copy."""
+        used = np.unique(np.concatenate([lists[i][:, 0] for i in sel] + [np.zeros(0, np.int64)]))
+        remap = np.full(len(pool), -1, dtype=np.int64)
+        remap[used] = np.arange(len(used))
+        out = [np.stack([remap[x[:, 0]], x[:, 1]], axis=1) if len(x) else x for x in lists]
+        return out, np.ascontiguousarray(pool[used])
+
+    send(L_.TreePieceCellListDataTransferLocal, reqs["cl"], np.arange(len(buckets)), 21)
+    send(L_.TreePieceCellListDataTransferRemote, reqs["cr"], first_half, 22)
+    crr, mn = missed(reqs["crr"], second_half, remote_mom)
+    send(L_.TreePieceCellListDataTransferRemoteResume, crr, second_half, 23, missed=mn)
+    send(L_.TreePiecePartListDataTransferLocal, reqs["pl"], first_half, 24)
+    send(L_.TreePiecePartListDataTransferLocalSmallPhase, reqs["pls"], second_half, 25,
+         small=local_parts)
+    send(L_.TreePiecePartListDataTransferRemote, reqs["pr"], first_half, 26)
+    prr, mp = missed(reqs["prr"], second_half, remote_parts)
+    send(L_.TreePiecePartListDataTransferRemoteResume, prr, second_half, 27, missed=mp)
+    out = np.zeros((hi - lo, 5), dtype=np.float32)
+    L_.TransferParticleVarsBack(out.ctypes.data, out.nbytes, dv, stream, ctypes.c_void_p(28))
+    torch.cuda.synchronize()
+    hip = ctypes.CDLL("libamdhip64.so")
+    for ptr in (dm, dp, dv, rm, rp):
+        assert hip.hipFree(ptr) == 0
+    L_.changaGpuSetCallbackHandler(gpu.CALLBACK(0))
+    assert sorted(fired) == [11, 12] + list(range(21, 29)), fired
+
+    # every process's rows in tree order
+    res = torch.zeros((N, 5), dtype=torch.float32)
+    res[lo:hi] = torch.from_numpy(out)
+    dist.all_reduce(res)
+    if rank == 0:
+        ev.zero()
+        ev.run_pc()
+        ev.run_pp()
+        torch.cuda.synchronize()
+        np.savez(out_path, split=res.numpy(), whole=ev.d_vars.cpu().numpy())
+    dist.destroy_process_group()
+
+
+def test_gpu_remote_chunks_two_processes(device, tmp_path):
+    import torch.multiprocessing as mp
+    out = str(tmp_path / "remote.npz")
+    mp.start_processes(_worker, args=(_free_port(), out), nprocs=2, join=True,
+                       start_method="spawn")
+    d = np.load(out)
+    split, whole = d["split"].astype(np.float64), d["whole"].astype(np.float64)
+    acc_err = np.linalg.norm(split[:, :3] - whole[:, :3], axis=1) / np.linalg.norm(whole[:, :3],
+                                                                                  axis=1)
+    pot_err = np.abs(split[:, 3] - whole[:, 3]) / np.abs(whole[:, 3])
+    print(f"remote split: acc max {acc_err.max():.2e} pot max {pot_err.max():.2e}", flush=True)
+    assert acc_err.max() < 1e-5 and pot_err.max() < 1e-5
+    np.testing.assert_array_equal(split[:, 4], whole[:, 4])  # dtGrav: a max of the same terms
