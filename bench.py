@@ -62,7 +62,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--form", choices=["precise", "plain"], default="precise")
-    ap.add_argument("--path", choices=["batched", "per-bucket"], default="batched")
+    ap.add_argument("--path", choices=["batched", "per-bucket", "fused"], default="batched")
     ap.add_argument("--config", type=int, choices=[2, 3, 4, 5], default=2)
     ap.add_argument("--ng", type=int, default=128, help="lattice size of the periodic box")
     ap.add_argument("--n", type=int, default=1 << 22, help="particles of the halo workload")
@@ -165,6 +165,12 @@ def main():
         return start.elapsed_time(stop) / args.steps
 
     step_ms = {"batched": time_steps(ev.step_batched), "per-bucket": time_steps(ev.step)}
+    if ev.precise:
+        # walk and force in one launch: the list build is inside this step
+        step_ms["fused"] = time_steps(ev.step_fused)
+        ev.fused_check()
+    if args.path not in step_ms:
+        raise SystemExit("the fused path runs on the precision form")
     ms = step_ms[args.path]
     value = n_int / (ms * 1e-3) / 1e9
 
@@ -181,12 +187,17 @@ def main():
     kern = {"moments": time_fn(ev.build_moments), "zero": time_fn(ev.zero),
             "pc": time_fn(ev.run_pc), "pp": time_fn(ev.run_pp),
             "batched_graph": time_fn(ev.run_batched)}
+    if ev.precise:
+        kern["fused"] = time_fn(ev.run_fused)
     if w["periodic"]:
         kern["ewald"] = time_fn(ev.run_ewald)
     pc_flop = FLOP["pc"] * counts["pc"] + FLOP["soft"] * counts["soft"]
     if args.path == "batched":
         # the graph holds the zeroing kernel, the queue reset and batch_kernel
         dom, dom_ms = "batch_kernel", kern["batched_graph"] - kern["zero"]
+        dom_flop = pc_flop + FLOP["pp"] * counts["pp"]
+    elif args.path == "fused":
+        dom, dom_ms = "fused_kernel", kern["fused"]
         dom_flop = pc_flop + FLOP["pp"] * counts["pp"]
     else:
         dom, dom_ms, dom_flop = "pc_kernel", kern["pc"], pc_flop
@@ -210,7 +221,8 @@ def main():
             "form": args.form,
             "path": args.path,
             "step": "GPU multipole build + zero + PC + PP" + (" + Ewald" if w["periodic"] else "")
-                    + " on resident lists",
+                    + (" with the tree walk fused in (no lists)" if args.path == "fused"
+                       else " on resident lists"),
             "interactions_per_step": counts,
             "n_buckets": int(ev.tree["n_buckets"]),
             "n_nodes": int(ev.tree["n_nodes"]),
@@ -238,14 +250,15 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved * 1e12 / PEAK_FP32, 4),
             "traffic": traffic_from_profile(
-                tag, f"batch_kernel<{opts.pcTile},{opts.ppTile}>" if dom == "batch_kernel" else dom),
+                tag, {"batch_kernel": f"batch_kernel<{opts.pcTile},{opts.ppTile}>",
+                      "fused_kernel": "fused_kernel<64,256>"}.get(dom, dom)),
         },
     }
 
     log(f"timed: {value:.1f} G interactions/s")
     if not args.no_accuracy:
-        out["accuracy"] = accuracy(ev, ev.step_batched if args.path == "batched" else ev.step,
-                                   args.samples)
+        out["accuracy"] = accuracy(ev, {"batched": ev.step_batched, "fused": ev.step_fused}.get(
+            args.path, ev.step), args.samples)
         log("accuracy leg done")
     if args.adjoint or args.kdk or args.grad_accuracy or args.config in (4, 5):
         out["adjoint"] = adjoint_leg(ev, w, args)

@@ -21,7 +21,7 @@ GPU_LIB = os.path.join(PKG, "libchanga_gpu.so")
 
 HOST_SRC = ["host/host_tree.cpp", "host/host_walk.cpp", "host/host_api.cpp"]
 GPU_SRC = ["gpu/gravity.hip", "gpu/batch.hip", "gpu/adjoint.hip", "gpu/ewald.hip", "gpu/moments.hip",
-           "gpu/walk.hip", "gpu/tree.hip", "gpu/hostcuda.hip"]
+           "gpu/walk.hip", "gpu/tree.hip", "gpu/fused.hip", "gpu/hostcuda.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instruction selection stays off for every HIP object.

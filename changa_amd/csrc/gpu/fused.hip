@@ -1,0 +1,311 @@
+// Fused walk and force: one persistent launch in which each wavefront walks
+// the tree for a target bucket and evaluates what the walk accepts, with no
+// interaction list in HBM (the counterpart of the reference's
+// gpuLocalTreeWalk, HostCUDA.cu:724-942, with the list semantics of
+// ListCompute::doWork, Compute.cpp:690-884).
+//
+// Walk: wavefront-cooperative over an LDS stack of (node, replica) entries
+// packed in 32 bits; each iteration pops up to 64 entries, one per lane, and
+// applies openCriterionBucket (gravity.h:597-635) with the softening re-test
+// (openSoftening, gravity.h:251-260) in FP64 without contraction, exactly as
+// the list builder (walk.hip) does. Accepted cells, softened cells and
+// opened buckets (particle runs) are appended, through wavefront prefix
+// sums, to the wavefront's own scratch lists (L2-resident); opened internal
+// nodes push their two children. The interaction set is the list builder's;
+// only its order differs (the walk is breadth-first in chunks of 64).
+//
+// Force: the bucket's targets then run the cell and particle loops of the
+// list path (gravity_body.h) over the scratch lists, precision form, into
+// FP64 accumulators and one update of the result array per chunk.
+#include "gravity_body.h"
+
+namespace cgpu {
+
+constexpr int kFusedStack = 2304;     // LDS stack entries (pending nodes): 9 KB, the PC tile's LDS
+constexpr int kFusedNodeBits = 22;    // node index bits of a stack entry; replica code above
+constexpr int kFusedPcTile = 64;
+constexpr int kFusedPpTile = 256;
+constexpr int kFusedRanges = kNumXcd;
+constexpr int kFusedCounterStride = 32;
+
+struct FusedArgs {
+  ChangaGravityLaunch pc, pp;  // lists are the wavefronts' scratch (set per wave)
+  const int* bucketNode;
+  const int* first;
+  const int* count;
+  const int* child;
+  const double* node64;
+  double kgeom, theta, thetaMono, period;
+  int R;
+  ILCell* cells;  // per wave: maxCells entries
+  ILPart* runs;   // per wave: maxRuns entries
+  int maxCells, maxRuns;
+  int* counters;  // kFusedRanges claim counters, kFusedCounterStride ints apart; then err
+  int* err;
+};
+
+template <int PCT, int PPT>
+struct FusedLds {
+  static constexpr size_t stack = kFusedStack * sizeof(unsigned);
+  static constexpr size_t pc = (size_t)PCT * kPcRec * sizeof(float);
+  static constexpr size_t pp = sizeof(PpLds<PPT>);
+  static constexpr size_t red = 4 * kWave * sizeof(double) + kWave * sizeof(float);
+  static constexpr size_t m1 = stack > pc ? stack : pc;
+  static constexpr size_t m2 = pp > red ? pp : red;
+  static constexpr size_t bytes = m1 > m2 ? m1 : m2;
+  static constexpr size_t f4 = (bytes + 15) / 16;
+};
+
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ bool fused_box_sphere(const double* lo, const double* hi,
+                                                 const double c[3], double r) {
+  double dsq = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double e = 0.0;
+    if (c[k] < lo[k])
+      e = lo[k] - c[k];
+    else if (c[k] > hi[k])
+      e = c[k] - hi[k];
+    dsq += e * e;
+  }
+  return dsq <= r * r;
+}
+
+// 0 open, 1 hexadecapole cell, 2 softened cell (the list builder's test)
+__device__ __forceinline__ int fused_decide(const FusedArgs& a, const double* bg, double rb,
+                                            int nd, int code) {
+  if (a.count[nd] <= 6) return 0;
+  const double* g = a.node64 + (size_t)nd * 34;
+  const int x = (code & 7) - 3, y = ((code >> 3) & 7) - 3, z = ((code >> 6) & 7) - 3;
+  const double c3[3] = {g[0] + x * a.period, g[1] + y * a.period, g[2] + z * a.period};
+  const double r = g[3];
+  double rad = a.kgeom * r / a.theta;
+  if (rad < r) rad = r;
+  if (fused_box_sphere(bg + 6, bg + 9, c3, rad)) return 0;
+  const double rs = 2.0 * g[4];
+  const double dx = c3[0] - bg[0], dy = c3[1] - bg[1], dz = c3[2] - bg[2];
+  const bool soft_overlap = dx * dx + dy * dy + dz * dz <= (rs + rb) * (rs + rb) ||
+                            fused_box_sphere(bg + 6, bg + 9, c3, rs);
+  if (!soft_overlap) return 1;
+  const double rm = a.kgeom * r / a.thetaMono;
+  return fused_box_sphere(bg + 6, bg + 9, c3, rm) ? 0 : 2;
+}
+
+#pragma clang fp contract(on)
+
+__device__ __forceinline__ int excl_count(unsigned long long mask, int lane) {
+  return __popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Walks bucket b into the wave's scratch: hexadecapole cells at [0, nhex),
+// softened cells at [maxCells - nsoft, maxCells), runs at [0, nruns).
+// Returns false (and raises err) when a capacity is exceeded.
+__device__ bool fused_walk(const FusedArgs& a, int b, int lane, unsigned* stack, ILCell* cells,
+                           ILPart* runs, int& nhex, int& nsoft, int& nruns) {
+  double bg[12];
+  {
+    const double* g = a.node64 + (size_t)a.bucketNode[b] * 34;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) bg[k] = g[k];
+  }
+  const double rb = 2.0 * bg[4];
+  nhex = nsoft = nruns = 0;
+  // the root of every replica (the interaction set does not depend on the order)
+  const int W = 2 * a.R + 1, nrep = W * W * W;
+  for (int k = lane; k < nrep; k += kWave) {
+    const int x = k / (W * W) - a.R, y = (k / W) % W - a.R, z = k % W - a.R;
+    const unsigned code = (unsigned)((x + 3) | ((y + 3) << 3) | ((z + 3) << 6));
+    stack[k] = code << kFusedNodeBits;  // node 0
+  }
+  int sp = nrep;
+  bool ok = true;
+  __syncthreads();
+  while (sp > 0) {
+    // pop up to 64 entries, fewer when the stack is nearly full: each popped
+    // entry pushes at most two, so popping at most (capacity - sp) keeps the
+    // stack within capacity; near the limit the walk turns depth first
+    const int n = min(min(kWave, sp), max(1, kFusedStack - sp));
+    unsigned e = 0;
+    if (lane < n) e = stack[sp - n + lane];
+    __syncthreads();
+    sp -= n;
+    int kind = -1, nd = 0, code = 0;  // -1 none, 0 open, 1 hex, 2 soft
+    if (lane < n) {
+      nd = (int)(e & ((1u << kFusedNodeBits) - 1u));
+      code = (int)(e >> kFusedNodeBits);
+      kind = fused_decide(a, bg, rb, nd, code);
+    }
+    const int oid = code << 22;
+    const bool is_bucket = kind == 0 && a.child[2 * nd] < 0;
+    const bool is_push = kind == 0 && !is_bucket;
+    const unsigned long long mh = __ballot(kind == 1), ms = __ballot(kind == 2),
+                             mr = __ballot(is_bucket), mp = __ballot(is_push);
+    const int ch = __popcll(mh), cs = __popcll(ms), cr = __popcll(mr), cp = __popcll(mp);
+    if (nhex + nsoft + ch + cs > a.maxCells || nruns + cr > a.maxRuns ||
+        sp + 2 * cp > kFusedStack) {
+      ok = false;
+      break;
+    }
+    if (kind == 1) cells[nhex + excl_count(mh, lane)] = {nd, oid | b};
+    if (kind == 2)
+      cells[a.maxCells - 1 - (nsoft + excl_count(ms, lane))] = {nd, oid | b | (int)0x80000000u};
+    if (is_bucket) runs[nruns + excl_count(mr, lane)] = {a.first[nd], oid, a.count[nd]};
+    if (is_push) {
+      const int at = sp + 2 * excl_count(mp, lane);
+      const unsigned hi = (unsigned)code << kFusedNodeBits;
+      stack[at] = hi | (unsigned)a.child[2 * nd + 1];
+      stack[at + 1] = hi | (unsigned)a.child[2 * nd];
+    }
+    nhex += ch;
+    nsoft += cs;
+    nruns += cr;
+    sp += 2 * cp;
+    __syncthreads();
+  }
+  __syncthreads();
+  if (!ok && lane == 0) atomicExch(a.err, 1);
+  return ok;
+}
+
+template <int PCT, int PPT>
+__global__ __launch_bounds__(64) void fused_kernel(FusedArgs a) {
+  __shared__ float4 lds[FusedLds<PCT, PPT>::f4];
+  unsigned* stack = reinterpret_cast<unsigned*>(lds);
+  double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(lds);
+  float* redi = reinterpret_cast<float*>(reinterpret_cast<double*>(lds) + 4 * kWave);
+  PpLds<PPT>& ps = *reinterpret_cast<PpLds<PPT>*>(lds);
+  const int lane = threadIdx.x;
+  ILCell* cells = a.cells + (size_t)blockIdx.x * a.maxCells;
+  ILPart* runs = a.runs + (size_t)blockIdx.x * a.maxRuns;
+  ChangaGravityLaunch pc = a.pc, pp = a.pp;
+  pc.list = cells;
+  pp.list = runs;
+  pp.listKind = 1;
+  const int nb = a.pc.numBuckets;
+  int q = blockIdx.x % kFusedRanges;
+  for (int tried = 0; tried < kFusedRanges;) {
+    const int r0 = (int)((long long)nb * q / kFusedRanges);
+    const int r1 = (int)((long long)nb * (q + 1) / kFusedRanges);
+    int k = 0;
+    if (lane == 0) k = atomicAdd(a.counters + q * kFusedCounterStride, 1);
+    k = __builtin_amdgcn_readfirstlane(k);
+    if (r0 + k >= r1) {
+      ++tried;
+      q = q + 1 == kFusedRanges ? 0 : q + 1;
+      continue;
+    }
+    const int b = r0 + k;
+    int nhex, nsoft, nruns;
+    if (!fused_walk(a, b, lane, stack, cells, runs, nhex, nsoft, nruns)) return;
+    const int bstart = a.pc.starts[b], n = a.pc.sizes[b];
+    double cx0, cy0, cz0;
+    bucket_centre<true>(a.pc.tpos64, bstart, n, cx0, cy0, cz0);
+    for (int t0 = 0; t0 < n; t0 += kWave) {
+      const Lanes l = lanes_for(lane, min(kWave, n - t0));
+      const Target t = load_target<true>(a.pc, bstart + t0 + (l.live ? l.lane - l.slot * l.nt : 0),
+                                         cx0, cy0, cz0);
+      Acc64 acc = {0.0, 0.0, 0.0, 0.0, 0.0f};
+      pc_cells<true, PCT>(pc, 0, nhex, cx0, cy0, cz0, t, l, lds, acc);
+      pc_cells<true, PCT>(pc, a.maxCells - nsoft, a.maxCells, cx0, cy0, cz0, t, l, lds, acc);
+      pp_sources<true, 1, PPT>(pp, 0, nruns, cx0, cy0, cz0, t, l, ps, acc);
+      reduce_store(red, redi, lane, l.nt, l.nslot, l.slot, acc.ax, acc.ay, acc.az, acc.pot,
+                   acc.idt, a.pc.vars + bstart + t0 + (lane < l.nt ? lane : 0));
+    }
+  }
+}
+
+__global__ void fused_reset_kernel(int* counters) {
+  const int i = threadIdx.x;
+  if (i < kFusedRanges * kFusedCounterStride + 1) counters[i] = 0;  // counters, then err
+}
+
+using FusedFn = void (*)(FusedArgs);
+
+static int fused_grid(int* grid) {
+  int dev = 0, ncu = 0, per = 0;
+  CGPU_CHECK(hipGetDevice(&dev));
+  CGPU_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  CGPU_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per, (const void*)fused_kernel<kFusedPcTile, kFusedPpTile>, kWave, 0));
+  *grid = ncu * (per > 0 ? per : 1);
+  return 0;
+}
+
+static size_t fused_bytes(int grid, int maxCells, int maxRuns) {
+  const size_t head = 256 * ((sizeof(int) * (kFusedRanges * kFusedCounterStride + 1) + 255) / 256);
+  return head + (size_t)grid * maxCells * sizeof(ILCell) + (size_t)grid * maxRuns * sizeof(ILPart) +
+         256;
+}
+
+}  // namespace cgpu
+
+using namespace cgpu;
+
+extern "C" size_t changaFusedGravityWorkBytes(int maxCells, int maxRuns) {
+  int grid = 0;
+  if (maxCells <= 0 || maxRuns <= 0 || fused_grid(&grid)) return 0;
+  return fused_bytes(grid, maxCells, maxRuns);
+}
+
+extern "C" int changaFusedGravity(const ChangaFusedLaunch* f, void* stream) {
+  if (!f || !f->bucketNode || !f->first || !f->count || !f->child || !f->node64 || !f->work)
+    return fail("changaFusedGravity: missing argument", hipErrorInvalidValue, __FILE__, __LINE__);
+  const ChangaGravityLaunch& pc = f->cells;
+  const ChangaGravityLaunch& pp = f->parts;
+  if (!pc.targets || !pc.vars || !pc.moments || !pc.starts || !pc.sizes || !pc.tpos64 ||
+      !pc.cm64 || !pp.sources || !pp.spos64 || pp.targets != pc.targets || pp.vars != pc.vars ||
+      pp.starts != pc.starts || pp.sizes != pc.sizes || pp.numBuckets != pc.numBuckets)
+    return fail("changaFusedGravity: cell/particle launches (precision form, same buckets)",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  if (f->nReplicas < 0 || f->nReplicas > 3 || !(f->theta > 0.0) ||
+      (f->nReplicas > 0 && !(f->period > 0.0)) || f->maxCells <= 0 || f->maxRuns <= 0)
+    return fail("changaFusedGravity: theta, nReplicas (0..3), period, capacities",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  const int W = 2 * f->nReplicas + 1;
+  if (W * W * W > kFusedStack)
+    return fail("changaFusedGravity: replicas exceed the walk stack", hipErrorInvalidValue,
+                __FILE__, __LINE__);
+  if (f->numNodes >= (1 << kFusedNodeBits))
+    return fail("changaFusedGravity: more than 2^22 nodes", hipErrorInvalidValue, __FILE__,
+                __LINE__);
+  int grid = 0;
+  if (int e = fused_grid(&grid)) return e;
+  if (f->workBytes < fused_bytes(grid, f->maxCells, f->maxRuns))
+    return fail("changaFusedGravity: work buffer too small", hipErrorInvalidValue, __FILE__,
+                __LINE__);
+  if (pc.numBuckets <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  char* wb = static_cast<char*>(f->work);
+  const size_t head =
+      256 * ((sizeof(int) * (kFusedRanges * kFusedCounterStride + 1) + 255) / 256);
+  FusedArgs a;
+  a.pc = pc;
+  a.pp = pp;
+  a.bucketNode = f->bucketNode;
+  a.first = f->first;
+  a.count = f->count;
+  a.child = f->child;
+  a.node64 = f->node64;
+  a.kgeom = 2.0 / sqrt(3.0);
+  a.theta = f->theta;
+  a.thetaMono = f->theta * f->theta * f->theta * f->theta;
+  a.period = f->nReplicas > 0 ? f->period : 0.0;
+  a.R = f->nReplicas;
+  a.counters = reinterpret_cast<int*>(wb);
+  a.err = a.counters + kFusedRanges * kFusedCounterStride;
+  a.cells = reinterpret_cast<ILCell*>(wb + head);
+  a.runs = reinterpret_cast<ILPart*>(wb + head + (size_t)grid * f->maxCells * sizeof(ILCell));
+  a.maxCells = f->maxCells;
+  a.maxRuns = f->maxRuns;
+  hipLaunchKernelGGL(fused_reset_kernel, dim3(1), dim3(kFusedRanges * kFusedCounterStride + 1), 0,
+                     s, a.counters);
+  CGPU_CHECK(hipGetLastError());
+  hipLaunchKernelGGL((fused_kernel<kFusedPcTile, kFusedPpTile>), dim3(grid), dim3(kWave), 0, s, a);
+  CGPU_CHECK(hipGetLastError());
+  if (f->errorFlag) {
+    CGPU_CHECK(hipMemcpyAsync(f->errorFlag, a.err, sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  return 0;
+}

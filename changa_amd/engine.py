@@ -2,11 +2,12 @@
 
 The sequence mirrors ChaNGa's GPU path (TreePiece::startGravity ->
 DataManager::transferLocalToGPU -> TreePiece{Cell,Part}ListDataTransferLocal
--> EwaldHost -> TransferParticleVarsBack; SURVEY call stack 3.1/3.3): the host
-builds the tree, node geometry and interaction lists (libchanga_host.so); the
-GPU builds the multipole moments, then evaluates PC, PP and Ewald into the
-VariablePartData array (libchanga_gpu.so). PyTorch only provides device memory
-and the stream.
+-> EwaldHost -> TransferParticleVarsBack): the tree and the interaction
+lists come from the host (libchanga_host.so) or the GPU (changaBuildTree,
+changaBuildLists*); the GPU builds the multipole moments, then evaluates PC,
+PP and Ewald into the VariablePartData array (libchanga_gpu.so), from the
+lists or with the walk fused into the force launch (changaFusedGravity).
+PyTorch only provides device memory and the stream.
 """
 import ctypes
 import functools
@@ -482,6 +483,60 @@ class Evaluation:
         if moments:
             self.build_moments()
         self.run_batched()
+        self.run_ewald()
+
+    # ------------------------------------------------------------------ fused form
+    def fused_setup(self, max_cells=16384, max_runs=8192):
+        """Scratch of the fused walk-and-force launch (per-wavefront cell and
+        run capacities; a bucket whose walk exceeds them fails the launch)."""
+        if not self.precise:
+            raise ValueError("the fused form runs on the precision form (FP64 positions)")
+        nbytes = gpu.lib().changaFusedGravityWorkBytes(int(max_cells), int(max_runs))
+        if nbytes == 0:
+            raise RuntimeError("changaFusedGravityWorkBytes failed")
+        self.d_fused_work = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        self._fused_err = torch.zeros(1, dtype=torch.int32).pin_memory()
+        if getattr(self, "d_bucket_node", None) is None:
+            self.d_bucket_node = self._t(self.tree["bucket_node"][self.t_keep].astype(np.int32))
+        f = gpu.ChangaFusedLaunch()
+        f.cells = self._launch(True)
+        f.parts = self._launch(False)
+        f.bucketNode = self.d_bucket_node.data_ptr()
+        f.first = self.d_first.data_ptr()
+        f.count = self.d_count.data_ptr()
+        f.child = self.d_child.data_ptr()
+        f.node64 = self.d_node64.data_ptr()
+        f.numNodes = self.tree["n_nodes"]
+        f.theta = float(self.w["theta"])
+        f.nReplicas = int(self.w["n_replicas"]) if self.w["periodic"] else 0
+        f.period = self.L
+        f.maxCells, f.maxRuns = int(max_cells), int(max_runs)
+        f.work = self.d_fused_work.data_ptr()
+        f.workBytes = nbytes
+        f.errorFlag = self._fused_err.data_ptr()
+        self._fused = f
+
+    def run_fused(self):
+        """PC + PP of every target bucket by the fused walk-and-force launch."""
+        if getattr(self, "_fused", None) is None:
+            self.fused_setup()
+        gpu.check(gpu.lib().changaFusedGravity(ctypes.byref(self._fused), self._sp()),
+                  "changaFusedGravity")
+
+    def fused_check(self):
+        """Raise when the last fused launch (after a synchronize) overflowed
+        its scratch."""
+        if int(self._fused_err[0]):
+            raise RuntimeError("changaFusedGravity: a walk exceeded the scratch capacities "
+                               "(raise max_cells / max_runs in fused_setup)")
+
+    def step_fused(self, moments=True):
+        """One force evaluation in the fused form: multipole build, zero, the
+        fused walk and force, Ewald."""
+        if moments:
+            self.build_moments()
+        self.zero()
+        self.run_fused()
         self.run_ewald()
 
     # ------------------------------------------------------------------ adjoint form

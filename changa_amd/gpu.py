@@ -105,6 +105,15 @@ class ChangaTreeBuild(ctypes.Structure):
                 ("nDepths", c_int)]
 
 
+class ChangaFusedLaunch(ctypes.Structure):
+    _fields_ = [("cells", ChangaGravityLaunch), ("parts", ChangaGravityLaunch),
+                ("bucketNode", c_void_p), ("first", c_void_p), ("count", c_void_p),
+                ("child", c_void_p), ("node64", c_void_p), ("numNodes", c_int),
+                ("theta", ctypes.c_double), ("nReplicas", c_int), ("period", ctypes.c_double),
+                ("maxCells", c_int), ("maxRuns", c_int), ("work", c_void_p),
+                ("workBytes", c_size_t), ("errorFlag", c_void_p)]
+
+
 CALLBACK = ctypes.CFUNCTYPE(None, c_void_p)
 
 # every extern "C" symbol include/changa_gpu.h declares
@@ -121,7 +130,8 @@ EXPORTS = [
     "changaBatchPlanCreate", "changaBatchPlanLaunch", "changaBatchPlanRun", "changaBatchPlanDestroy",
     "changaGravityAdjoint", "changaGravityAdjointWorkInts", "changaGravityAdjointEwaldWorkDoubles",
     "changaBuildListsWorkBytes", "changaBuildListsCount", "changaBuildListsFill",
-    "changaBuildTreeWorkBytes", "changaBuildTree",
+    "changaBuildTreeWorkBytes", "changaBuildTree", "changaFusedGravityWorkBytes",
+    "changaFusedGravity",
 ]
 
 _lib = None
@@ -190,6 +200,9 @@ def lib():
     L.changaBuildTreeWorkBytes.argtypes = [c_int]
     L.changaBuildTreeWorkBytes.restype = c_size_t
     L.changaBuildTree.argtypes = [c_void_p, c_void_p]
+    L.changaFusedGravityWorkBytes.argtypes = [c_int, c_int]
+    L.changaFusedGravityWorkBytes.restype = c_size_t
+    L.changaFusedGravity.argtypes = [c_void_p, c_void_p]
     _lib = L
     return L
 

@@ -318,6 +318,41 @@ typedef struct {
 size_t changaBuildTreeWorkBytes(int n);
 int changaBuildTree(ChangaTreeBuild *t, void *stream);
 
+/* ======================================================================
+ * Part 7: fused walk and force
+ *
+ * Replaces gpuLocalTreeWalk (HostCUDA.h:129-131, HostCUDA.cu:724-942): one
+ * persistent launch in which each wavefront walks the tree for a target
+ * bucket (the opening tests of changaBuildLists*, so the same interaction
+ * set) and evaluates what it accepts at once, with no interaction list in
+ * device memory beyond a per-wavefront scratch of maxCells cells and maxRuns
+ * particle runs. Precision form only (pos64 in both launches). Results are
+ * added into cells.vars like changaNodeGravity + changaPartGravity; they
+ * equal the list path's up to the FP32 order of the per-tile sums.
+ * errorFlag (host, optional) receives 1, once the stream reaches it, when
+ * a bucket's walk exceeded a capacity; that bucket's results are then
+ * incomplete.
+ * ====================================================================== */
+typedef struct {
+  ChangaGravityLaunch cells;      /* targets, vars, moments, cm64, tpos64, starts, sizes,
+                                     numBuckets, fperiod; list and marks unused */
+  ChangaGravityLaunch parts;      /* sources, spos64 (same targets and buckets) */
+  const int *bucketNode;          /* device: node of each target bucket */
+  const int *first, *count, *child;  /* device: topology, as for changaBuildMoments */
+  const double *node64;           /* device: node records of changaBuildMoments */
+  int numNodes;
+  double theta;
+  int nReplicas;                  /* 0..3 */
+  double period;
+  int maxCells, maxRuns;          /* per-wavefront scratch capacities */
+  void *work;                     /* device scratch: changaFusedGravityWorkBytes(...) */
+  size_t workBytes;
+  int *errorFlag;                 /* host out, optional */
+} ChangaFusedLaunch;
+
+size_t changaFusedGravityWorkBytes(int maxCells, int maxRuns);
+int changaFusedGravity(const ChangaFusedLaunch *f, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
