@@ -28,14 +28,24 @@ constexpr int kFusedPpTile = 256;
 constexpr int kFusedRanges = kNumXcd;
 constexpr int kFusedCounterStride = 32;
 
+// What the walk reads of a node, 64 bytes in one record so that a lane's
+// visit is one cache line: the centre of mass, the opening radius
+// max(2/sqrt(3) r / theta, r), twice the softening, the monopole opening
+// radius 2/sqrt(3) r / theta^4, the particle count, children and first
+// particle. Built from the node records by fused_records_kernel with the
+// list builder's operations, so every decision is the list builder's.
+struct alignas(16) WalkRec {
+  double cx, cy, cz, rad, rs, rm;
+  int count, c0, c1, first;
+};
+static_assert(sizeof(WalkRec) == 64, "one cache line per node");
+
 struct FusedArgs {
-  ChangaGravityLaunch pc, pp;  // lists are the wavefronts' scratch (set per wave)
+  ChangaGravityLaunch pc, pp;  // lists: the bases of the wavefronts' scratch
   const int* bucketNode;
-  const int* first;
-  const int* count;
-  const int* child;
-  const double* node64;
-  double kgeom, theta, thetaMono, period;
+  const double* node64;        // the target bucket's geometry
+  const WalkRec* rec;          // walk records of every node
+  double period;
   int R;
   ILCell* cells;  // per wave: maxCells entries
   ILPart* runs;   // per wave: maxRuns entries
@@ -73,24 +83,42 @@ __device__ __forceinline__ bool fused_box_sphere(const double* lo, const double*
   return dsq <= r * r;
 }
 
+__global__ void fused_records_kernel(int nn, const double* node64, const int* first,
+                                     const int* count, const int* child, double kgeom,
+                                     double theta, double thetaMono, WalkRec* rec) {
+  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= nn) return;
+  const double* g = node64 + (size_t)nd * 34;
+  WalkRec v;
+  v.cx = g[0];
+  v.cy = g[1];
+  v.cz = g[2];
+  const double r = g[3];
+  double rad = kgeom * r / theta;
+  if (rad < r) rad = r;
+  v.rad = rad;
+  v.rs = 2.0 * g[4];
+  v.rm = kgeom * r / thetaMono;
+  v.count = count[nd];
+  v.c0 = child[2 * nd];
+  v.c1 = child[2 * nd + 1];
+  v.first = first[nd];
+  rec[nd] = v;
+}
+
 // 0 open, 1 hexadecapole cell, 2 softened cell (the list builder's test)
 __device__ __forceinline__ int fused_decide(const FusedArgs& a, const double* bg, double rb,
-                                            int nd, int code) {
-  if (a.count[nd] <= 6) return 0;
-  const double* g = a.node64 + (size_t)nd * 34;
+                                            const WalkRec& v, int code) {
+  if (v.count <= 6) return 0;
   const int x = (code & 7) - 3, y = ((code >> 3) & 7) - 3, z = ((code >> 6) & 7) - 3;
-  const double c3[3] = {g[0] + x * a.period, g[1] + y * a.period, g[2] + z * a.period};
-  const double r = g[3];
-  double rad = a.kgeom * r / a.theta;
-  if (rad < r) rad = r;
-  if (fused_box_sphere(bg + 6, bg + 9, c3, rad)) return 0;
-  const double rs = 2.0 * g[4];
+  const double c3[3] = {v.cx + x * a.period, v.cy + y * a.period, v.cz + z * a.period};
+  if (fused_box_sphere(bg + 6, bg + 9, c3, v.rad)) return 0;
+  const double rs = v.rs;
   const double dx = c3[0] - bg[0], dy = c3[1] - bg[1], dz = c3[2] - bg[2];
   const bool soft_overlap = dx * dx + dy * dy + dz * dz <= (rs + rb) * (rs + rb) ||
                             fused_box_sphere(bg + 6, bg + 9, c3, rs);
   if (!soft_overlap) return 1;
-  const double rm = a.kgeom * r / a.thetaMono;
-  return fused_box_sphere(bg + 6, bg + 9, c3, rm) ? 0 : 2;
+  return fused_box_sphere(bg + 6, bg + 9, c3, v.rm) ? 0 : 2;
 }
 
 #pragma clang fp contract(on)
@@ -132,13 +160,15 @@ __device__ bool fused_walk(const FusedArgs& a, int b, int lane, unsigned* stack,
     __syncthreads();
     sp -= n;
     int kind = -1, nd = 0, code = 0;  // -1 none, 0 open, 1 hex, 2 soft
+    WalkRec v = {};
     if (lane < n) {
       nd = (int)(e & ((1u << kFusedNodeBits) - 1u));
       code = (int)(e >> kFusedNodeBits);
-      kind = fused_decide(a, bg, rb, nd, code);
+      v = a.rec[nd];
+      kind = fused_decide(a, bg, rb, v, code);
     }
     const int oid = code << 22;
-    const bool is_bucket = kind == 0 && a.child[2 * nd] < 0;
+    const bool is_bucket = kind == 0 && v.c0 < 0;
     const bool is_push = kind == 0 && !is_bucket;
     const unsigned long long mh = __ballot(kind == 1), ms = __ballot(kind == 2),
                              mr = __ballot(is_bucket), mp = __ballot(is_push);
@@ -151,12 +181,12 @@ __device__ bool fused_walk(const FusedArgs& a, int b, int lane, unsigned* stack,
     if (kind == 1) cells[nhex + excl_count(mh, lane)] = {nd, oid | b};
     if (kind == 2)
       cells[a.maxCells - 1 - (nsoft + excl_count(ms, lane))] = {nd, oid | b | (int)0x80000000u};
-    if (is_bucket) runs[nruns + excl_count(mr, lane)] = {a.first[nd], oid, a.count[nd]};
+    if (is_bucket) runs[nruns + excl_count(mr, lane)] = {v.first, oid, v.count};
     if (is_push) {
       const int at = sp + 2 * excl_count(mp, lane);
       const unsigned hi = (unsigned)code << kFusedNodeBits;
-      stack[at] = hi | (unsigned)a.child[2 * nd + 1];
-      stack[at + 1] = hi | (unsigned)a.child[2 * nd];
+      stack[at] = hi | (unsigned)v.c1;
+      stack[at + 1] = hi | (unsigned)v.c0;
     }
     nhex += ch;
     nsoft += cs;
@@ -177,12 +207,10 @@ __global__ __launch_bounds__(64) void fused_kernel(FusedArgs a) {
   float* redi = reinterpret_cast<float*>(reinterpret_cast<double*>(lds) + 4 * kWave);
   PpLds<PPT>& ps = *reinterpret_cast<PpLds<PPT>*>(lds);
   const int lane = threadIdx.x;
-  ILCell* cells = a.cells + (size_t)blockIdx.x * a.maxCells;
-  ILPart* runs = a.runs + (size_t)blockIdx.x * a.maxRuns;
-  ChangaGravityLaunch pc = a.pc, pp = a.pp;
-  pc.list = cells;
-  pp.list = runs;
-  pp.listKind = 1;
+  // this wavefront's scratch: a.pc.list / a.pp.list are the scratch bases
+  const int cbase = blockIdx.x * a.maxCells, rbase = blockIdx.x * a.maxRuns;
+  ILCell* cells = a.cells + cbase;
+  ILPart* runs = a.runs + rbase;
   const int nb = a.pc.numBuckets;
   int q = blockIdx.x % kFusedRanges;
   for (int tried = 0; tried < kFusedRanges;) {
@@ -207,9 +235,10 @@ __global__ __launch_bounds__(64) void fused_kernel(FusedArgs a) {
       const Target t = load_target<true>(a.pc, bstart + t0 + (l.live ? l.lane - l.slot * l.nt : 0),
                                          cx0, cy0, cz0);
       Acc64 acc = {0.0, 0.0, 0.0, 0.0, 0.0f};
-      pc_cells<true, PCT>(pc, 0, nhex, cx0, cy0, cz0, t, l, lds, acc);
-      pc_cells<true, PCT>(pc, a.maxCells - nsoft, a.maxCells, cx0, cy0, cz0, t, l, lds, acc);
-      pp_sources<true, 1, PPT>(pp, 0, nruns, cx0, cy0, cz0, t, l, ps, acc);
+      pc_cells<true, PCT>(a.pc, cbase, cbase + nhex, cx0, cy0, cz0, t, l, lds, acc);
+      pc_cells<true, PCT>(a.pc, cbase + a.maxCells - nsoft, cbase + a.maxCells, cx0, cy0, cz0, t,
+                          l, lds, acc);
+      pp_sources<true, 1, PPT>(a.pp, rbase, rbase + nruns, cx0, cy0, cz0, t, l, ps, acc);
       reduce_store(red, redi, lane, l.nt, l.nslot, l.slot, acc.ax, acc.ay, acc.az, acc.pot,
                    acc.idt, a.pc.vars + bstart + t0 + (lane < l.nt ? lane : 0));
     }
@@ -233,20 +262,29 @@ static int fused_grid(int* grid) {
   return 0;
 }
 
-static size_t fused_bytes(int grid, int maxCells, int maxRuns) {
-  const size_t head = 256 * ((sizeof(int) * (kFusedRanges * kFusedCounterStride + 1) + 255) / 256);
-  return head + (size_t)grid * maxCells * sizeof(ILCell) + (size_t)grid * maxRuns * sizeof(ILPart) +
-         256;
+// work buffer: claim counters and error flag (256-byte aligned head), the
+// walk records, then the per-wavefront cell and run scratch
+struct FusedWork {
+  size_t rec, cells, runs, total;
+};
+
+static FusedWork fused_layout(int grid, int numNodes, int maxCells, int maxRuns) {
+  FusedWork w;
+  w.rec = 256 * ((sizeof(int) * (kFusedRanges * kFusedCounterStride + 1) + 255) / 256);
+  w.cells = w.rec + 256 * (((size_t)numNodes * sizeof(WalkRec) + 255) / 256);
+  w.runs = w.cells + 256 * (((size_t)grid * maxCells * sizeof(ILCell) + 255) / 256);
+  w.total = w.runs + (size_t)grid * maxRuns * sizeof(ILPart);
+  return w;
 }
 
 }  // namespace cgpu
 
 using namespace cgpu;
 
-extern "C" size_t changaFusedGravityWorkBytes(int maxCells, int maxRuns) {
+extern "C" size_t changaFusedGravityWorkBytes(int numNodes, int maxCells, int maxRuns) {
   int grid = 0;
-  if (maxCells <= 0 || maxRuns <= 0 || fused_grid(&grid)) return 0;
-  return fused_bytes(grid, maxCells, maxRuns);
+  if (numNodes <= 0 || maxCells <= 0 || maxRuns <= 0 || fused_grid(&grid)) return 0;
+  return fused_layout(grid, numNodes, maxCells, maxRuns).total;
 }
 
 extern "C" int changaFusedGravity(const ChangaFusedLaunch* f, void* stream) {
@@ -267,40 +305,47 @@ extern "C" int changaFusedGravity(const ChangaFusedLaunch* f, void* stream) {
   if (W * W * W > kFusedStack)
     return fail("changaFusedGravity: replicas exceed the walk stack", hipErrorInvalidValue,
                 __FILE__, __LINE__);
-  if (f->numNodes >= (1 << kFusedNodeBits))
-    return fail("changaFusedGravity: more than 2^22 nodes", hipErrorInvalidValue, __FILE__,
+  if (f->numNodes <= 0 || f->numNodes > (1 << kFusedNodeBits))
+    return fail("changaFusedGravity: numNodes in 1..2^22", hipErrorInvalidValue, __FILE__,
                 __LINE__);
   int grid = 0;
   if (int e = fused_grid(&grid)) return e;
-  if (f->workBytes < fused_bytes(grid, f->maxCells, f->maxRuns))
+  if ((long long)grid * f->maxCells > 0x7fffffffll || (long long)grid * f->maxRuns > 0x7fffffffll)
+    return fail("changaFusedGravity: scratch capacities exceed the int32 index range",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  const FusedWork L = fused_layout(grid, f->numNodes, f->maxCells, f->maxRuns);
+  if (f->workBytes < L.total)
     return fail("changaFusedGravity: work buffer too small", hipErrorInvalidValue, __FILE__,
                 __LINE__);
   if (pc.numBuckets <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   char* wb = static_cast<char*>(f->work);
-  const size_t head =
-      256 * ((sizeof(int) * (kFusedRanges * kFusedCounterStride + 1) + 255) / 256);
   FusedArgs a;
   a.pc = pc;
   a.pp = pp;
   a.bucketNode = f->bucketNode;
-  a.first = f->first;
-  a.count = f->count;
-  a.child = f->child;
   a.node64 = f->node64;
-  a.kgeom = 2.0 / sqrt(3.0);
-  a.theta = f->theta;
-  a.thetaMono = f->theta * f->theta * f->theta * f->theta;
+  a.rec = reinterpret_cast<const WalkRec*>(wb + L.rec);
   a.period = f->nReplicas > 0 ? f->period : 0.0;
   a.R = f->nReplicas;
   a.counters = reinterpret_cast<int*>(wb);
   a.err = a.counters + kFusedRanges * kFusedCounterStride;
-  a.cells = reinterpret_cast<ILCell*>(wb + head);
-  a.runs = reinterpret_cast<ILPart*>(wb + head + (size_t)grid * f->maxCells * sizeof(ILCell));
+  a.cells = reinterpret_cast<ILCell*>(wb + L.cells);
+  a.runs = reinterpret_cast<ILPart*>(wb + L.runs);
   a.maxCells = f->maxCells;
   a.maxRuns = f->maxRuns;
+  a.pc.list = a.cells;
+  a.pc.listKind = 0;
+  a.pp.list = a.runs;
+  a.pp.listKind = 1;
   hipLaunchKernelGGL(fused_reset_kernel, dim3(1), dim3(kFusedRanges * kFusedCounterStride + 1), 0,
                      s, a.counters);
+  CGPU_CHECK(hipGetLastError());
+  // the opening radii in the list builder's operations (its theta^4 order)
+  const double theta = f->theta, thetaMono = theta * theta * theta * theta;
+  hipLaunchKernelGGL(fused_records_kernel, dim3((f->numNodes + 255) / 256), dim3(256), 0, s,
+                     f->numNodes, f->node64, f->first, f->count, f->child, 2.0 / sqrt(3.0),
+                     theta, thetaMono, const_cast<WalkRec*>(a.rec));
   CGPU_CHECK(hipGetLastError());
   hipLaunchKernelGGL((fused_kernel<kFusedPcTile, kFusedPpTile>), dim3(grid), dim3(kWave), 0, s, a);
   CGPU_CHECK(hipGetLastError());

@@ -491,7 +491,8 @@ class Evaluation:
         run capacities; a bucket whose walk exceeds them fails the launch)."""
         if not self.precise:
             raise ValueError("the fused form runs on the precision form (FP64 positions)")
-        nbytes = gpu.lib().changaFusedGravityWorkBytes(int(max_cells), int(max_runs))
+        nbytes = gpu.lib().changaFusedGravityWorkBytes(self.tree["n_nodes"], int(max_cells),
+                                                       int(max_runs))
         if nbytes == 0:
             raise RuntimeError("changaFusedGravityWorkBytes failed")
         self.d_fused_work = torch.empty(nbytes, dtype=torch.uint8, device=self.device)

@@ -200,7 +200,7 @@ def lib():
     L.changaBuildTreeWorkBytes.argtypes = [c_int]
     L.changaBuildTreeWorkBytes.restype = c_size_t
     L.changaBuildTree.argtypes = [c_void_p, c_void_p]
-    L.changaFusedGravityWorkBytes.argtypes = [c_int, c_int]
+    L.changaFusedGravityWorkBytes.argtypes = [c_int, c_int, c_int]
     L.changaFusedGravityWorkBytes.restype = c_size_t
     L.changaFusedGravity.argtypes = [c_void_p, c_void_p]
     _lib = L

@@ -345,12 +345,13 @@ typedef struct {
   int nReplicas;                  /* 0..3 */
   double period;
   int maxCells, maxRuns;          /* per-wavefront scratch capacities */
-  void *work;                     /* device scratch: changaFusedGravityWorkBytes(...) */
+  void *work;                     /* device scratch: changaFusedGravityWorkBytes(numNodes,
+                                     maxCells, maxRuns) */
   size_t workBytes;
   int *errorFlag;                 /* host out, optional */
 } ChangaFusedLaunch;
 
-size_t changaFusedGravityWorkBytes(int maxCells, int maxRuns);
+size_t changaFusedGravityWorkBytes(int numNodes, int maxCells, int maxRuns);
 int changaFusedGravity(const ChangaFusedLaunch *f, void *stream);
 
 #ifdef __cplusplus
