@@ -6,9 +6,8 @@
 //
 // Walk: wavefront-cooperative over an LDS stack of (node, replica) entries
 // packed in 32 bits; each iteration pops up to 64 entries, one per lane, and
-// applies openCriterionBucket (gravity.h:597-635) with the softening re-test
-// (openSoftening, gravity.h:251-260) in FP64 without contraction, exactly as
-// the list builder (walk.hip) does. Accepted cells, softened cells and
+// applies the list builder's opening test (openCriterionBucket with the
+// softening re-test, walk_rec.h) to their walk records. Accepted cells, softened cells and
 // opened buckets (particle runs) are appended, through wavefront prefix
 // sums, to the wavefront's own scratch lists (L2-resident); opened internal
 // nodes push their two children. The interaction set is the list builder's;
@@ -18,6 +17,7 @@
 // list path (gravity_body.h) over the scratch lists, precision form, into
 // FP64 accumulators and one update of the result array per chunk.
 #include "gravity_body.h"
+#include "walk_rec.h"
 
 namespace cgpu {
 
@@ -27,18 +27,6 @@ constexpr int kFusedPcTile = 64;
 constexpr int kFusedPpTile = 256;
 constexpr int kFusedRanges = kNumXcd;
 constexpr int kFusedCounterStride = 32;
-
-// What the walk reads of a node, 64 bytes in one record so that a lane's
-// visit is one cache line: the centre of mass, the opening radius
-// max(2/sqrt(3) r / theta, r), twice the softening, the monopole opening
-// radius 2/sqrt(3) r / theta^4, the particle count, children and first
-// particle. Built from the node records by fused_records_kernel with the
-// list builder's operations, so every decision is the list builder's.
-struct alignas(16) WalkRec {
-  double cx, cy, cz, rad, rs, rm;
-  int count, c0, c1, first;
-};
-static_assert(sizeof(WalkRec) == 64, "one cache line per node");
 
 struct FusedArgs {
   ChangaGravityLaunch pc, pp;  // lists: the bases of the wavefronts' scratch
@@ -65,63 +53,6 @@ struct FusedLds {
   static constexpr size_t bytes = m1 > m2 ? m1 : m2;
   static constexpr size_t f4 = (bytes + 15) / 16;
 };
-
-#pragma clang fp contract(off)
-
-__device__ __forceinline__ bool fused_box_sphere(const double* lo, const double* hi,
-                                                 const double c[3], double r) {
-  double dsq = 0.0;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double e = 0.0;
-    if (c[k] < lo[k])
-      e = lo[k] - c[k];
-    else if (c[k] > hi[k])
-      e = c[k] - hi[k];
-    dsq += e * e;
-  }
-  return dsq <= r * r;
-}
-
-__global__ void fused_records_kernel(int nn, const double* node64, const int* first,
-                                     const int* count, const int* child, double kgeom,
-                                     double theta, double thetaMono, WalkRec* rec) {
-  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= nn) return;
-  const double* g = node64 + (size_t)nd * 34;
-  WalkRec v;
-  v.cx = g[0];
-  v.cy = g[1];
-  v.cz = g[2];
-  const double r = g[3];
-  double rad = kgeom * r / theta;
-  if (rad < r) rad = r;
-  v.rad = rad;
-  v.rs = 2.0 * g[4];
-  v.rm = kgeom * r / thetaMono;
-  v.count = count[nd];
-  v.c0 = child[2 * nd];
-  v.c1 = child[2 * nd + 1];
-  v.first = first[nd];
-  rec[nd] = v;
-}
-
-// 0 open, 1 hexadecapole cell, 2 softened cell (the list builder's test)
-__device__ __forceinline__ int fused_decide(const FusedArgs& a, const double* bg, double rb,
-                                            const WalkRec& v, int code) {
-  if (v.count <= 6) return 0;
-  const int x = (code & 7) - 3, y = ((code >> 3) & 7) - 3, z = ((code >> 6) & 7) - 3;
-  const double c3[3] = {v.cx + x * a.period, v.cy + y * a.period, v.cz + z * a.period};
-  if (fused_box_sphere(bg + 6, bg + 9, c3, v.rad)) return 0;
-  const double rs = v.rs;
-  const double dx = c3[0] - bg[0], dy = c3[1] - bg[1], dz = c3[2] - bg[2];
-  const bool soft_overlap = dx * dx + dy * dy + dz * dz <= (rs + rb) * (rs + rb) ||
-                            fused_box_sphere(bg + 6, bg + 9, c3, rs);
-  if (!soft_overlap) return 1;
-  return fused_box_sphere(bg + 6, bg + 9, c3, v.rm) ? 0 : 2;
-}
-
-#pragma clang fp contract(on)
 
 __device__ __forceinline__ int excl_count(unsigned long long mask, int lane) {
   return __popcll(mask & ((1ull << lane) - 1ull));
@@ -165,7 +96,8 @@ __device__ bool fused_walk(const FusedArgs& a, int b, int lane, unsigned* stack,
       nd = (int)(e & ((1u << kFusedNodeBits) - 1u));
       code = (int)(e >> kFusedNodeBits);
       v = a.rec[nd];
-      kind = fused_decide(a, bg, rb, v, code);
+      const int x = (code & 7) - 3, y = ((code >> 3) & 7) - 3, z = ((code >> 6) & 7) - 3;
+      kind = walk_decide(v, bg, rb, x * a.period, y * a.period, z * a.period);
     }
     const int oid = code << 22;
     const bool is_bucket = kind == 0 && v.c0 < 0;
@@ -235,10 +167,12 @@ __global__ __launch_bounds__(64) void fused_kernel(FusedArgs a) {
       const Target t = load_target<true>(a.pc, bstart + t0 + (l.live ? l.lane - l.slot * l.nt : 0),
                                          cx0, cy0, cz0);
       Acc64 acc = {0.0, 0.0, 0.0, 0.0, 0.0f};
-      pc_cells<true, PCT>(a.pc, cbase, cbase + nhex, cx0, cy0, cz0, t, l, lds, acc);
-      pc_cells<true, PCT>(a.pc, cbase + a.maxCells - nsoft, cbase + a.maxCells, cx0, cy0, cz0, t,
-                          l, lds, acc);
-      pp_sources<true, 1, PPT>(a.pp, rbase, rbase + nruns, cx0, cy0, cz0, t, l, ps, acc);
+      // the scratch is re-read per chunk and rewritten per bucket: plain loads
+      // keep it in L2
+      pc_cells<true, PCT, false>(a.pc, cbase, cbase + nhex, cx0, cy0, cz0, t, l, lds, acc);
+      pc_cells<true, PCT, false>(a.pc, cbase + a.maxCells - nsoft, cbase + a.maxCells, cx0, cy0,
+                                 cz0, t, l, lds, acc);
+      pp_sources<true, 1, PPT, false>(a.pp, rbase, rbase + nruns, cx0, cy0, cz0, t, l, ps, acc);
       reduce_store(red, redi, lane, l.nt, l.nslot, l.slot, acc.ax, acc.ay, acc.az, acc.pot,
                    acc.idt, a.pc.vars + bstart + t0 + (lane < l.nt ? lane : 0));
     }
@@ -341,12 +275,8 @@ extern "C" int changaFusedGravity(const ChangaFusedLaunch* f, void* stream) {
   hipLaunchKernelGGL(fused_reset_kernel, dim3(1), dim3(kFusedRanges * kFusedCounterStride + 1), 0,
                      s, a.counters);
   CGPU_CHECK(hipGetLastError());
-  // the opening radii in the list builder's operations (its theta^4 order)
-  const double theta = f->theta, thetaMono = theta * theta * theta * theta;
-  hipLaunchKernelGGL(fused_records_kernel, dim3((f->numNodes + 255) / 256), dim3(256), 0, s,
-                     f->numNodes, f->node64, f->first, f->count, f->child, 2.0 / sqrt(3.0),
-                     theta, thetaMono, const_cast<WalkRec*>(a.rec));
-  CGPU_CHECK(hipGetLastError());
+  CGPU_CHECK(walk_records(f->numNodes, f->node64, f->first, f->count, f->child, f->theta,
+                          const_cast<WalkRec*>(a.rec), s));
   hipLaunchKernelGGL((fused_kernel<kFusedPcTile, kFusedPpTile>), dim3(grid), dim3(kWave), 0, s, a);
   CGPU_CHECK(hipGetLastError());
   if (f->errorFlag) {

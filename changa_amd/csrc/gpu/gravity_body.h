@@ -214,11 +214,19 @@ __device__ __forceinline__ ILPart load_stream(const ILPart* p) {
   return r;
 }
 
-template <bool PRECISE>
+// STREAM false: plain loads, for lists that are re-read or were just written
+// (a streaming load of a dirty line sends it back to memory early).
+template <bool STREAM, typename T>
+__device__ __forceinline__ T load_entry(const T* p) {
+  if constexpr (STREAM) return load_stream(p);
+  else return *p;
+}
+
+template <bool PRECISE, bool STREAM = true>
 __device__ __forceinline__ void load_cell(const ChangaGravityLaunch& a, const ILCell* cells, int e,
                                           int end, WireCell& w) {
   if (e < end) {
-    const ILCell c = load_stream(cells + e);
+    const ILCell c = load_entry<STREAM>(cells + e);
     w.idx = c.index;
     w.oid = c.offsetID;
     const float* src = reinterpret_cast<const float*>(a.moments) + (size_t)c.index * 27;
@@ -337,7 +345,7 @@ __device__ __forceinline__ Target load_target(const ChangaGravityLaunch& a, int 
 // LDS); the next tile's wire records are in flight while this one is
 // evaluated. A tile without softened cells (ballot) runs the hexadecapole
 // loop with no per-cell test.
-template <bool PRECISE, int TILE>
+template <bool PRECISE, int TILE, bool STREAM = true>
 __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start, int end,
                                          double cx0, double cy0, double cz0, const Target& t,
                                          const Lanes& l, float4* rec4, Acc64& acc) {
@@ -347,7 +355,7 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
   const bool stager = l.lane < TILE;
   WireCell w;
   w.oid = 0;
-  load_cell<PRECISE>(a, cells, start + l.lane, stager ? end : 0, w);
+  load_cell<PRECISE, STREAM>(a, cells, start + l.lane, stager ? end : 0, w);
   for (int base = start; base < end; base += TILE) {
     const int nc = min(TILE, end - base);
     const bool mine = l.lane < nc;
@@ -355,7 +363,7 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
     if (mine) stage_cell<PRECISE>(a, w, L, cx0, cy0, cz0, rec4 + l.lane * (kPcRec / 4));
     __syncthreads();
     // issued after the barrier, whose fence would otherwise wait for them
-    load_cell<PRECISE>(a, cells, base + TILE + l.lane, stager ? end : 0, w);
+    load_cell<PRECISE, STREAM>(a, cells, base + TILE + l.lane, stager ? end : 0, w);
     if (l.live) {
       Acc32 A = {0.f, 0.f, 0.f, 0.f, acc.idt};
       if (!soft_tile) {
@@ -398,7 +406,7 @@ struct PpLds {
 // ILCell per source particle (the list ChaNGa's GenericList<ILPart>::serialize
 // produces, Compute.cpp:1118-1218); KIND 1: ILPart runs expanded here through
 // a wavefront prefix sum.
-template <bool PRECISE, int KIND, int TILE, typename Consume>
+template <bool PRECISE, int KIND, int TILE, bool STREAM = true, typename Consume>
 __device__ __forceinline__ void pp_tiles(const ChangaGravityLaunch& a, int start, int end,
                                          double cx0, double cy0, double cz0, int lane,
                                          PpLds<TILE>& s, Consume&& consume) {
@@ -427,7 +435,7 @@ __device__ __forceinline__ void pp_tiles(const ChangaGravityLaunch& a, int start
     for (int base = start; base < end; base += TILE) {
       const int ns = min(TILE, end - base);
       for (int i = lane; i < ns; i += kWave) {
-        const ILCell c = load_stream(lst + base + i);
+        const ILCell c = load_entry<STREAM>(lst + base + i);
         stage(i, c.index, c.offsetID);
       }
       __syncthreads();
@@ -440,7 +448,7 @@ __device__ __forceinline__ void pp_tiles(const ChangaGravityLaunch& a, int start
       const int nr = min(kWave, end - rb);
       int num = 0;
       if (lane < nr) {
-        const ILPart p = load_stream(lst + rb + lane);
+        const ILPart p = load_entry<STREAM>(lst + rb + lane);
         s.r_idx[lane] = p.index;
         s.r_oid[lane] = p.off;
         num = p.num;
@@ -478,11 +486,11 @@ __device__ __forceinline__ void pp_tiles(const ChangaGravityLaunch& a, int start
 }
 
 // Sources [start, end) of a.list onto the chunk's targets (partBucketForce).
-template <bool PRECISE, int KIND, int TILE>
+template <bool PRECISE, int KIND, int TILE, bool STREAM = true>
 __device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int start, int end,
                                            double cx0, double cy0, double cz0, const Target& t,
                                            const Lanes& l, PpLds<TILE>& s, Acc64& acc) {
-  pp_tiles<PRECISE, KIND, TILE>(a, start, end, cx0, cy0, cz0, l.lane, s, [&](int ns) {
+  pp_tiles<PRECISE, KIND, TILE, STREAM>(a, start, end, cx0, cy0, cz0, l.lane, s, [&](int ns) {
     if (l.live) {
       Acc32 A = {0.f, 0.f, 0.f, 0.f, acc.idt};
       for (int k = l.slot; k < ns; k += l.nslot) {

@@ -15,16 +15,18 @@
 // softened cells, then particle runs (the layout ListCompute::stateReady
 // serializes, Compute.cpp:1735-1741, 1806).
 //
-// One lane per target bucket walks depth first with a private stack, in the
+// One lane per target bucket walks depth first with its own stack (in LDS), in the
 // host walk's order (replicas from (-R, -R, -R) to (R, R, R), z fastest;
 // the first child first), so the lists are the host walk's entry for entry.
-// The geometry tests run in FP64 with contraction off, so every opening
-// decision is the host's on the same node geometry. Two passes: count (per
-// bucket: hexadecapole cells, softened cells, runs, source particles), then,
-// after the caller allocated the lists, fill.
+// The nodes are read as 64-byte walk records and tested in FP64 with
+// contraction off (walk_rec.h), so every opening decision is the host's on
+// the same node geometry. Two passes: count (per bucket: hexadecapole cells,
+// softened cells, runs, source particles; it also writes the walk records),
+// then, after the caller allocated the lists, fill.
 #include <rocprim/device/device_scan.hpp>
 
 #include "common.h"
+#include "walk_rec.h"
 
 namespace cgpu {
 
@@ -36,11 +38,9 @@ struct WalkArgs {
   int* nCell;  // count pass: cells (hexadecapole + softened) and runs per bucket
   int* nPart;
   const int* bucketNode;
-  const int* first;
-  const int* count;
-  const int* child;
-  const double* node64;
-  double kgeom, theta, thetaMono, period;
+  const double* node64;  // the target buckets' geometry
+  const WalkRec* rec;    // walk records of every node
+  double period;
   int R;
   int* cellMarks;
   int* partMarks;
@@ -51,27 +51,12 @@ struct WalkArgs {
   int* err;
 };
 
-#pragma clang fp contract(off)
-
-__device__ __forceinline__ bool box_sphere(const double* lo, const double* hi, const double c[3],
-                                           double r) {
-  double dsq = 0.0;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double e = 0.0;
-    if (c[k] < lo[k])
-      e = lo[k] - c[k];
-    else if (c[k] > hi[k])
-      e = c[k] - hi[k];
-    dsq += e * e;
-  }
-  return dsq <= r * r;
-}
-
 // node record (node64): cm.xyz, radius, soft, mass, lo.xyz, hi.xyz, moments
 template <bool FILL>
 __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
-  const int b = blockIdx.x * kWalkBlock + threadIdx.x;
+  // blocks of 64 consecutive buckets, each XCD taking a contiguous range of
+  // blocks so that its L2 holds the nodes near its buckets
+  const int b = xcd_bucket(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
   if (b >= a.nb) return;
   double bg[12];
   {
@@ -87,64 +72,39 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
     spos = a.softStart[b];
     ppos = a.partMarks[b];
   }
-  int stack[kWalkStack];
+  // the lanes' stacks in LDS, entry k of lane l at [k][l] (a private array
+  // would live in scratch memory)
+  __shared__ int s_stack[kWalkStack * kWalkBlock];
+  int* stack = s_stack + threadIdx.x;
   bool overflow = false;
   for (int x = -a.R; x <= a.R && !overflow; ++x)
     for (int y = -a.R; y <= a.R && !overflow; ++y)
       for (int z = -a.R; z <= a.R && !overflow; ++z) {
         const int oid = ((x + 3) | ((y + 3) << 3) | ((z + 3) << 6)) << 22;
-        const double off[3] = {x * a.period, y * a.period, z * a.period};
+        const double ox = x * a.period, oy = y * a.period, oz = z * a.period;
         int sp = 0;
-        stack[sp++] = 0;
+        stack[kWalkBlock * sp++] = 0;
         while (sp > 0) {
-          const int nd = stack[--sp];
-          const double* g = a.node64 + (size_t)nd * 34;
-          const int cnt = a.count[nd];
-          bool open = true, softened = false;
-          if (cnt > 6) {
-            const double c3[3] = {g[0] + off[0], g[1] + off[1], g[2] + off[2]};
-            const double r = g[3];
-            double rad = a.kgeom * r / a.theta;
-            if (rad < r) rad = r;
-            if (!box_sphere(bg + 6, bg + 9, c3, rad)) {
-              // openSoftening: the node's softening sphere against the bucket's
-              const double rs = 2.0 * g[4];
-              const double dx = c3[0] - bg[0], dy = c3[1] - bg[1], dz = c3[2] - bg[2];
-              const bool soft_overlap = dx * dx + dy * dy + dz * dz <= (rs + rb) * (rs + rb) ||
-                                        box_sphere(bg + 6, bg + 9, c3, rs);
-              if (!soft_overlap) {
-                open = false;
-              } else {
-                const double rm = a.kgeom * r / a.thetaMono;
-                if (!box_sphere(bg + 6, bg + 9, c3, rm)) {
-                  open = false;
-                  softened = true;
-                }
-              }
-            }
-          }
-          if (!open) {
-            if (softened) {
-              if (FILL) a.cells[spos + nsoft] = {nd, oid | b | (int)0x80000000u};
-              ++nsoft;
-            } else {
-              if (FILL) a.cells[hpos + nhex] = {nd, oid | b};
-              ++nhex;
-            }
+          const int nd = stack[kWalkBlock * --sp];
+          const WalkRec v = a.rec[nd];
+          const int kind = walk_decide(v, bg, rb, ox, oy, oz);
+          if (kind == 2) {
+            if (FILL) a.cells[spos + nsoft] = {nd, oid | b | (int)0x80000000u};
+            ++nsoft;
+          } else if (kind == 1) {
+            if (FILL) a.cells[hpos + nhex] = {nd, oid | b};
+            ++nhex;
+          } else if (v.c0 < 0) {
+            if (FILL) a.parts[ppos + nruns] = {v.first, oid, v.count};
+            ++nruns;
+            nsrc += v.count;
           } else {
-            const int c0 = a.child[2 * nd];
-            if (c0 < 0) {
-              if (FILL) a.parts[ppos + nruns] = {a.first[nd], oid, cnt};
-              ++nruns;
-              nsrc += cnt;
-            } else {
-              if (sp + 2 > kWalkStack) {
-                overflow = true;
-                break;
-              }
-              stack[sp++] = a.child[2 * nd + 1];
-              stack[sp++] = c0;
+            if (sp + 2 > kWalkStack) {
+              overflow = true;
+              break;
             }
+            stack[kWalkBlock * sp++] = v.c1;
+            stack[kWalkBlock * sp++] = v.c0;
           }
         }
       }
@@ -164,7 +124,42 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
   }
 }
 
+#pragma clang fp contract(off)
+
+__global__ void walk_records_kernel(int nn, const double* node64, const int* first,
+                                    const int* count, const int* child, double kgeom,
+                                    double theta, double thetaMono, WalkRec* rec) {
+  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= nn) return;
+  const double* g = node64 + (size_t)nd * 34;
+  WalkRec v;
+  v.cx = g[0];
+  v.cy = g[1];
+  v.cz = g[2];
+  const double r = g[3];
+  double rad = kgeom * r / theta;
+  if (rad < r) rad = r;
+  v.rad = rad;
+  v.rs = 2.0 * g[4];
+  v.rm = kgeom * r / thetaMono;
+  v.count = count[nd];
+  v.c0 = child[2 * nd];
+  v.c1 = child[2 * nd + 1];
+  v.first = first[nd];
+  rec[nd] = v;
+}
+
 #pragma clang fp contract(on)
+
+hipError_t walk_records(int nn, const double* node64, const int* first, const int* count,
+                        const int* child, double theta, WalkRec* rec, hipStream_t s) {
+  if (nn <= 0) return hipSuccess;
+  // theta^4 in the host walk's order
+  const double thetaMono = theta * theta * theta * theta;
+  hipLaunchKernelGGL(walk_records_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, nn, node64,
+                     first, count, child, 2.0 / sqrt(3.0), theta, thetaMono, rec);
+  return hipGetLastError();
+}
 
 // after the scans: softStart = first softened cell of the bucket
 __global__ void walk_soft_start_kernel(int nb, const int* cellMarks, int* softStart) {
@@ -184,9 +179,10 @@ __global__ void walk_init_kernel(int nb, int* nCell, int* nPart, int* err) {
 }
 
 // work buffer: error flag and two 64-bit totals (256 B), per-bucket counts
-// (2 x (nb + 1) ints, 256-byte aligned), rocPRIM scan storage
+// (2 x (nb + 1) ints, 256-byte aligned), the walk records, rocPRIM scan
+// storage
 struct WalkWork {
-  size_t counts, temp, total;
+  size_t counts, rec, temp, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -200,7 +196,8 @@ static size_t scan_bytes(int nb) {
 }
 
 static int check_args(const ChangaListBuild* L, bool fill) {
-  if (!L || L->nBuckets < 0 || (L->nBuckets > 0 && (!L->bucketNode || !L->first || !L->count ||
+  if (!L || L->nBuckets < 0 || L->numNodes <= 0 ||
+      (L->nBuckets > 0 && (!L->bucketNode || !L->first || !L->count ||
                                                       !L->child || !L->node64 ||
                                                       !L->cellMarks || !L->partMarks ||
                                                       !L->softStart)))
@@ -215,10 +212,11 @@ static int check_args(const ChangaListBuild* L, bool fill) {
   return 0;
 }
 
-static WalkWork work_layout(int nb, size_t scan) {
+static WalkWork work_layout(int nb, int nn, size_t scan) {
   WalkWork w;
   w.counts = 256;
-  w.temp = w.counts + align256(2 * sizeof(int) * ((size_t)nb + 1));
+  w.rec = w.counts + align256(2 * sizeof(int) * ((size_t)nb + 1));
+  w.temp = w.rec + align256(sizeof(WalkRec) * (size_t)nn);
   w.total = w.temp + align256(scan);
   return w;
 }
@@ -230,13 +228,8 @@ static WalkArgs walk_args(const ChangaListBuild* L, int* err) {
   a.nCell = reinterpret_cast<int*>(base + 256);
   a.nPart = a.nCell + (L->nBuckets + 1);
   a.bucketNode = L->bucketNode;
-  a.first = L->first;
-  a.count = L->count;
-  a.child = L->child;
   a.node64 = L->node64;
-  a.kgeom = 2.0 / sqrt(3.0);
-  a.theta = L->theta;
-  a.thetaMono = L->theta * L->theta * L->theta * L->theta;  // the host walk's order
+  a.rec = reinterpret_cast<const WalkRec*>(base + work_layout(L->nBuckets, L->numNodes, 0).rec);
   a.period = L->nReplicas > 0 ? L->period : 0.0;
   a.R = L->nReplicas;
   a.cellMarks = L->cellMarks;
@@ -253,10 +246,10 @@ static WalkArgs walk_args(const ChangaListBuild* L, int* err) {
 
 using namespace cgpu;
 
-extern "C" size_t changaBuildListsWorkBytes(int nBuckets) {
-  if (nBuckets < 0) return 0;
+extern "C" size_t changaBuildListsWorkBytes(int nBuckets, int numNodes) {
+  if (nBuckets < 0 || numNodes <= 0) return 0;
   const size_t s = scan_bytes(nBuckets);
-  return s == 0 ? 0 : work_layout(nBuckets, s).total;
+  return s == 0 ? 0 : work_layout(nBuckets, numNodes, s).total;
 }
 
 extern "C" int changaBuildListsCount(const ChangaListBuild* L, long long* numCells,
@@ -264,15 +257,17 @@ extern "C" int changaBuildListsCount(const ChangaListBuild* L, long long* numCel
   if (int e = check_args(L, false)) return e;
   hipStream_t s = (hipStream_t)stream;
   const int nb = L->nBuckets;
-  if (!L->work || L->workBytes < changaBuildListsWorkBytes(nb))
+  if (!L->work || L->workBytes < changaBuildListsWorkBytes(nb, L->numNodes))
     return fail("changaBuildListsCount: work buffer too small", hipErrorInvalidValue, __FILE__,
                 __LINE__);
   int* err = static_cast<int*>(L->work);
-  const WalkWork W = work_layout(nb, scan_bytes(nb));
+  const WalkWork W = work_layout(nb, L->numNodes, scan_bytes(nb));
   void* temp = static_cast<char*>(L->work) + W.temp;
   const WalkArgs args = walk_args(L, err);
   hipLaunchKernelGGL(walk_init_kernel, dim3(1), dim3(1), 0, s, nb, args.nCell, args.nPart, err);
   CGPU_CHECK(hipGetLastError());
+  CGPU_CHECK(walk_records(L->numNodes, L->node64, L->first, L->count, L->child, L->theta,
+                          const_cast<WalkRec*>(args.rec), s));
   if (nb > 0) {
     hipLaunchKernelGGL(walk_kernel<false>, dim3((nb + kWalkBlock - 1) / kWalkBlock),
                        dim3(kWalkBlock), 0, s, args);
@@ -309,7 +304,7 @@ extern "C" int changaBuildListsFill(const ChangaListBuild* L, void* stream) {
   if (int e = check_args(L, true)) return e;
   hipStream_t s = (hipStream_t)stream;
   const int nb = L->nBuckets;
-  if (!L->work || L->workBytes < changaBuildListsWorkBytes(nb))
+  if (!L->work || L->workBytes < changaBuildListsWorkBytes(nb, L->numNodes))
     return fail("changaBuildListsFill: work buffer too small", hipErrorInvalidValue, __FILE__,
                 __LINE__);
   if (nb > 0) {

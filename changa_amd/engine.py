@@ -288,12 +288,13 @@ class Evaluation:
         self.d_pmarks = torch.empty(nb + 1, **i32)
         self.d_soft_start = torch.empty(max(nb, 1), **i32)
         d_src = torch.empty(max(nb, 1), **i32)
-        nbytes = gpu.lib().changaBuildListsWorkBytes(nb)
+        nbytes = gpu.lib().changaBuildListsWorkBytes(nb, tr["n_nodes"])
         if nbytes == 0:
             raise RuntimeError("changaBuildListsWorkBytes failed")
         work = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         b = gpu.ChangaListBuild()
         b.nBuckets = nb
+        b.numNodes = tr["n_nodes"]
         b.bucketNode = self.d_bucket_node.data_ptr()
         b.first = self.d_first.data_ptr()
         b.count = self.d_count.data_ptr()

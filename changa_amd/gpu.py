@@ -87,7 +87,8 @@ class ChangaAdjointLaunch(ctypes.Structure):
 
 
 class ChangaListBuild(ctypes.Structure):
-    _fields_ = [("nBuckets", c_int), ("bucketNode", c_void_p), ("first", c_void_p),
+    _fields_ = [("nBuckets", c_int), ("numNodes", c_int), ("bucketNode", c_void_p),
+                ("first", c_void_p),
                 ("count", c_void_p), ("child", c_void_p), ("node64", c_void_p),
                 ("theta", ctypes.c_double), ("nReplicas", c_int), ("period", ctypes.c_double),
                 ("cellMarks", c_void_p), ("partMarks", c_void_p), ("softStart", c_void_p),
@@ -193,7 +194,7 @@ def lib():
     L.changaGravityAdjointWorkInts.restype = c_size_t
     L.changaGravityAdjointEwaldWorkDoubles.argtypes = [c_int, c_int]
     L.changaGravityAdjointEwaldWorkDoubles.restype = c_size_t
-    L.changaBuildListsWorkBytes.argtypes = [c_int]
+    L.changaBuildListsWorkBytes.argtypes = [c_int, c_int]
     L.changaBuildListsWorkBytes.restype = c_size_t
     L.changaBuildListsCount.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
     L.changaBuildListsFill.argtypes = [c_void_p, c_void_p]

@@ -264,6 +264,7 @@ int changaGravityAdjoint(const ChangaAdjointLaunch *a, void *stream);
  * ====================================================================== */
 typedef struct {
   int nBuckets;                   /* target buckets */
+  int numNodes;                   /* nodes of the tree */
   const int *bucketNode;          /* device: node of each target bucket */
   const int *first, *count, *child;  /* device: topology, as for changaBuildMoments */
   const double *node64;           /* device: node records of changaBuildMoments */
@@ -275,11 +276,12 @@ typedef struct {
   int *sourceCounts;              /* device out, optional: source particles of each bucket */
   ILCell *cells;                  /* device out (Fill): the cell list */
   ILPart *parts;                  /* device out (Fill): the particle runs */
-  void *work;                     /* device scratch: changaBuildListsWorkBytes(nBuckets) */
+  void *work;                     /* device scratch: changaBuildListsWorkBytes(nBuckets,
+                                     numNodes); Count leaves the walk records there for Fill */
   size_t workBytes;
 } ChangaListBuild;
 
-size_t changaBuildListsWorkBytes(int nBuckets);
+size_t changaBuildListsWorkBytes(int nBuckets, int numNodes);
 int changaBuildListsCount(const ChangaListBuild *b, long long *numCells, long long *numParts,
                           void *stream);
 int changaBuildListsFill(const ChangaListBuild *b, void *stream);
