@@ -323,10 +323,13 @@ int changaBuildTree(ChangaTreeBuild *t, void *stream);
 /* ======================================================================
  * Part 7: fused walk and force
  *
- * Replaces gpuLocalTreeWalk (HostCUDA.h:129-131, HostCUDA.cu:724-942): one
- * persistent launch in which each wavefront walks the tree for a target
+ * Replaces gpuLocalTreeWalk (CudaFunctions.h:10-23, HostCUDA.cu:724-942),
+ * which the reference launches from TreePieceCellListDataTransferLocal
+ * (HostCUDA.cu:210-224) with the CudaRequest fields of HostCUDA.h:80-89.
+ * One persistent launch in which each wavefront walks the tree for a target
  * bucket (the opening tests of changaBuildLists*, so the same interaction
- * set) and evaluates what it accepts at once, with no interaction list in
+ * set; the reference kernel walks per particle instead) and evaluates what
+ * it accepts at once, with no interaction list in
  * device memory beyond a per-wavefront scratch of maxCells cells and maxRuns
  * particle runs. Precision form only (pos64 in both launches). Results are
  * added into cells.vars like changaNodeGravity + changaPartGravity; they
