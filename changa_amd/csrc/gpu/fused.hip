@@ -21,8 +21,6 @@
 
 namespace cgpu {
 
-constexpr int kFusedStack = 2304;     // LDS stack entries (pending nodes): 9 KB, the PC tile's LDS
-constexpr int kFusedNodeBits = 22;    // node index bits of a stack entry; replica code above
 constexpr int kFusedPcTile = 64;
 constexpr int kFusedPpTile = 256;
 constexpr int kFusedRanges = kNumXcd;
@@ -44,7 +42,7 @@ struct FusedArgs {
 
 template <int PCT, int PPT>
 struct FusedLds {
-  static constexpr size_t stack = kFusedStack * sizeof(unsigned);
+  static constexpr size_t stack = kCoopStack * sizeof(unsigned);
   static constexpr size_t pc = (size_t)PCT * kPcRec * sizeof(float);
   static constexpr size_t pp = sizeof(PpLds<PPT>);
   static constexpr size_t red = 4 * kWave * sizeof(double) + kWave * sizeof(float);
@@ -53,10 +51,6 @@ struct FusedLds {
   static constexpr size_t bytes = m1 > m2 ? m1 : m2;
   static constexpr size_t f4 = (bytes + 15) / 16;
 };
-
-__device__ __forceinline__ int excl_count(unsigned long long mask, int lane) {
-  return __popcll(mask & ((1ull << lane) - 1ull));
-}
 
 // Walks bucket b into the wave's scratch: hexadecapole cells at [0, nhex),
 // softened cells at [maxCells - nsoft, maxCells), runs at [0, nruns).
@@ -69,64 +63,19 @@ __device__ bool fused_walk(const FusedArgs& a, int b, int lane, unsigned* stack,
 #pragma unroll
     for (int k = 0; k < 12; ++k) bg[k] = g[k];
   }
-  const double rb = 2.0 * bg[4];
-  nhex = nsoft = nruns = 0;
-  // the root of every replica (the interaction set does not depend on the order)
-  const int W = 2 * a.R + 1, nrep = W * W * W;
-  for (int k = lane; k < nrep; k += kWave) {
-    const int x = k / (W * W) - a.R, y = (k / W) % W - a.R, z = k % W - a.R;
-    const unsigned code = (unsigned)((x + 3) | ((y + 3) << 3) | ((z + 3) << 6));
-    stack[k] = code << kFusedNodeBits;  // node 0
-  }
-  int sp = nrep;
-  bool ok = true;
-  __syncthreads();
-  while (sp > 0) {
-    // pop up to 64 entries, fewer when the stack is nearly full: each popped
-    // entry pushes at most two, so popping at most (capacity - sp) keeps the
-    // stack within capacity; near the limit the walk turns depth first
-    const int n = min(min(kWave, sp), max(1, kFusedStack - sp));
-    unsigned e = 0;
-    if (lane < n) e = stack[sp - n + lane];
-    __syncthreads();
-    sp -= n;
-    int kind = -1, nd = 0, code = 0;  // -1 none, 0 open, 1 hex, 2 soft
-    WalkRec v = {};
-    if (lane < n) {
-      nd = (int)(e & ((1u << kFusedNodeBits) - 1u));
-      code = (int)(e >> kFusedNodeBits);
-      v = a.rec[nd];
-      const int x = (code & 7) - 3, y = ((code >> 3) & 7) - 3, z = ((code >> 6) & 7) - 3;
-      kind = walk_decide(v, bg, rb, x * a.period, y * a.period, z * a.period);
-    }
-    const int oid = code << 22;
-    const bool is_bucket = kind == 0 && v.c0 < 0;
-    const bool is_push = kind == 0 && !is_bucket;
-    const unsigned long long mh = __ballot(kind == 1), ms = __ballot(kind == 2),
-                             mr = __ballot(is_bucket), mp = __ballot(is_push);
-    const int ch = __popcll(mh), cs = __popcll(ms), cr = __popcll(mr), cp = __popcll(mp);
-    if (nhex + nsoft + ch + cs > a.maxCells || nruns + cr > a.maxRuns ||
-        sp + 2 * cp > kFusedStack) {
-      ok = false;
-      break;
-    }
-    if (kind == 1) cells[nhex + excl_count(mh, lane)] = {nd, oid | b};
-    if (kind == 2)
-      cells[a.maxCells - 1 - (nsoft + excl_count(ms, lane))] = {nd, oid | b | (int)0x80000000u};
-    if (is_bucket) runs[nruns + excl_count(mr, lane)] = {v.first, oid, v.count};
-    if (is_push) {
-      const int at = sp + 2 * excl_count(mp, lane);
-      const unsigned hi = (unsigned)code << kFusedNodeBits;
-      stack[at] = hi | (unsigned)v.c1;
-      stack[at + 1] = hi | (unsigned)v.c0;
-    }
-    nhex += ch;
-    nsoft += cs;
-    nruns += cr;
-    sp += 2 * cp;
-    __syncthreads();
-  }
-  __syncthreads();
+  const int maxCells = a.maxCells, maxRuns = a.maxRuns;
+  const bool ok = coop_walk(
+      a.rec, bg, 2.0 * bg[4], a.R, a.period, lane, stack, nhex, nsoft, nruns,
+      [&](int h, int s, int r) { return h + s <= maxCells && r <= maxRuns; },
+      [&](int kind, int nd, int code, const WalkRec& v, int slot) {
+        const int oid = code << 22;
+        if (kind == 1)
+          cells[slot] = {nd, oid | b};
+        else if (kind == 2)
+          cells[maxCells - 1 - slot] = {nd, oid | b | (int)0x80000000u};
+        else
+          runs[slot] = {v.first, oid, v.count};
+      });
   if (!ok && lane == 0) atomicExch(a.err, 1);
   return ok;
 }
@@ -236,11 +185,11 @@ extern "C" int changaFusedGravity(const ChangaFusedLaunch* f, void* stream) {
     return fail("changaFusedGravity: theta, nReplicas (0..3), period, capacities",
                 hipErrorInvalidValue, __FILE__, __LINE__);
   const int W = 2 * f->nReplicas + 1;
-  if (W * W * W > kFusedStack)
+  if (W * W * W > kCoopStack)
     return fail("changaFusedGravity: replicas exceed the walk stack", hipErrorInvalidValue,
                 __FILE__, __LINE__);
-  if (f->numNodes <= 0 || f->numNodes > (1 << kFusedNodeBits))
-    return fail("changaFusedGravity: numNodes in 1..2^22", hipErrorInvalidValue, __FILE__,
+  if (f->numNodes <= 0 || f->numNodes > (1 << kCoopNodeBits))
+    return fail("changaFusedGravity: numNodes in 1..2^23", hipErrorInvalidValue, __FILE__,
                 __LINE__);
   int grid = 0;
   if (int e = fused_grid(&grid)) return e;

@@ -22,7 +22,9 @@
 // contraction off (walk_rec.h), so every opening decision is the host's on
 // the same node geometry. Two passes: count (per bucket: hexadecapole cells,
 // softened cells, runs, source particles; it also writes the walk records),
-// then, after the caller allocated the lists, fill.
+// then, after the caller allocated the lists, fill. (A wavefront-cooperative
+// count walk, one wavefront per bucket, measured slower: a lane per bucket
+// keeps 64 times more independent node loads in flight.)
 #include <rocprim/device/device_scan.hpp>
 
 #include "common.h"
@@ -290,8 +292,8 @@ extern "C" int changaBuildListsCount(const ChangaListBuild* L, long long* numCel
   CGPU_CHECK(hipMemcpyAsync(tot, err + 2, sizeof(tot), hipMemcpyDeviceToHost, s));
   CGPU_CHECK(hipStreamSynchronize(s));
   if (flag)
-    return fail("changaBuildListsCount: tree deeper than the walk stack", hipErrorInvalidValue,
-                __FILE__, __LINE__);
+    return fail("changaBuildListsCount: a walk overflowed its stack (tree too deep)",
+                hipErrorInvalidValue, __FILE__, __LINE__);
   if (tot[0] > 0x7fffffffull || tot[1] > 0x7fffffffull)
     return fail("changaBuildListsCount: list exceeds the int32 marker range",
                 hipErrorInvalidValue, __FILE__, __LINE__);

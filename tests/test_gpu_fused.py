@@ -17,6 +17,8 @@ CASES = {
     "softened": lambda: configs.plummer(n=4096, eps=0.5),
     "small_buckets": lambda: dict(configs.plummer(n=8192), bucket=4),
     "periodic": lambda: configs.zeldovich(ng=16, rms=0.05),
+    "periodic_r2": lambda: dict(configs.zeldovich(ng=16, rms=0.05), n_replicas=2),
+    "periodic_r3": lambda: dict(configs.zeldovich(ng=12, rms=0.05), n_replicas=3),
     "active": lambda: configs.zeldovich(ng=16, rms=0.1, active_frac=0.2),
     "halos": lambda: configs.halos(n=32768),
 }
