@@ -80,9 +80,11 @@ def slice_lists(lists, buckets):
     return out
 
 
-def gpu_build_tree(pos, bucket_size, box_lo=None, box_size=None, device="cuda:0"):
+def gpu_build_tree(pos, bucket_size, box_lo=None, box_size=None, device="cuda:0",
+                   keep_device=False):
     """The tree host.build_tree makes, built on the GPU (changaBuildTree);
-    returns the same dict of host arrays."""
+    returns the same dict of host arrays (with keep_device, also the device
+    arrays and the uploaded positions under "_dev")."""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     n = pos.shape[0]
     if box_lo is None:
@@ -129,6 +131,10 @@ def gpu_build_tree(pos, bucket_size, box_lo=None, box_size=None, device="cuda:0"
               level_offsets=lo_off, bucket_node=d_bnode[:nb].cpu().numpy())
     tr["bucket_start"] = np.ascontiguousarray(tr["node_first"][tr["bucket_node"]])
     tr["bucket_size"] = np.ascontiguousarray(tr["node_count"][tr["bucket_node"]])
+    if keep_device:
+        # the device copies, for an evaluation that gathers on the device
+        tr["_dev"] = dict(pos=d_pos, perm=d_perm, first=d_first[:nn], count=d_count[:nn],
+                          child=d_child[:2 * nn], box_radius=d_boxr[:nn], bucket_node=d_bnode[:nb])
     return tr
 
 
@@ -158,18 +164,32 @@ class Evaluation:
         self.device = torch.device(device)
         periodic = bool(w["periodic"])
         self.L = float(w["L"]) if periodic else 0.0
-        build = host.build_tree if tree == "host" else functools.partial(gpu_build_tree,
-                                                                         device=device)
+        build = host.build_tree if tree == "host" else functools.partial(
+            gpu_build_tree, device=device, keep_device=True)
         if periodic:
             tr = build(w["pos"], w["bucket"], box_lo=np.zeros(3), box_size=self.L)
         else:
             tr = build(w["pos"], w["bucket"])
+        self._dev = tr.pop("_dev", None)
         self.tree = tr
-        perm = tr["perm"]
-        self.pos = np.ascontiguousarray(w["pos"][perm])
-        self.mass = np.ascontiguousarray(w["mass"][perm])
-        self.soft = np.ascontiguousarray(w["soft"][perm])
-        self.active = None if w.get("active") is None else np.ascontiguousarray(w["active"][perm])
+        self._host = {}
+        if self._dev is None:
+            perm = tr["perm"]
+            self._host = dict(pos=np.ascontiguousarray(w["pos"][perm]),
+                              mass=np.ascontiguousarray(w["mass"][perm]),
+                              soft=np.ascontiguousarray(w["soft"][perm]),
+                              active=None if w.get("active") is None
+                              else np.ascontiguousarray(w["active"][perm]))
+        else:
+            # tree order gathered on the device from the device permutation;
+            # host copies are made on first use
+            d_perm = self._dev["perm"].long()
+            self._dev["pos"] = self._dev["pos"][d_perm].contiguous()
+            for key in ("mass", "soft"):
+                self._dev[key] = torch.from_numpy(np.ascontiguousarray(
+                    w[key], dtype=np.float64)).to(self.device)[d_perm].contiguous()
+            self._dev["active"] = None if w.get("active") is None else torch.from_numpy(
+                np.ascontiguousarray(w["active"], dtype=np.bool_)).to(self.device)[d_perm]
         self._lists = None
         if lists == "host":
             self.geo = host.node_geometry(tr, self.pos, self.mass, self.soft)
@@ -178,6 +198,30 @@ class Evaluation:
             self.counts = host.interaction_counts(tr, self._lists, self.active)
         self._targets()
         self._upload()
+
+    def _host_array(self, key):
+        if key not in self._host:
+            d = self._dev[key]
+            self._host[key] = None if d is None else d.cpu().numpy()
+        return self._host[key]
+
+    @property
+    def pos(self):
+        """Positions in tree order (host FP64)."""
+        return self._host_array("pos")
+
+    @property
+    def mass(self):
+        return self._host_array("mass")
+
+    @property
+    def soft(self):
+        return self._host_array("soft")
+
+    @property
+    def active(self):
+        """Rung mask in tree order, or None when every particle is a target."""
+        return self._host_array("active")
 
     @property
     def lists(self):
@@ -190,9 +234,27 @@ class Evaluation:
 
     def _targets(self):
         tr = self.tree
+        self.n = int(tr["n"])
+        if self._dev is not None and self._dev["active"] is not None:
+            # the rung's targets, bucket by bucket, selected on the device
+            nb = tr["n_buckets"]
+            d_size = torch.from_numpy(tr["bucket_size"].astype(np.int64)).to(self.device)
+            owner = torch.repeat_interleave(torch.arange(nb, device=self.device), d_size)
+            tidx = torch.nonzero(self._dev["active"]).flatten()
+            counts = torch.bincount(owner[tidx], minlength=nb)
+            keep = torch.nonzero(counts > 0).flatten()
+            sizes = counts[keep].to(torch.int32)
+            self._dev["tindex"] = tidx.to(torch.int32)
+            self.t_index = self._dev["tindex"].cpu().numpy()
+            self.n_targets = len(self.t_index)
+            self.t_sizes = np.ascontiguousarray(sizes.cpu().numpy())
+            self.t_starts = np.ascontiguousarray(
+                np.concatenate([[0], np.cumsum(self.t_sizes)[:-1]]).astype(np.int32))
+            self.t_keep = keep.cpu().numpy()
+            return
         if self.active is None:
             self.t_index = None
-            self.n_targets = self.pos.shape[0]
+            self.n_targets = self.n
             self.t_starts, self.t_sizes = tr["bucket_start"], tr["bucket_size"]
             self.t_keep = np.arange(tr["n_buckets"])
             if self._lists is not None:
@@ -218,22 +280,38 @@ class Evaluation:
 
     def _upload(self):
         tr = self.tree
-        self.d_parts = self._t(compact_parts(self.pos, self.mass, self.soft))
-        self.d_pos64 = self._t(self.pos)
+        dv = self._dev
+        if dv is None:
+            self.d_parts = self._t(compact_parts(self.pos, self.mass, self.soft))
+            self.d_pos64 = self._t(self.pos)
+            self.d_mass64 = self._t(self.mass)
+            self.d_soft64 = self._t(self.soft)
+        else:
+            self.d_pos64, self.d_mass64, self.d_soft64 = dv["pos"], dv["mass"], dv["soft"]
+            # CompactPartData {mass, soft, x, y, z}, each FP64 value rounded to FP32
+            self.d_parts = torch.cat([self.d_mass64[:, None], self.d_soft64[:, None],
+                                      self.d_pos64], dim=1).to(torch.float32).contiguous()
         if self.t_index is None:
             self.d_targets, self.d_tpos64, self.d_tindex = self.d_parts, self.d_pos64, None
+        elif dv is not None:
+            self.d_tindex = dv["tindex"]
+            ti = self.d_tindex.long()
+            self.d_targets = self.d_parts[ti].contiguous()
+            self.d_tpos64 = self.d_pos64[ti].contiguous()
         else:
             ti = self.t_index
             self.d_targets = self._t(compact_parts(self.pos[ti], self.mass[ti], self.soft[ti]))
             self.d_tpos64 = self._t(self.pos[ti])
             self.d_tindex = self._t(ti)
         self.d_vars = torch.zeros((self.n_targets, 5), dtype=torch.float32, device=self.device)
-        self.d_mass64 = self._t(self.mass)
-        self.d_soft64 = self._t(self.soft)
-        self.d_first = self._t(tr["node_first"])
-        self.d_count = self._t(tr["node_count"])
-        self.d_child = self._t(tr["node_child"])
-        self.d_boxr = self._t(tr["node_box_radius"])
+        if dv is None:
+            self.d_first = self._t(tr["node_first"])
+            self.d_count = self._t(tr["node_count"])
+            self.d_child = self._t(tr["node_child"])
+            self.d_boxr = self._t(tr["node_box_radius"])
+        else:
+            self.d_first, self.d_count = dv["first"], dv["count"]
+            self.d_child, self.d_boxr = dv["child"], dv["box_radius"]
         nn = tr["n_nodes"]
         self.d_moments = torch.zeros((nn, 27), dtype=torch.float32, device=self.device)
         self.d_node64 = torch.zeros((nn, 34), dtype=torch.float64, device=self.device)
@@ -267,6 +345,15 @@ class Evaluation:
             self.d_plist = self._t(cells)
             self.d_pmarks = self._t(marks.astype(np.int32))
 
+    def _bucket_nodes(self):
+        """Device int32: the node of each target bucket."""
+        if self._dev is not None:
+            bn = self._dev["bucket_node"]
+            if len(self.t_keep) == self.tree["n_buckets"]:
+                return bn
+            return bn[torch.from_numpy(self.t_keep).to(self.device)].contiguous()
+        return self._t(self.tree["bucket_node"][self.t_keep].astype(np.int32))
+
     def build_lists(self):
         """Rebuild the GPU-built lists from the current node records (after
         build_moments, e.g. for new positions on the same tree topology)."""
@@ -283,7 +370,7 @@ class Evaluation:
         keep = self.t_keep
         nb = len(keep)
         i32 = dict(dtype=torch.int32, device=self.device)
-        self.d_bucket_node = self._t(tr["bucket_node"][keep].astype(np.int32))
+        self.d_bucket_node = self._bucket_nodes()
         self.d_cmarks = torch.empty(nb + 1, **i32)
         self.d_pmarks = torch.empty(nb + 1, **i32)
         self.d_soft_start = torch.empty(max(nb, 1), **i32)
@@ -399,7 +486,7 @@ class Evaluation:
             c = ewald_real_counts(self.pos[sel], self.root_cm64, self.L, self.w["n_replicas"],
                                   self.w["ew_cut"])
             self.ewald_count_real = int(c.sum())
-        nact = self.pos.shape[0] if self.active is None else int(self.active.sum())
+        nact = self.n if self.active is None else int(self.active.sum())
         return self.ewald_count_real, nact * len(self.ewt64)
 
     # ------------------------------------------------------------------ launches
@@ -499,7 +586,7 @@ class Evaluation:
         self.d_fused_work = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         self._fused_err = torch.zeros(1, dtype=torch.int32).pin_memory()
         if getattr(self, "d_bucket_node", None) is None:
-            self.d_bucket_node = self._t(self.tree["bucket_node"][self.t_keep].astype(np.int32))
+            self.d_bucket_node = self._bucket_nodes()
         f = gpu.ChangaFusedLaunch()
         f.cells = self._launch(True)
         f.parts = self._launch(False)
@@ -550,7 +637,7 @@ class Evaluation:
         ewald: include the Ewald correction in a (default: when periodic)."""
         if not self.precise:
             raise ValueError("the adjoint runs on the precision form (FP64 positions)")
-        n = self.pos.shape[0]
+        n = self.n
         wt = torch.as_tensor(w, dtype=torch.float32).to(self.device).contiguous()
         if tuple(wt.shape) != (self.n_targets, 3):
             raise ValueError(f"w must be [{self.n_targets}, 3]")
@@ -625,7 +712,7 @@ class Evaluation:
         zero for particles that are not targets."""
         v = self.d_vars.cpu().numpy().astype(np.float64)
         if self.t_index is not None:
-            full = np.zeros((self.pos.shape[0], 5))
+            full = np.zeros((self.n, 5))
             full[self.t_index] = v
             v = full
         return v[:, 0:3], v[:, 3], v[:, 4]
