@@ -80,8 +80,12 @@ __device__ bool fused_walk(const FusedArgs& a, int b, int lane, unsigned* stack,
   return ok;
 }
 
+// Four wavefronts per SIMD (128 VGPRs, a few spills outside the force loops)
+// measured 11-12% faster than the three that 141 VGPRs allow: the walk's load
+// latency is hidden behind more resident wavefronts.
 template <int PCT, int PPT>
-__global__ __launch_bounds__(64) void fused_kernel(FusedArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fused_kernel(
+    FusedArgs a) {
   __shared__ float4 lds[FusedLds<PCT, PPT>::f4];
   unsigned* stack = reinterpret_cast<unsigned*>(lds);
   double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(lds);
@@ -108,6 +112,12 @@ __global__ __launch_bounds__(64) void fused_kernel(FusedArgs a) {
     const int b = r0 + k;
     int nhex, nsoft, nruns;
     if (!fused_walk(a, b, lane, stack, cells, runs, nhex, nsoft, nruns)) return;
+    // softened cells (filled from the end of the region) moved behind the
+    // hexadecapole cells, so one cell loop covers both
+    if (nsoft > 0) {
+      for (int i = lane; i < nsoft; i += kWave) cells[nhex + i] = cells[a.maxCells - 1 - i];
+      __syncthreads();
+    }
     const int bstart = a.pc.starts[b], n = a.pc.sizes[b];
     double cx0, cy0, cz0;
     bucket_centre<true>(a.pc.tpos64, bstart, n, cx0, cy0, cz0);
@@ -118,9 +128,8 @@ __global__ __launch_bounds__(64) void fused_kernel(FusedArgs a) {
       Acc64 acc = {0.0, 0.0, 0.0, 0.0, 0.0f};
       // the scratch is re-read per chunk and rewritten per bucket: plain loads
       // keep it in L2
-      pc_cells<true, PCT, false>(a.pc, cbase, cbase + nhex, cx0, cy0, cz0, t, l, lds, acc);
-      pc_cells<true, PCT, false>(a.pc, cbase + a.maxCells - nsoft, cbase + a.maxCells, cx0, cy0,
-                                 cz0, t, l, lds, acc);
+      pc_cells<true, PCT, false>(a.pc, cbase, cbase + nhex + nsoft, cx0, cy0, cz0, t, l, lds,
+                                 acc);
       pp_sources<true, 1, PPT, false>(a.pp, rbase, rbase + nruns, cx0, cy0, cz0, t, l, ps, acc);
       reduce_store(red, redi, lane, l.nt, l.nslot, l.slot, acc.ax, acc.ay, acc.az, acc.pot,
                    acc.idt, a.pc.vars + bstart + t0 + (lane < l.nt ? lane : 0));
