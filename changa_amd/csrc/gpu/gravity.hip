@@ -1,5 +1,6 @@
 // Per-bucket particle-cell and particle-particle kernels and their C-ABI
-// launchers (changaNodeGravity, changaPartGravity, changaZeroVars). The
+// launchers (changaNodeGravity, changaPartGravity, changaZeroVars,
+// changaAddVars). The
 // interaction bodies and the per-chunk loops are in gravity_body.h, shared
 // with the persistent batched kernel of batch.hip.
 #include "gravity_body.h"
@@ -250,6 +251,21 @@ __global__ void zero_vars_kernel(VariablePartData* v, int n) {
   if (i < n) v[i] = VariablePartData{{0.f, 0.f, 0.f}, 0.f, 0.f};
 }
 
+// dst += src with the result array's semantics (+= for a and the potential,
+// max for dtGrav)
+__global__ void add_vars_kernel(VariablePartData* dst, const VariablePartData* src, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  VariablePartData d = dst[i];
+  const VariablePartData s = src[i];
+  d.a.x += s.a.x;
+  d.a.y += s.a.y;
+  d.a.z += s.a.z;
+  d.potential += s.potential;
+  d.dtGrav = fmaxf(d.dtGrav, s.dtGrav);
+  dst[i] = d;
+}
+
 static int check_launch(const ChangaGravityLaunch* a) {
   if (!a || a->numBuckets < 0 || !a->marks || !a->starts || !a->sizes || !a->vars ||
       !a->targets)
@@ -300,6 +316,17 @@ extern "C" int changaPartGravity(const ChangaGravityLaunch* a, void* stream) {
     if (pr) hipLaunchKernelGGL((pp_kernel<true, 1>), g, blk, 0, s, *a);
     else hipLaunchKernelGGL((pp_kernel<false, 1>), g, blk, 0, s, *a);
   }
+  CGPU_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int changaAddVars(VariablePartData* dst, const VariablePartData* src, int n,
+                             void* stream) {
+  if (n <= 0) return 0;
+  if (!dst || !src)
+    return fail("changaAddVars: missing argument", hipErrorInvalidValue, __FILE__, __LINE__);
+  hipLaunchKernelGGL(add_vars_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     dst, src, n);
   CGPU_CHECK(hipGetLastError());
   return 0;
 }

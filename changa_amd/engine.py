@@ -522,16 +522,44 @@ class Evaluation:
         a = self._launch(False)
         gpu.check(gpu.lib().changaPartGravity(ctypes.byref(a), self._sp()), "changaPartGravity")
 
-    def run_ewald(self):
+    def run_ewald(self, vars_=None, stream=None):
+        """The Ewald correction added into vars_ (default: the results) on
+        stream (default: the evaluation's)."""
         if not self.w["periodic"]:
             return
         pos64 = _dp(self.d_tpos64) if self.precise else None
         cm = self.root_cm64.ctypes.data_as(ctypes.c_void_p) if self.precise else None
-        rc = gpu.lib().changaEwald(_dp(self.d_targets), _dp(self.d_vars),
+        sp = self._sp() if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        rc = gpu.lib().changaEwald(_dp(self.d_targets), _dp(self.d_vars if vars_ is None else vars_),
                                    _dp(self.ew_markers) if self.ew_markers is not None else None,
                                    0, self.ewald_ro.n, ctypes.byref(self.ewald_ro), _dp(self.d_ewt),
-                                   pos64, cm, self._sp())
+                                   pos64, cm, sp)
         gpu.check(rc, "changaEwald")
+
+    def _ewald_begin(self):
+        """Start the Ewald correction on a second stream into its own result
+        array, so that it overlaps the multipole build and the tree forces;
+        _ewald_end merges it. It uses only the targets' positions and the
+        root multipole (fixed at set-up), so it does not wait for the
+        moments."""
+        if getattr(self, "_ew_stream", None) is None:
+            self._ew_stream = torch.cuda.Stream(self.device)  # default priority
+            self._ew_vars = torch.empty_like(self.d_vars)
+            self._ew_go = torch.cuda.Event()
+            self._ew_done = torch.cuda.Event()
+        # the previous merge must have read the array before it is cleared
+        self._ew_go.record(self.stream)
+        self._ew_stream.wait_event(self._ew_go)
+        sp = ctypes.c_void_p(self._ew_stream.cuda_stream)
+        gpu.check(gpu.lib().changaZeroVars(_dp(self._ew_vars), self.n_targets, sp),
+                  "changaZeroVars")
+        self.run_ewald(self._ew_vars, self._ew_stream)
+        self._ew_done.record(self._ew_stream)
+
+    def _ewald_end(self):
+        self.stream.wait_event(self._ew_done)
+        gpu.check(gpu.lib().changaAddVars(_dp(self.d_vars), _dp(self._ew_vars), self.n_targets,
+                                          self._sp()), "changaAddVars")
 
     def step(self, moments=True):
         """One force evaluation: multipole build, zero, PC, PP, Ewald."""
@@ -565,13 +593,22 @@ class Evaluation:
         fn = gpu.lib().changaBatchPlanRun if graph else gpu.lib().changaBatchPlanLaunch
         gpu.check(fn(self._batch_plan, self._sp()), "changaBatchPlan")
 
-    def step_batched(self, moments=True):
+    def step_batched(self, moments=True, overlap_ewald=True):
         """One force evaluation in the batched form: multipole build, the
-        plan's graph (zeroing, PC + PP), Ewald."""
+        plan's graph (zeroing, PC + PP), Ewald. With overlap_ewald (periodic
+        boxes) the Ewald pass runs on a second stream beside the first two
+        and is merged after them; the results are bit-identical (the
+        merge adds the same FP32 values the direct pass adds)."""
+        overlap = overlap_ewald and self.w["periodic"]
+        if overlap:
+            self._ewald_begin()
         if moments:
             self.build_moments()
         self.run_batched()
-        self.run_ewald()
+        if overlap:
+            self._ewald_end()
+        else:
+            self.run_ewald()
 
     # ------------------------------------------------------------------ fused form
     def fused_setup(self, max_cells=16384, max_runs=8192):
@@ -619,14 +656,21 @@ class Evaluation:
             raise RuntimeError("changaFusedGravity: a walk exceeded the scratch capacities "
                                "(raise max_cells / max_runs in fused_setup)")
 
-    def step_fused(self, moments=True):
+    def step_fused(self, moments=True, overlap_ewald=True):
         """One force evaluation in the fused form: multipole build, zero, the
-        fused walk and force, Ewald."""
+        fused walk and force, Ewald (overlapped on a second stream as in
+        step_batched)."""
+        overlap = overlap_ewald and self.w["periodic"]
+        if overlap:
+            self._ewald_begin()
         if moments:
             self.build_moments()
         self.zero()
         self.run_fused()
-        self.run_ewald()
+        if overlap:
+            self._ewald_end()
+        else:
+            self.run_ewald()
 
     # ------------------------------------------------------------------ adjoint form
     def adjoint(self, w, ewald=None):

@@ -127,7 +127,7 @@ EXPORTS = [
     "TreePiecePartListDataTransferRemoteResume", "EwaldHostMemorySetup", "EwaldHostMemoryFree",
     "EwaldHost", "changaGpuSetCallbackHandler", "changaGpuSetAbortOnError", "changaGpuLastError",
     "changaGpuReleasePools", "TreePiecePartListDataTransferLocalCompact", "changaNodeGravity",
-    "changaPartGravity", "changaZeroVars", "changaEwald", "changaBuildMoments",
+    "changaPartGravity", "changaZeroVars", "changaAddVars", "changaEwald", "changaBuildMoments",
     "changaBatchPlanCreate", "changaBatchPlanLaunch", "changaBatchPlanRun", "changaBatchPlanDestroy",
     "changaGravityAdjoint", "changaGravityAdjointWorkInts", "changaGravityAdjointEwaldWorkDoubles",
     "changaBuildListsWorkBytes", "changaBuildListsCount", "changaBuildListsFill",
@@ -174,6 +174,8 @@ def lib():
     L.changaPartGravity.restype = c_int
     L.changaZeroVars.argtypes = [P, c_int, P]
     L.changaZeroVars.restype = c_int
+    L.changaAddVars.argtypes = [P, P, c_int, P]
+    L.changaAddVars.restype = c_int
     L.changaEwald.argtypes = [P, P, P, c_int, c_int, ctypes.POINTER(EwaldReadOnlyData), P, P, P, P]
     L.changaEwald.restype = c_int
     L.changaBuildMoments.argtypes = [c_int, c_int, P, P, P, P, P, P, P, P, P, P, P, P]

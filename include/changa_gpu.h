@@ -137,6 +137,9 @@ typedef struct {
 int changaNodeGravity(const ChangaGravityLaunch *a, void *stream);
 int changaPartGravity(const ChangaGravityLaunch *a, void *stream);
 int changaZeroVars(VariablePartData *vars, int n, void *stream);
+/* dst += src (a and potential; dtGrav = max): merges a result array filled on
+   another stream, e.g. an Ewald pass overlapped with the tree forces. */
+int changaAddVars(VariablePartData *dst, const VariablePartData *src, int n, void *stream);
 
 /* Ewald on device-resident data. markers: n active particle indices into
    parts (device memory; NULL: particles first..first+n-1). ewt: the

@@ -138,6 +138,27 @@ def test_gpu_batched_periodic_step(device):
     ev.free_batch()
 
 
+def test_gpu_ewald_overlap_is_bit_identical(device):
+    """The Ewald pass overlapped on a second stream (its own result array,
+    merged by changaAddVars after the tree forces) gives the same bits as
+    the Ewald pass enqueued after the tree forces, for the batched and the
+    fused steps, over repeated steps (the second stream's array is cleared
+    only after the previous merge read it)."""
+    import torch
+    ev = engine.Evaluation(configs.zeldovich(ng=24, rms=0.05, active_frac=0.3), precise=True,
+                           device=device, lists="gpu")
+    ev.batch_plan()
+    for step in (ev.step_batched, ev.step_fused):
+        step(overlap_ewald=False)
+        torch.cuda.synchronize()
+        ref = ev.d_vars.cpu().numpy().copy()
+        for _ in range(3):
+            step(overlap_ewald=True)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(ev.d_vars.cpu().numpy(), ref)
+    ev.free_batch()
+
+
 def test_gpu_batched_rejects_bad_arguments(device):
     """Rejected on the host with an error code (no launch happens)."""
     ev = engine.Evaluation(configs.plummer(n=4096), device=device)
