@@ -87,13 +87,14 @@ def gpu_build_tree(pos, bucket_size, box_lo=None, box_size=None, device="cuda:0"
     arrays and the uploaded positions under "_dev")."""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     n = pos.shape[0]
-    if box_lo is None:
-        lo, hi = pos.min(axis=0), pos.max(axis=0)
-        box_size = float((hi - lo).max()) * (1 + 1e-12) + 1e-300
-        box_lo = lo
     dev = torch.device(device)
     i32 = dict(dtype=torch.int32, device=dev)
     d_pos = torch.from_numpy(pos).to(dev)
+    if box_lo is None:
+        # the host builder's box; min and max are exact, so the device gives the same values
+        lo, hi = d_pos.amin(dim=0).cpu().numpy(), d_pos.amax(dim=0).cpu().numpy()
+        box_size = float((hi - lo).max()) * (1 + 1e-12) + 1e-300
+        box_lo = lo
     nn_cap = max(2 * n - 1, 1)
     d_perm = torch.empty(n, **i32)
     d_first, d_count = torch.empty(nn_cap, **i32), torch.empty(nn_cap, **i32)
