@@ -153,8 +153,9 @@ class Evaluation:
 
     def __init__(self, w, precise=False, pp_compact=True, n_threads=0, device="cuda:0",
                  lists="host", tree="host"):
-        if lists not in ("host", "gpu") or tree not in ("host", "gpu"):
-            raise ValueError("lists and tree: 'host' or 'gpu'")
+        if lists not in ("host", "gpu", "none") or tree not in ("host", "gpu"):
+            raise ValueError("lists: 'host', 'gpu' or 'none' (fused form only); tree: 'host' or "
+                             "'gpu'")
         if lists == "gpu" and not pp_compact:
             raise ValueError("GPU-built lists are ILPart runs (pp_compact=True)")
         self.w = w
@@ -230,6 +231,8 @@ class Evaluation:
         host walk's layout); with GPU-built lists, downloaded on first use
         (buckets without targets have empty lists)."""
         if self._lists is None:
+            if self.list_source == "none":
+                raise ValueError("this evaluation has no interaction lists (lists='none')")
             self._lists = self._download_lists()
         return self._lists
 
@@ -323,6 +326,11 @@ class Evaluation:
         if self.list_source == "gpu":
             self.build_moments()
             self._gpu_lists()
+        elif self.list_source == "none":
+            # no interaction lists: the fused walk-and-force form only
+            self.build_moments()
+            self.counts = None
+            self.t_lists = None
         else:
             self._upload_host_lists()
             self.build_moments()
@@ -497,8 +505,9 @@ class Evaluation:
         a.vars = self.d_vars.data_ptr()
         a.moments = self.d_moments.data_ptr()
         a.sources = self.d_parts.data_ptr()
-        a.list = (self.d_cells if pc else self.d_plist).data_ptr()
-        a.marks = (self.d_cmarks if pc else self.d_pmarks).data_ptr()
+        if self.list_source != "none":
+            a.list = (self.d_cells if pc else self.d_plist).data_ptr()
+            a.marks = (self.d_cmarks if pc else self.d_pmarks).data_ptr()
         a.starts = self.d_starts.data_ptr()
         a.sizes = self.d_sizes.data_ptr()
         a.numBuckets = len(self.t_sizes)
@@ -515,11 +524,18 @@ class Evaluation:
         gpu.check(gpu.lib().changaZeroVars(_dp(self.d_vars), self.n_targets, self._sp()),
                   "changaZeroVars")
 
+    def _need_lists(self):
+        if self.list_source == "none":
+            raise ValueError("this evaluation has no interaction lists (lists='none'): "
+                             "use the fused form")
+
     def run_pc(self):
+        self._need_lists()
         a = self._launch(True)
         gpu.check(gpu.lib().changaNodeGravity(ctypes.byref(a), self._sp()), "changaNodeGravity")
 
     def run_pp(self):
+        self._need_lists()
         a = self._launch(False)
         gpu.check(gpu.lib().changaPartGravity(ctypes.byref(a), self._sp()), "changaPartGravity")
 
@@ -576,6 +592,7 @@ class Evaluation:
         """Make (replacing any previous) the batched-form plan over this
         evaluation's lists; options left at 0 are autotuned. Returns the
         chosen ChangaBatchOptions."""
+        self._need_lists()
         self.free_batch()
         o = gpu.ChangaBatchOptions(pc_tile, pp_tile, lists_per_grab, order, int(zero_vars), 0.0)
         self._batch_launches = (self._launch(True), self._launch(False))
@@ -682,6 +699,7 @@ class Evaluation:
         ewald: include the Ewald correction in a (default: when periodic)."""
         if not self.precise:
             raise ValueError("the adjoint runs on the precision form (FP64 positions)")
+        self._need_lists()
         n = self.n
         wt = torch.as_tensor(w, dtype=torch.float32).to(self.device).contiguous()
         if tuple(wt.shape) != (self.n_targets, 3):

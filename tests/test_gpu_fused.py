@@ -87,3 +87,21 @@ def test_gpu_fused_capacity_error(device):
     torch.cuda.synchronize()
     with pytest.raises(RuntimeError, match="scratch capacities"):
         ev.fused_check()
+
+
+def test_gpu_fused_without_lists(device):
+    """An evaluation built with lists='none' (tree and moments only, as the
+    forward passes of a kick-drift-kick step use it) gives the fused step's
+    results bit for bit, and has no lists to hand out."""
+    import torch
+    w = configs.zeldovich(ng=16, rms=0.05, active_frac=0.3)
+    ref = engine.Evaluation(w, precise=True, device=device, lists="gpu", tree="gpu")
+    ref.step_fused()
+    torch.cuda.synchronize()
+    ev = engine.Evaluation(w, precise=True, device=device, lists="none", tree="gpu")
+    ev.step_fused()
+    torch.cuda.synchronize()
+    ev.fused_check()
+    np.testing.assert_array_equal(ev.d_vars.cpu().numpy(), ref.d_vars.cpu().numpy())
+    with pytest.raises(ValueError, match="no interaction lists"):
+        ev.lists
