@@ -1498,6 +1498,8 @@ extern "C" size_t changaGravityAdjointEwaldWorkDoubles(int nTargets, int nH) {
 }
 
 extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) {
+  if (a && a->cells.numBuckets == 0 && a->grad && a->nParticles > 0)
+    return zero(a->grad, 4 * (size_t)a->nParticles, (hipStream_t)stream);  // no targets: L = 0
   if (!a || !a->w || !a->grad || !a->nodeWork || !a->listWork || !a->levelOffsets || !a->first ||
       !a->count || !a->child || !a->node64 || !a->pos64 || !a->mass64 || a->nNodes <= 0 ||
       a->nDepths <= 0 || a->nParticles <= 0 || a->numCellEntries < 0 || a->numPartEntries < 0)

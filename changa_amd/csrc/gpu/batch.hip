@@ -240,8 +240,10 @@ extern "C" int changaBatchPlanCreate(const ChangaGravityLaunch* cells,
   *plan = nullptr;
   const ChangaGravityLaunch& pc = *cells;
   const ChangaGravityLaunch& pp = *parts;
-  if (pc.numBuckets < 0 || !pc.marks || !pc.starts || !pc.sizes || !pc.vars || !pc.targets ||
-      !pc.moments || !pc.list || !pp.marks || !pp.list || !pp.sources)
+  // a plan over no buckets (an empty rung) needs no arrays and runs nothing
+  if (pc.numBuckets < 0 ||
+      (pc.numBuckets > 0 && (!pc.marks || !pc.starts || !pc.sizes || !pc.vars || !pc.targets ||
+                             !pc.moments || !pc.list || !pp.marks || !pp.list || !pp.sources)))
     return fail("changaBatchPlanCreate: missing array", hipErrorInvalidValue, __FILE__, __LINE__);
   if (!same_buckets(pc, pp))
     return fail("changaBatchPlanCreate: cell and particle launches differ in their buckets",

@@ -180,6 +180,7 @@ extern "C" size_t changaFusedGravityWorkBytes(int numNodes, int maxCells, int ma
 }
 
 extern "C" int changaFusedGravity(const ChangaFusedLaunch* f, void* stream) {
+  if (f && f->cells.numBuckets == 0) return 0;  // an empty rung
   if (!f || !f->bucketNode || !f->first || !f->count || !f->child || !f->node64 || !f->work)
     return fail("changaFusedGravity: missing argument", hipErrorInvalidValue, __FILE__, __LINE__);
   const ChangaGravityLaunch& pc = f->cells;

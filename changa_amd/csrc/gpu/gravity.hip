@@ -266,9 +266,10 @@ __global__ void add_vars_kernel(VariablePartData* dst, const VariablePartData* s
   dst[i] = d;
 }
 
+// A launch over no buckets (an empty rung) needs no arrays.
 static int check_launch(const ChangaGravityLaunch* a) {
-  if (!a || a->numBuckets < 0 || !a->marks || !a->starts || !a->sizes || !a->vars ||
-      !a->targets)
+  if (!a || a->numBuckets < 0 ||
+      (a->numBuckets > 0 && (!a->marks || !a->starts || !a->sizes || !a->vars || !a->targets)))
     return fail("gravity launch: missing argument", hipErrorInvalidValue, __FILE__, __LINE__);
   return 0;
 }

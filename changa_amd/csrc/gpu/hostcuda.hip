@@ -322,6 +322,7 @@ void EwaldHost(CompactPartData* d_localParts, VariablePartData* d_localVars, Ewa
 int changaEwald(const CompactPartData* parts, VariablePartData* vars, const int* markers,
                 int first, int n, const EwaldReadOnlyData* ro, const EwtData* ewt,
                 const double* pos64, const double* rootCm64, void* stream) {
+  if (n == 0) return 0;  // no active particles
   if (!ro || !ewt || !vars || (!parts && !pos64))
     return fail("changaEwald: missing argument", hipErrorInvalidValue, __FILE__, __LINE__);
   return launch_ewald(parts, vars, markers, first, n, ro, ewt, pos64, rootCm64,

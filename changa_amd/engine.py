@@ -253,7 +253,7 @@ class Evaluation:
             self.n_targets = len(self.t_index)
             self.t_sizes = np.ascontiguousarray(sizes.cpu().numpy())
             self.t_starts = np.ascontiguousarray(
-                np.concatenate([[0], np.cumsum(self.t_sizes)[:-1]]).astype(np.int32))
+                (np.cumsum(self.t_sizes) - self.t_sizes).astype(np.int32))
             self.t_keep = keep.cpu().numpy()
             return
         if self.active is None:
@@ -273,7 +273,7 @@ class Evaluation:
         self.n_targets = len(tidx)
         self.t_sizes = np.ascontiguousarray(counts[keep].astype(np.int32))
         self.t_starts = np.ascontiguousarray(
-            np.concatenate([[0], np.cumsum(self.t_sizes)[:-1]]).astype(np.int32))
+            (np.cumsum(self.t_sizes) - self.t_sizes).astype(np.int32))
         self.t_keep = keep
         if self._lists is not None:
             self.t_lists = slice_lists(self._lists, keep)

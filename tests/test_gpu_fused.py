@@ -105,3 +105,26 @@ def test_gpu_fused_without_lists(device):
     np.testing.assert_array_equal(ev.d_vars.cpu().numpy(), ref.d_vars.cpu().numpy())
     with pytest.raises(ValueError, match="no interaction lists"):
         ev.lists
+
+
+def test_gpu_empty_rung(device):
+    """A rung with no active particle: every path (per bucket, batched,
+    fused, the Ewald pass, the adjoint) runs over no buckets and leaves the
+    results and the gradient at zero."""
+    import torch
+    w = configs.zeldovich(ng=16, rms=0.05, active_frac=0.3)
+    w = dict(w, active=np.zeros_like(w["active"]))
+    ev = engine.Evaluation(w, precise=True, device=device, lists="gpu", tree="gpu")
+    assert ev.n_targets == 0 and len(ev.t_sizes) == 0 and len(ev.t_starts) == 0
+    ev.step()
+    ev.batch_plan()
+    ev.step_batched()
+    ev.step_fused()
+    torch.cuda.synchronize()
+    ev.fused_check()
+    acc, pot, dtg = ev.results()
+    assert not acc.any() and not pot.any() and not dtg.any()
+    g = ev.adjoint(np.zeros((0, 3)))
+    torch.cuda.synchronize()
+    assert not g.cpu().numpy().any()
+    ev.free_batch()
