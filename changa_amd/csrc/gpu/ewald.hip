@@ -161,7 +161,8 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
   // k space. On the cube |h_i| <= kEwCube: e^(2 pi i h.u) (u = d / L) as the
   // product of per-axis powers of e^(2 pi i u_x) etc., which are formed once
   // per particle; the plane hx is advanced by one complex multiplication.
-  // Sums in FP32 within a plane, in FP64 across planes.
+  // Compensated FP32 sums within a plane (the residual c is subtracted when
+  // the plane's sum is carried into FP64), FP64 across planes.
   const float iL = (float)(1.0 / p.L);
   const float ux = fx * iL, uy = fy * iL, uz = fz * iL;
   {
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
     float2 ex = cpow_neg(e1x);  // e^(-2 pi i kEwCube u_x)
     const float kf = (float)(2.0 * M_PI / p.L);
     for (int hx = -kEwCube; hx <= kEwCube; ++hx) {
-      float sp = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+      KahanF sp, sx, sy, sz;
       const float2* cw = cube + (hx + kEwCube) * kEwCubeW * kEwCubeW;
 #pragma unroll
       for (int hy = -kEwCube; hy <= kEwCube; ++hy) {
@@ -183,18 +184,18 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
           const float2 wgt = cw[(hy + kEwCube) * kEwCubeW + hz + kEwCube];
           if (wgt.x != 0.0f || wgt.y != 0.0f) {
             const float2 e = cmul(exy, ez[hz + kEwCube]);
-            sp += wgt.x * e.x + wgt.y * e.y;
+            sp.add(wgt.x * e.x + wgt.y * e.y);
             const float t = wgt.x * e.y - wgt.y * e.x;
-            sx += (float)hx * t;
-            sy += (float)hy * t;
-            sz += (float)hz * t;
+            sx.add((float)hx * t);
+            sy.add((float)hy * t);
+            sz.add((float)hz * t);
           }
         }
       }
-      pot += (double)sp;
-      ax += (double)(kf * sx);
-      ay += (double)(kf * sy);
-      az += (double)(kf * sz);
+      pot += (double)sp.s - (double)sp.c;
+      ax += (double)kf * ((double)sx.s - (double)sx.c);
+      ay += (double)kf * ((double)sy.s - (double)sy.c);
+      az += (double)kf * ((double)sz.s - (double)sz.c);
       ex = cmul(ex, e1x);
     }
   }

@@ -129,6 +129,18 @@ constexpr int kEwCube = 3;  // k-space table entries with |h_i| <= 3 use the pro
 constexpr int kEwCubeW = 2 * kEwCube + 1;
 constexpr int kEwCubeN = kEwCubeW * kEwCubeW * kEwCubeW;
 
+// Compensated (Kahan) FP32 sum: the rounding error of each addition is
+// carried in c and fed back into the next.
+struct KahanF {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
