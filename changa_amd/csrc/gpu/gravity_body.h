@@ -25,8 +25,8 @@
 namespace cgpu {
 
 constexpr int kPcTile = 64;   // cells per staged tile of the plain kernel
-constexpr int kPcRec = 36;    // floats per staged cell record (9 x float4; 36 keeps the
-                              // records read together 4 LDS banks apart)
+constexpr int kPcRec = 36;    // floats per staged cell record (9 x float4; stored whole,
+                              // 36 keeps records read together 4 LDS banks apart)
 constexpr int kPpTile = 256;  // source particles per staged tile
 
 // Staged cell record. Moments of order n are multiplied by n c_n radius^n
@@ -95,13 +95,17 @@ __device__ __forceinline__ void pp_interact(float rx, float ry, float rz, float 
 //   phi -= dir (m + dir^2 (P2/2 + dir (P3/3 + dir P4/4)))
 //   a   += dir^4 (V2 + dir (V3 + dir V4)) - dir^2 S n,
 //   S = m + dir^2 (5/2 P2 + dir (7/3 P3 + dir 9/4 P4)).
+// The record's float4 k is rec[k * STRIDE] (STRIDE 1: records stored whole;
+// STRIDE = tile width: component planes).
+template <int STRIDE = 1>
 __device__ __forceinline__ void pc_interact(float rx, float ry, float rz, const float4* rec,
                                             float mt, Acc32& A) {
   float r2 = rx * rx + ry * ry + rz * rz;
   if (r2 == 0.0f) return;
   float dir = rsq(r2);
-  const float4 c0 = rec[0], c1 = rec[1], c2 = rec[2], c3 = rec[3];
-  const float4 c4 = rec[4], c5 = rec[5], c6 = rec[6], c7 = rec[7];
+  const float4 c0 = rec[0], c1 = rec[STRIDE], c2 = rec[2 * STRIDE], c3 = rec[3 * STRIDE];
+  const float4 c4 = rec[4 * STRIDE], c5 = rec[5 * STRIDE], c6 = rec[6 * STRIDE],
+               c7 = rec[7 * STRIDE];
   const float m = c0.w;
   const float qxx = c1.x, qxy = c1.y, qxz = c1.z, qyy = c1.w, qyz = c2.x, qzz = c2.y;
   const float qxxx = c2.z, qxyy = c2.w, qxxy = c3.x, qyyy = c3.y, qxxz = c3.z, qyyz = c3.w,
@@ -235,8 +239,9 @@ __device__ __forceinline__ void load_cell(const ChangaGravityLaunch& a, const IL
   }
 }
 
-// wire record -> staged record (in registers), then to LDS
-template <bool PRECISE>
+// wire record -> staged record (in registers), then to LDS (float4 k at
+// dst[k * STRIDE])
+template <bool PRECISE, int STRIDE = 1>
 __device__ __forceinline__ void stage_cell(const ChangaGravityLaunch& a, const WireCell& w, float L,
                                            double cx0, double cy0, double cz0, float4* dst) {
   // wire: 0 radius, 1 soft, 2 mass, 3-5 cm, 6-10 Q2 (xx xy xz yy yz),
@@ -276,7 +281,7 @@ __device__ __forceinline__ void stage_cell(const ChangaGravityLaunch& a, const W
   }
 #pragma unroll
   for (int k = 0; k < kPcRec / 4; ++k)
-    dst[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+    dst[k * STRIDE] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
 }
 
 
@@ -342,9 +347,12 @@ __device__ __forceinline__ Target load_target(const ChangaGravityLaunch& a, int 
 
 // Cells [start, end) of a.list onto the chunk's targets. Tiles of TILE <= 64
 // cells: lane c stages cell c of the tile into rec4 (TILE * kPcRec floats of
-// LDS); the next tile's wire records are in flight while this one is
-// evaluated. A tile without softened cells (ballot) runs the hexadecapole
-// loop with no per-cell test.
+// LDS) as component planes: float4 k of cell c at rec4[k * TILE + c], so
+// lanes reading or writing different cells touch consecutive 16-byte words
+// (no bank conflicts however many distinct cells a read instruction
+// touches: one per lane when a bucket has a single target). The next tile's
+// wire records are in flight while this one is evaluated. A tile without
+// softened cells (ballot) runs the hexadecapole loop with no per-cell test.
 template <bool PRECISE, int TILE, bool STREAM = true>
 __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start, int end,
                                          double cx0, double cy0, double cz0, const Target& t,
@@ -360,7 +368,7 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
     const int nc = min(TILE, end - base);
     const bool mine = l.lane < nc;
     const bool soft_tile = __ballot(mine && (w.oid & CHANGA_ILCELL_SOFTENED)) != 0ull;
-    if (mine) stage_cell<PRECISE>(a, w, L, cx0, cy0, cz0, rec4 + l.lane * (kPcRec / 4));
+    if (mine) stage_cell<PRECISE, TILE>(a, w, L, cx0, cy0, cz0, rec4 + l.lane);
     __syncthreads();
     // issued after the barrier, whose fence would otherwise wait for them
     load_cell<PRECISE, STREAM>(a, cells, base + TILE + l.lane, stager ? end : 0, w);
@@ -368,18 +376,18 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
       Acc32 A = {0.f, 0.f, 0.f, 0.f, acc.idt};
       if (!soft_tile) {
         for (int c = l.slot; c < nc; c += l.nslot) {
-          const float4* r4 = rec4 + c * (kPcRec / 4);
+          const float4* r4 = rec4 + c;
           const float4 h = r4[0];
-          pc_interact(t.x - h.x, t.y - h.y, t.z - h.z, r4, t.m, A);
+          pc_interact<TILE>(t.x - h.x, t.y - h.y, t.z - h.z, r4, t.m, A);
         }
       } else {
         for (int c = l.slot; c < nc; c += l.nslot) {
-          const float4* r4 = rec4 + c * (kPcRec / 4);
+          const float4* r4 = rec4 + c;
           const float4 h = r4[0];
           if (h.w < 0.0f)  // softened cell: spline monopole at the cm
-            pp_interact(h.x - t.x, h.y - t.y, h.z - t.z, -h.w, r4[1].x, t.m, t.soft, A);
+            pp_interact(h.x - t.x, h.y - t.y, h.z - t.z, -h.w, r4[TILE].x, t.m, t.soft, A);
           else
-            pc_interact(t.x - h.x, t.y - h.y, t.z - h.z, r4, t.m, A);
+            pc_interact<TILE>(t.x - h.x, t.y - h.y, t.z - h.z, r4, t.m, A);
         }
       }
       acc.ax += A.ax;
