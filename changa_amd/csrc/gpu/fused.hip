@@ -234,8 +234,8 @@ extern "C" int changaFusedGravity(const ChangaFusedLaunch* f, void* stream) {
   hipLaunchKernelGGL(fused_reset_kernel, dim3(1), dim3(kFusedRanges * kFusedCounterStride + 1), 0,
                      s, a.counters);
   CGPU_CHECK(hipGetLastError());
-  CGPU_CHECK(walk_records(f->numNodes, f->node64, f->first, f->count, f->child, f->theta,
-                          const_cast<WalkRec*>(a.rec), s));
+  CGPU_CHECK(walk_records(f->numNodes, f->node64, f->first, f->count, f->child, nullptr,
+                          f->theta, const_cast<WalkRec*>(a.rec), s));
   hipLaunchKernelGGL((fused_kernel<kFusedPcTile, kFusedPpTile>), dim3(grid), dim3(kWave), 0, s, a);
   CGPU_CHECK(hipGetLastError());
   if (f->errorFlag) {

@@ -15,13 +15,14 @@
 // softened cells, then particle runs (the layout ListCompute::stateReady
 // serializes, Compute.cpp:1735-1741, 1806).
 //
-// One lane per target bucket walks depth first with its own stack (in LDS), in the
+// One lane per target bucket walks depth first, in the
 // host walk's order (replicas from (-R, -R, -R) to (R, R, R), z fastest;
 // the first child first), so the lists are the host walk's entry for entry.
 // The nodes are read as 64-byte walk records and tested in FP64 with
 // contraction off (walk_rec.h), so every opening decision is the host's on
 // the same node geometry. Two passes: count (per bucket: hexadecapole cells,
-// softened cells, runs, source particles; it also writes the walk records),
+// softened cells, runs, source particles; it also writes the walk records and
+// the escape nodes that let a lane walk depth first without a stack),
 // then, after the caller allocated the lists, fill. (A wavefront-cooperative
 // count walk, one wavefront per bucket, measured slower: a lane per bucket
 // keeps 64 times more independent node loads in flight.)
@@ -32,8 +33,8 @@
 
 namespace cgpu {
 
-constexpr int kWalkStack = 72;  // depth-first stack: tree depth + 1 (Morton trees: <= 64)
 constexpr int kWalkBlock = 64;
+constexpr int kEscPasses = 8;  // pointer-jumping passes (even): chains up to 2^8 > the tree depth
 
 struct WalkArgs {
   int nb;
@@ -41,7 +42,8 @@ struct WalkArgs {
   int* nPart;
   const int* bucketNode;
   const double* node64;  // the target buckets' geometry
-  const WalkRec* rec;    // walk records of every node
+  const WalkRec* rec;    // walk records of every node (link: the escape node)
+  const int* esc;        // escape node of every node (depth-first successor of its subtree)
   double period;
   int R;
   int* cellMarks;
@@ -74,46 +76,37 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
     spos = a.softStart[b];
     ppos = a.partMarks[b];
   }
-  // the lanes' stacks in LDS, entry k of lane l at [k][l] (a private array
-  // would live in scratch memory)
-  __shared__ int s_stack[kWalkStack * kWalkBlock];
-  int* stack = s_stack + threadIdx.x;
-  bool overflow = false;
-  for (int x = -a.R; x <= a.R && !overflow; ++x)
-    for (int y = -a.R; y <= a.R && !overflow; ++y)
-      for (int z = -a.R; z <= a.R && !overflow; ++z) {
+  // depth first without a stack: an opened internal node continues with its
+  // first child; anything else continues with its escape node (the next node
+  // of the depth-first order after its subtree; -1 after the last)
+  for (int x = -a.R; x <= a.R; ++x)
+    for (int y = -a.R; y <= a.R; ++y)
+      for (int z = -a.R; z <= a.R; ++z) {
         const int oid = ((x + 3) | ((y + 3) << 3) | ((z + 3) << 6)) << 22;
         const double ox = x * a.period, oy = y * a.period, oz = z * a.period;
-        int sp = 0;
-        stack[kWalkBlock * sp++] = 0;
-        while (sp > 0) {
-          const int nd = stack[kWalkBlock * --sp];
+        int nd = 0;
+        while (nd >= 0) {
           const WalkRec v = a.rec[nd];
+          const int next = v.link;  // the escape node
           const int kind = walk_decide(v, bg, rb, ox, oy, oz);
           if (kind == 2) {
             if (FILL) a.cells[spos + nsoft] = {nd, oid | b | (int)0x80000000u};
             ++nsoft;
+            nd = next;
           } else if (kind == 1) {
             if (FILL) a.cells[hpos + nhex] = {nd, oid | b};
             ++nhex;
+            nd = next;
           } else if (v.c0 < 0) {
             if (FILL) a.parts[ppos + nruns] = {v.first, oid, v.count};
             ++nruns;
             nsrc += v.count;
+            nd = next;
           } else {
-            if (sp + 2 > kWalkStack) {
-              overflow = true;
-              break;
-            }
-            stack[kWalkBlock * sp++] = v.c1;
-            stack[kWalkBlock * sp++] = v.c0;
+            nd = v.c0;
           }
         }
       }
-  if (overflow) {
-    atomicExch(a.err, 1);
-    return;
-  }
   if (!FILL) {
     a.nCell[b] = nhex + nsoft;
     a.nPart[b] = nruns;
@@ -129,8 +122,8 @@ __global__ __launch_bounds__(kWalkBlock) void walk_kernel(WalkArgs a) {
 #pragma clang fp contract(off)
 
 __global__ void walk_records_kernel(int nn, const double* node64, const int* first,
-                                    const int* count, const int* child, double kgeom,
-                                    double theta, double thetaMono, WalkRec* rec) {
+                                    const int* count, const int* child, const int* esc,
+                                    double kgeom, double theta, double thetaMono, WalkRec* rec) {
   const int nd = blockIdx.x * blockDim.x + threadIdx.x;
   if (nd >= nn) return;
   const double* g = node64 + (size_t)nd * 34;
@@ -146,20 +139,80 @@ __global__ void walk_records_kernel(int nn, const double* node64, const int* fir
   v.rm = kgeom * r / thetaMono;
   v.count = count[nd];
   v.c0 = child[2 * nd];
-  v.c1 = child[2 * nd + 1];
+  v.link = esc ? esc[nd] : child[2 * nd + 1];
   v.first = first[nd];
   rec[nd] = v;
 }
 
 #pragma clang fp contract(on)
 
+// Escape nodes. A first child escapes to its sibling; a second child to its
+// parent's escape; the root to -1. Chains of second children are resolved
+// by pointer jumping: jump[n] = the ancestor whose escape n inherits (-1 once
+// esc[n] is final), halving every chain per pass.
+__global__ void esc_init_kernel(int nn, const int* child, int* esc, int* jump) {
+  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= nn) return;
+  if (nd == 0) {
+    esc[0] = -1;
+    jump[0] = -1;
+  }
+  const int c0 = child[2 * nd], c1 = child[2 * nd + 1];
+  if (c0 >= 0) {
+    esc[c0] = c1;
+    jump[c0] = -1;
+    jump[c1] = nd;
+  }
+}
+
+__global__ void esc_jump_kernel(int nn, const int* esc_in, const int* jump_in, int* esc_out,
+                                int* jump_out) {
+  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= nn) return;
+  const int q = jump_in[nd];
+  if (q < 0) {
+    esc_out[nd] = esc_in[nd];
+    jump_out[nd] = -1;
+  } else if (jump_in[q] < 0) {
+    esc_out[nd] = esc_in[q];
+    jump_out[nd] = -1;
+  } else {
+    esc_out[nd] = esc_in[nd];
+    jump_out[nd] = jump_in[q];
+  }
+}
+
+__global__ void esc_check_kernel(int nn, const int* jump, int* err) {
+  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd < nn && jump[nd] >= 0) atomicExch(err, 1);
+}
+
+// esc (and its pass buffers esc2, jump, jump2: nn ints each) from the topology
+static int walk_escapes(int nn, const int* child, int* esc, int* esc2, int* jump, int* jump2,
+                        int* err, hipStream_t s) {
+  const dim3 grid((nn + 255) / 256), block(256);
+  hipLaunchKernelGGL(esc_init_kernel, grid, block, 0, s, nn, child, esc, jump);
+  CGPU_CHECK(hipGetLastError());
+  for (int k = 0; k < kEscPasses; ++k) {
+    const bool even = (k & 1) == 0;
+    hipLaunchKernelGGL(esc_jump_kernel, grid, block, 0, s, nn, even ? esc : esc2,
+                       even ? jump : jump2, even ? esc2 : esc, even ? jump2 : jump);
+    CGPU_CHECK(hipGetLastError());
+  }
+  static_assert(kEscPasses % 2 == 0, "the last pass writes esc / jump");
+  hipLaunchKernelGGL(esc_check_kernel, grid, block, 0, s, nn, jump, err);
+  CGPU_CHECK(hipGetLastError());
+  return 0;
+}
+
 hipError_t walk_records(int nn, const double* node64, const int* first, const int* count,
-                        const int* child, double theta, WalkRec* rec, hipStream_t s) {
+                        const int* child, const int* esc, double theta, WalkRec* rec,
+                        hipStream_t s) {
   if (nn <= 0) return hipSuccess;
   // theta^4 in the host walk's order
   const double thetaMono = theta * theta * theta * theta;
   hipLaunchKernelGGL(walk_records_kernel, dim3((nn + 255) / 256), dim3(256), 0, s, nn, node64,
-                     first, count, child, 2.0 / sqrt(3.0), theta, thetaMono, rec);
+                     first, count, child, esc, 2.0 / sqrt(3.0), theta, thetaMono, rec);
   return hipGetLastError();
 }
 
@@ -181,10 +234,10 @@ __global__ void walk_init_kernel(int nb, int* nCell, int* nPart, int* err) {
 }
 
 // work buffer: error flag and two 64-bit totals (256 B), per-bucket counts
-// (2 x (nb + 1) ints, 256-byte aligned), the walk records, rocPRIM scan
-// storage
+// (2 x (nb + 1) ints, 256-byte aligned), the walk records, the escape nodes
+// and their pass buffers (4 x nn ints), rocPRIM scan storage
 struct WalkWork {
-  size_t counts, rec, temp, total;
+  size_t counts, rec, esc, temp, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -218,7 +271,8 @@ static WalkWork work_layout(int nb, int nn, size_t scan) {
   WalkWork w;
   w.counts = 256;
   w.rec = w.counts + align256(2 * sizeof(int) * ((size_t)nb + 1));
-  w.temp = w.rec + align256(sizeof(WalkRec) * (size_t)nn);
+  w.esc = w.rec + align256(sizeof(WalkRec) * (size_t)nn);
+  w.temp = w.esc + align256(4 * sizeof(int) * (size_t)nn);
   w.total = w.temp + align256(scan);
   return w;
 }
@@ -231,7 +285,9 @@ static WalkArgs walk_args(const ChangaListBuild* L, int* err) {
   a.nPart = a.nCell + (L->nBuckets + 1);
   a.bucketNode = L->bucketNode;
   a.node64 = L->node64;
-  a.rec = reinterpret_cast<const WalkRec*>(base + work_layout(L->nBuckets, L->numNodes, 0).rec);
+  const WalkWork W = work_layout(L->nBuckets, L->numNodes, 0);
+  a.rec = reinterpret_cast<const WalkRec*>(base + W.rec);
+  a.esc = reinterpret_cast<const int*>(base + W.esc);
   a.period = L->nReplicas > 0 ? L->period : 0.0;
   a.R = L->nReplicas;
   a.cellMarks = L->cellMarks;
@@ -268,8 +324,16 @@ extern "C" int changaBuildListsCount(const ChangaListBuild* L, long long* numCel
   const WalkArgs args = walk_args(L, err);
   hipLaunchKernelGGL(walk_init_kernel, dim3(1), dim3(1), 0, s, nb, args.nCell, args.nPart, err);
   CGPU_CHECK(hipGetLastError());
-  CGPU_CHECK(walk_records(L->numNodes, L->node64, L->first, L->count, L->child, L->theta,
-                          const_cast<WalkRec*>(args.rec), s));
+  {
+    // escape nodes first: the records carry them
+    const int nn = L->numNodes;
+    int* esc = const_cast<int*>(args.esc);
+    if (int e = walk_escapes(nn, L->child, esc, esc + nn, esc + 2 * (size_t)nn,
+                             esc + 3 * (size_t)nn, err, s))
+      return e;
+    CGPU_CHECK(walk_records(nn, L->node64, L->first, L->count, L->child, esc, L->theta,
+                            const_cast<WalkRec*>(args.rec), s));
+  }
   if (nb > 0) {
     hipLaunchKernelGGL(walk_kernel<false>, dim3((nb + kWalkBlock - 1) / kWalkBlock),
                        dim3(kWalkBlock), 0, s, args);
@@ -292,7 +356,7 @@ extern "C" int changaBuildListsCount(const ChangaListBuild* L, long long* numCel
   CGPU_CHECK(hipMemcpyAsync(tot, err + 2, sizeof(tot), hipMemcpyDeviceToHost, s));
   CGPU_CHECK(hipStreamSynchronize(s));
   if (flag)
-    return fail("changaBuildListsCount: a walk overflowed its stack (tree too deep)",
+    return fail("changaBuildListsCount: tree deeper than 256 levels",
                 hipErrorInvalidValue, __FILE__, __LINE__);
   if (tot[0] > 0x7fffffffull || tot[1] > 0x7fffffffull)
     return fail("changaBuildListsCount: list exceeds the int32 marker range",

@@ -18,17 +18,22 @@ namespace cgpu {
 
 // 64 bytes, one cache line per visited node: the centre of mass, the
 // opening radius, twice the softening, the monopole opening radius, the
-// particle count, the children (c0 < 0: a bucket) and the first particle.
+// particle count, the first child (c0 < 0: a bucket), a link and the first
+// particle. The link is the second child for the cooperative walk and the
+// escape node (the depth-first successor of the node's subtree) for the
+// stackless depth-first walk of the list builder.
 struct alignas(16) WalkRec {
   double cx, cy, cz, rad, rs, rm;
-  int count, c0, c1, first;
+  int count, c0, link, first;
 };
 static_assert(sizeof(WalkRec) == 64, "one cache line per node");
 
 // The records of nodes [0, nn) from the node records of changaBuildMoments
-// (node64: cm.xyz, radius, soft, mass, lo.xyz, hi.xyz, moments; 34 doubles).
+// (node64: cm.xyz, radius, soft, mass, lo.xyz, hi.xyz, moments; 34 doubles);
+// link = esc[node] when esc is given, else the second child.
 hipError_t walk_records(int nn, const double* node64, const int* first, const int* count,
-                        const int* child, double theta, WalkRec* rec, hipStream_t s);
+                        const int* child, const int* esc, double theta, WalkRec* rec,
+                        hipStream_t s);
 
 #pragma clang fp contract(off)
 
@@ -131,7 +136,7 @@ __device__ __forceinline__ bool coop_walk(const WalkRec* rec, const double* bg, 
     if (is_push) {
       const int at = sp + 2 * lanes_below(mp, lane);
       const unsigned hi = (unsigned)code << kCoopNodeBits;
-      stack[at] = hi | (unsigned)v.c1;
+      stack[at] = hi | (unsigned)v.link;  // the second child
       stack[at + 1] = hi | (unsigned)v.c0;
     }
     nhex += ch;
