@@ -40,3 +40,16 @@ def rel_err(a, b):
 
 def summary(e):
     return dict(median=float(np.median(e)), p99=float(np.percentile(e, 99)), max=float(e.max()))
+
+
+def deep_chain_workload(n=1500):
+    """Particles at radii 2^(-k/25) (k < n) around the box centre: a chain of
+    ~60 tree levels, ending in one bucket of ~950 particles whose Morton keys
+    coincide (beyond 21 bits per axis)."""
+    from changa_amd import configs
+    w = configs.plummer(n=n)
+    rng = np.random.default_rng(3)
+    r = 2.0 ** (-np.arange(n) / 25.0)
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return dict(w, pos=np.ascontiguousarray(0.5 + 0.5 * r[:, None] * d))

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from changa_amd import configs, engine
-from tests.helpers import oracle_forces, rel_err, summary
+from tests.helpers import deep_chain_workload, oracle_forces, rel_err, summary
 
 pytestmark = pytest.mark.gpu
 
@@ -21,6 +21,7 @@ CASES = {
     "periodic_r3": lambda: dict(configs.zeldovich(ng=12, rms=0.05), n_replicas=3),
     "active": lambda: configs.zeldovich(ng=16, rms=0.1, active_frac=0.2),
     "halos": lambda: configs.halos(n=32768),
+    "deep_chain": deep_chain_workload,
 }
 
 

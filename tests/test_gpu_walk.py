@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from changa_amd import configs, engine, host
-from tests.helpers import oracle_forces, rel_err, summary
+from tests.helpers import deep_chain_workload, oracle_forces, rel_err, summary
 
 pytestmark = pytest.mark.gpu
 
@@ -16,6 +16,7 @@ CASES = {
     "softened": lambda: configs.plummer(n=4096, eps=0.5),
     "periodic": lambda: configs.zeldovich(ng=16, rms=0.05),
     "halos": lambda: configs.halos(n=32768),
+    "deep_chain": deep_chain_workload,
 }
 
 
