@@ -249,11 +249,13 @@ def main():
             "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s",
             "frac": round(achieved * 1e12 / PEAK_FP32, 4),
-            "traffic": traffic_from_profile(
-                tag, {"batch_kernel": f"batch_kernel<{opts.pcTile},{opts.ppTile}>",
-                      "fused_kernel": "fused_kernel<64,256>"}.get(dom, dom)),
         },
     }
+    traffic, traffic_kernel = traffic_from_profile(
+        tag, {"batch_kernel": f"batch_kernel<{opts.pcTile},{opts.ppTile}>",
+              "fused_kernel": "fused_kernel<64,256>"}.get(dom, dom))
+    out["roofline"]["traffic"] = traffic
+    out["roofline"]["traffic_kernel"] = traffic_kernel
 
     log(f"timed: {value:.1f} G interactions/s")
     if not args.no_accuracy:
@@ -471,16 +473,24 @@ def grad_accuracy(ev, wt, w, n_samples=4096):
 
 
 def traffic_from_profile(tag, kernel):
-    """HBM bytes per launch of `kernel` (for the batched kernel, of its
-    variant, e.g. "batch_kernel<64,256>"), from the committed rocprofv3 --pmc
-    summary of this workload (FETCH_SIZE and WRITE_SIZE, copy-calibrated)."""
+    """(HBM bytes per launch, the profiled kernel) of `kernel` (for the
+    batched kernel, of its variant, e.g. "batch_kernel<64,256>"), from the
+    committed rocprofv3 --pmc summary of this workload (FETCH_SIZE and
+    WRITE_SIZE, copy-calibrated). The autotuner may have picked another
+    tile variant in the profiled run: then that variant's figure is returned,
+    with its name."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.md")
     if not os.path.exists(path):
-        return None
+        return None, None
+    found = {}
     for line in open(path):
-        if line.startswith(f"{kernel} traffic_bytes_per_launch:"):
-            return int(line.split(":")[1])
-    return None
+        if " traffic_bytes_per_launch:" in line:
+            name, v = line.split(" traffic_bytes_per_launch:")
+            found[name.strip()] = int(v)
+    if kernel in found:
+        return found[kernel], kernel
+    family = [k for k in found if k.split("<")[0] == kernel.split("<")[0]]
+    return (found[family[0]], family[0]) if family else (None, None)
 
 
 def accuracy(ev, step, n_samples):
