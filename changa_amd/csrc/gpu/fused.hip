@@ -54,7 +54,9 @@ struct FusedLds {
 
 // Walks bucket b into the wave's scratch: hexadecapole cells at [0, nhex),
 // softened cells at [maxCells - nsoft, maxCells), runs at [0, nruns).
-// Returns false (and raises err) when a capacity is exceeded.
+// Returns false (and raises err) when a capacity is exceeded. Softened cells
+// count twice against maxCells, so that the two regions never overlap and
+// the move behind the hexadecapole cells reads no slot it has written.
 __device__ bool fused_walk(const FusedArgs& a, int b, int lane, unsigned* stack, ILCell* cells,
                            ILPart* runs, int& nhex, int& nsoft, int& nruns) {
   double bg[12];
@@ -66,7 +68,7 @@ __device__ bool fused_walk(const FusedArgs& a, int b, int lane, unsigned* stack,
   const int maxCells = a.maxCells, maxRuns = a.maxRuns;
   const bool ok = coop_walk(
       a.rec, bg, 2.0 * bg[4], a.R, a.period, lane, stack, nhex, nsoft, nruns,
-      [&](int h, int s, int r) { return h + s <= maxCells && r <= maxRuns; },
+      [&](int h, int s, int r) { return h + 2 * s <= maxCells && r <= maxRuns; },
       [&](int kind, int nd, int code, const WalkRec& v, int slot) {
         const int oid = code << 22;
         if (kind == 1)

@@ -338,8 +338,8 @@ int changaBuildTree(ChangaTreeBuild *t, void *stream);
  * added into cells.vars like changaNodeGravity + changaPartGravity; they
  * equal the list path's up to the FP32 order of the per-tile sums.
  * errorFlag (host, optional) receives 1, once the stream reaches it, when
- * a bucket's walk exceeded a capacity; that bucket's results are then
- * incomplete.
+ * a bucket's walk exceeded a capacity (hexadecapole cells + 2 x softened
+ * cells > maxCells, or runs > maxRuns); the results are then incomplete.
  * ====================================================================== */
 typedef struct {
   ChangaGravityLaunch cells;      /* targets, vars, moments, cm64, tpos64, starts, sizes,
