@@ -58,3 +58,32 @@ def test_header_compiles_as_c_and_links(tmp_path):
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I" + inc, str(src), "-o", str(exe),
                     "-L" + libdir, "-lchanga_gpu", "-Wl,-rpath," + libdir,
                     "-Wl,--unresolved-symbols=ignore-in-shared-libs"], check=True)
+
+
+def test_ctypes_mirrors_match_the_header(tmp_path):
+    """Every struct the Python harness mirrors has the header's size and
+    field offsets (a C program compiled against the header prints them)."""
+    import subprocess
+    structs = {name: cls for name, cls in vars(gpu).items()
+               if isinstance(cls, type) and issubclass(cls, ctypes.Structure)
+               and name.startswith("Changa")}
+    assert {"ChangaGravityLaunch", "ChangaAdjointLaunch", "ChangaListBuild", "ChangaTreeBuild",
+            "ChangaFusedLaunch"} <= set(structs)
+    lines = []
+    for name, cls in sorted(structs.items()):
+        lines.append(f'  printf("{name} %zu\\n", sizeof({name}));')
+        for f in cls._fields_:
+            lines.append(f'  printf("{name}.{f[0]} %zu\\n", offsetof({name}, {f[0]}));')
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "changa_gpu.h"\n'
+                   "int main(void) {\n" + "\n".join(lines) + "\n  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I" + os.path.dirname(HDR), str(src),
+                    "-o", str(exe)], check=True)
+    out = dict(line.rsplit(" ", 1) for line in
+               subprocess.run([str(exe)], check=True, capture_output=True,
+                              text=True).stdout.splitlines())
+    for name, cls in structs.items():
+        assert int(out[name]) == ctypes.sizeof(cls), name
+        for f in cls._fields_:
+            assert int(out[f"{name}.{f[0]}"]) == getattr(cls, f[0]).offset, f"{name}.{f[0]}"
