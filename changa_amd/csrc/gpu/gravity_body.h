@@ -405,15 +405,16 @@ template <int TILE>
 struct PpLds {
   float4 src4[TILE];
   float srcs[TILE];
-  int r_idx[kWave], r_oid[kWave], r_end[kWave];
+  int2 map[TILE];  // ILPart runs expanded: (source particle, offsetID) per tile slot
 };
 
 // Tiles of the sources [start, end) of a.list staged into LDS (s.src4: x, y,
 // z relative to the bucket centre with the replica offset applied, mass;
 // s.srcs: softening); consume(ns) runs after each tile is staged. KIND 0: one
 // ILCell per source particle (the list ChaNGa's GenericList<ILPart>::serialize
-// produces, Compute.cpp:1118-1218); KIND 1: ILPart runs expanded here through
-// a wavefront prefix sum.
+// produces, Compute.cpp:1118-1218); KIND 1: ILPart runs expanded here: a
+// wavefront prefix sum places each run, and each lane writes its run's slots
+// of the tile into LDS.
 template <bool PRECISE, int KIND, int TILE, bool STREAM = true, typename Consume>
 __device__ __forceinline__ void pp_tiles(const ChangaGravityLaunch& a, int start, int end,
                                          double cx0, double cy0, double cz0, int lane,
@@ -454,36 +455,33 @@ __device__ __forceinline__ void pp_tiles(const ChangaGravityLaunch& a, int start
     const ILPart* lst = static_cast<const ILPart*>(a.list);
     for (int rb = start; rb < end; rb += kWave) {
       const int nr = min(kWave, end - rb);
-      int num = 0;
+      int num = 0, first = 0, oid = 0;
       if (lane < nr) {
         const ILPart p = load_entry<STREAM>(lst + rb + lane);
-        s.r_idx[lane] = p.index;
-        s.r_oid[lane] = p.off;
+        first = p.index;
+        oid = p.off;
         num = p.num;
       }
-      // inclusive prefix sum of run lengths across the wavefront
+      // inclusive prefix sum of run lengths across the wavefront: this lane's
+      // run covers [incl - num, incl) of the expanded sequence
       int incl = num;
 #pragma unroll
       for (int d = 1; d < kWave; d <<= 1) {
         int y = __shfl_up(incl, d, kWave);
         if (lane >= d) incl += y;
       }
-      s.r_end[lane] = incl;
       const int total = __shfl(incl, kWave - 1, kWave);
-      __syncthreads();
+      const int begin = incl - num;
       for (int sb = 0; sb < total; sb += TILE) {
         const int ns = min(TILE, total - sb);
+        // each lane expands the part of its run that falls in this tile
+        const int j1 = min(incl, sb + ns);
+        for (int j = max(begin, sb); j < j1; ++j)
+          s.map[j - sb] = make_int2(first + (j - begin), oid);
+        __syncthreads();
         for (int i = lane; i < ns; i += kWave) {
-          int q = sb + i;
-          // first run whose inclusive end exceeds q
-          int lo = 0, hi = nr - 1;
-          while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (s.r_end[mid] > q) hi = mid;
-            else lo = mid + 1;
-          }
-          int begin = lo ? s.r_end[lo - 1] : 0;
-          stage(i, s.r_idx[lo] + (q - begin), s.r_oid[lo]);
+          const int2 m = s.map[i];
+          stage(i, m.x, m.y);
         }
         __syncthreads();
         consume(ns);
