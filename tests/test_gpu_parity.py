@@ -17,7 +17,7 @@ import pytest
 
 from changa_amd import configs, engine
 from oracle import oracle as O
-from tests.helpers import oracle_forces, rel_err, summary
+from tests.helpers import deep_chain_workload, oracle_forces, rel_err, summary
 
 pytestmark = pytest.mark.gpu
 
@@ -35,17 +35,25 @@ def _check(ev, tol_max, tol_p99, label, tol_med=1e-5):
         assert e.max() <= tol_max, summary(e)
 
 
-def test_gpu_moments_match_oracle(device):
+@pytest.mark.parametrize("case", ["plummer", "deep_chain_gpu_tree"])
+def test_gpu_moments_match_oracle(device, case):
+    """GPU multipole build against the oracle's; the deep chain (61 levels,
+    one 950-particle bucket) on the GPU-built tree with device gathers. The
+    two builds round differently (the GPU shifts axis by axis, the oracle by
+    the binomial sum) and a chain node carries up to 60 shift-and-project
+    steps of rounding: per-row bar 1e-9 there, 1e-10 on the sphere."""
     import torch
-    w = configs.plummer(n=4096)
-    ev = engine.Evaluation(w, device=device)
+    if case == "plummer":
+        ev = engine.Evaluation(configs.plummer(n=4096), device=device)
+    else:
+        ev = engine.Evaluation(deep_chain_workload(), device=device, tree="gpu")
     torch.cuda.synchronize()
     g = ev.node64()
     geo, mom = O.build_moments(ev.tree, ev.pos, ev.mass, ev.soft)
     np.testing.assert_allclose(g[:, 0:12], geo, rtol=1e-12, atol=1e-14)
     scale = np.abs(mom).max(axis=1, keepdims=True)
     assert np.abs(g[:, 12:] - mom).max() / scale.max() < 1e-11
-    assert (np.abs(g[:, 12:] - mom) / scale).max() < 1e-10
+    assert (np.abs(g[:, 12:] - mom) / scale).max() < (1e-10 if case == "plummer" else 1e-9)
 
 
 @pytest.mark.parametrize("precise", [True, False])
