@@ -66,14 +66,16 @@ def test_gpu_adjoint_matches_oracle(device, case):
     assert tot < 1e-5, tot
 
 
-@pytest.mark.parametrize("active", [False, True])
-def test_gpu_adjoint_ewald_matches_oracle(device, active):
+@pytest.mark.parametrize("active,reps", [(False, 1), (True, 1), (False, 2)])
+def test_gpu_adjoint_ewald_matches_oracle(device, active, reps):
     """Periodic box, L over tree plus Ewald acceleration: the Ewald term's
     target side, and its source side through the root's moments, centre of
     mass and h-table, against the oracle's adjoint_ewald added to the list
-    adjoint (root_extra)."""
+    adjoint (root_extra); with one and with two replica shells (BucketEwald's
+    hole follows the shells)."""
     import torch
-    w = configs.zeldovich(ng=16, rms=0.05, active_frac=0.3 if active else None)
+    w = dict(configs.zeldovich(ng=16, rms=0.05, active_frac=0.3 if active else None),
+             n_replicas=reps)
     ev = engine.Evaluation(w, precise=True, device=device)
     wt = np.random.default_rng(22).standard_normal((ev.n_targets, 3))
     g = ev.adjoint(wt).cpu().numpy()
@@ -82,7 +84,7 @@ def test_gpu_adjoint_ewald_matches_oracle(device, active):
     wv[ev.t_index if ev.t_index is not None else slice(None)] = wt
     go = _oracle_grad(ev, wv, ewald=True)
     ex, em = _errors(g, go)
-    print(f"adjoint ewald active={active}: dL/dx {summary(ex)} dL/dm {summary(em)}", flush=True)
+    print(f"adjoint ewald active={active} R={reps}: dL/dx {summary(ex)} dL/dm {summary(em)}", flush=True)
     for e in (ex, em):
         assert np.median(e) <= 2e-5 and np.percentile(e, 99) <= 5e-4, summary(e)
     tot = np.abs(g[:, :3].sum(axis=0)).max() / np.abs(g[:, :3]).sum(axis=0).max()
