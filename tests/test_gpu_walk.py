@@ -15,6 +15,8 @@ CASES = {
     "plummer": lambda: configs.plummer(n=32768),
     "softened": lambda: configs.plummer(n=4096, eps=0.5),
     "periodic": lambda: configs.zeldovich(ng=16, rms=0.05),
+    "periodic_r2": lambda: dict(configs.zeldovich(ng=16, rms=0.05), n_replicas=2),
+    "periodic_r3": lambda: dict(configs.zeldovich(ng=12, rms=0.05), n_replicas=3),
     "halos": lambda: configs.halos(n=32768),
     "deep_chain": deep_chain_workload,
 }
