@@ -112,13 +112,13 @@ def _run_mirror(ev, pp_expanded):
     return out, fired, expect
 
 
-@pytest.mark.parametrize("case", ["plummer_soft", "periodic"])
+@pytest.mark.parametrize("case", ["plummer_soft", "periodic", "periodic_r2"])
 def test_gpu_mirror_matches_device_path(device, case):
     import torch
     if case == "plummer_soft":
         w = configs.plummer(n=3000, eps=0.5)
     else:
-        w = configs.zeldovich(ng=16, rms=0.03)
+        w = dict(configs.zeldovich(ng=16, rms=0.03), n_replicas=2 if case == "periodic_r2" else 1)
     ev = engine.Evaluation(w, precise=False, pp_compact=False, device=device)
     assert case != "plummer_soft" or ev.counts[1] > 0
     out, fired, expect = _run_mirror(ev, pp_expanded=True)
