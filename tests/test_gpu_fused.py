@@ -39,11 +39,7 @@ def _both(ev):
     return ev.d_vars.cpu().numpy().astype(np.float64), ref
 
 
-@pytest.mark.parametrize("case", list(CASES))
-def test_gpu_fused_matches_list_path(device, case):
-    ev = engine.Evaluation(CASES[case](), precise=True, device=device, lists="gpu")
-    print(f"fused {case}: {ev.n_targets} targets", flush=True)
-    got, ref = _both(ev)
+def _same(got, ref, case):
     # differences relative to the rms magnitude (periodic forces are small
     # residuals of large cancelling sums), and per particle at the median
     ra = np.linalg.norm(ref[:, :3], axis=1)
@@ -54,10 +50,29 @@ def test_gpu_fused_matches_list_path(device, case):
     assert ea.max() < 2e-5 and ep.max() < 2e-5
     assert np.median(da / ra) < 1e-6 and np.median(dp / np.abs(ref[:, 3])) < 1e-6
     np.testing.assert_array_equal(got[:, 4], ref[:, 4])
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_gpu_fused_matches_list_path(device, case):
+    ev = engine.Evaluation(CASES[case](), precise=True, device=device, lists="gpu")
+    print(f"fused {case}: {ev.n_targets} targets", flush=True)
+    got, ref = _both(ev)
+    _same(got, ref, case)
     # a second launch on the same scratch: each bucket's walk and sums are
     # in a fixed order whichever wavefront claims it
     got2, _ = _both(ev)
     np.testing.assert_array_equal(got2, got)
+
+
+def test_gpu_fused_full_box(device):
+    """The 128^3 box (2,097,152 particles, theta 0.7, bucket 16, one replica
+    shell) at full size, tree and lists built on the GPU: the same bars
+    against the list path."""
+    ev = engine.Evaluation(configs.zeldovich(ng=128), precise=True, device=device, lists="gpu",
+                           tree="gpu")
+    print(f"fused full box: {ev.n_targets} targets", flush=True)
+    got, ref = _both(ev)
+    _same(got, ref, "full box")
 
 
 def test_gpu_fused_periodic_oracle(device):
