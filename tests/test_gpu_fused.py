@@ -64,15 +64,17 @@ def test_gpu_fused_matches_list_path(device, case):
     np.testing.assert_array_equal(got2, got)
 
 
-def test_gpu_fused_full_box(device):
-    """The 128^3 box (2,097,152 particles, theta 0.7, bucket 16, one replica
-    shell) at full size, tree and lists built on the GPU: the same bars
-    against the list path."""
-    ev = engine.Evaluation(configs.zeldovich(ng=128), precise=True, device=device, lists="gpu",
+@pytest.mark.parametrize("cfg", [2, 3, 4, 5])
+def test_gpu_fused_full_size(device, cfg):
+    """The full-size workloads, tree and lists built on the GPU: the 128^3
+    box (2,097,152 particles), the 2^22-particle halos, the 2^20 Plummer
+    sphere and the 256^3 box with a 10% active rung, at the same bars against
+    the list path."""
+    ev = engine.Evaluation(configs.workload(cfg), precise=True, device=device, lists="gpu",
                            tree="gpu")
-    print(f"fused full box: {ev.n_targets} targets", flush=True)
+    print(f"fused full size {cfg}: {ev.n_targets} targets", flush=True)
     got, ref = _both(ev)
-    _same(got, ref, "full box")
+    _same(got, ref, f"full size {cfg}")
 
 
 def test_gpu_fused_periodic_oracle(device):
