@@ -116,14 +116,16 @@ def test_gpu_periodic_more_replica_shells(device, reps):
         summary(ep)
 
 
-@pytest.mark.parametrize("cfg", [2, 5])
+@pytest.mark.parametrize("cfg", [2, 3, 4, 5])
 def test_gpu_full_size_sampled_oracle(device, cfg):
-    """The full-size periodic boxes (the 128^3 box; the 256^3 box with a 10%
-    active rung), tree and lists built on the GPU, through the batched step
-    with the overlapped Ewald pass: the targets of 128 seeded buckets
-    against the FP64 oracle on the same lists, at the periodic bars. The
-    periodic potential (zero mean over the box) crosses zero among the
-    clustered box's particles, so its error is taken relative to the
+    """The full-size workloads (the 128^3 box, the 2^22-particle halos, the
+    2^20 Plummer sphere, the 256^3 box with a 10% active rung), tree and
+    lists built on the GPU, through the batched step (with the overlapped
+    Ewald pass for the boxes): the targets of 128 seeded buckets against the
+    FP64 oracle on the same lists, at the periodic bars for the boxes and
+    median 1e-6, 99th percentile 1e-5 for the isolated ones. The periodic
+    potential (zero mean over the box) crosses zero among the clustered
+    box's particles, so the potential's error is taken relative to the
     sample's rms potential rather than per particle."""
     import torch
     ev = engine.Evaluation(configs.workload(cfg), precise=True, device=device, lists="gpu",
@@ -143,6 +145,7 @@ def test_gpu_full_size_sampled_oracle(device, cfg):
     ea = rel_err(acc[idx], oa[idx])
     ep = np.abs(pot[idx] - op[idx]) / np.sqrt(np.mean(op[idx] ** 2))
     print(f"full size {cfg}: {len(idx)} targets, acc {summary(ea)} pot {summary(ep)}", flush=True)
+    med, p99 = (5e-6, 5e-5) if ev.w["periodic"] else (1e-6, 1e-5)
     for e in (ea, ep):
-        assert np.median(e) <= 5e-6 and np.percentile(e, 99) <= 5e-5, summary(e)
+        assert np.median(e) <= med and np.percentile(e, 99) <= p99, summary(e)
     ev.free_batch()
