@@ -46,3 +46,15 @@ def test_active_target_set_groups_active_particles_by_bucket():
     counts = host.interaction_counts(tr, ev.lists, ev.active)
     pc_active = (np.diff(ev.t_lists["cell_marks"]) * ev.t_sizes).sum()
     assert pc_active == counts[0] + counts[1]
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """Without the built HIP library the product path raises; there is no
+    CPU fallback to route through."""
+    import pytest
+
+    from changa_amd import gpu
+    monkeypatch.setattr(gpu, "_lib", None)
+    monkeypatch.setattr(gpu._build, "GPU_LIB", str(tmp_path / "libchanga_gpu.so"))
+    with pytest.raises(RuntimeError, match="not built"):
+        gpu.lib()
