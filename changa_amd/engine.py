@@ -162,7 +162,6 @@ class Evaluation:
         self.precise = precise
         self.pp_compact = pp_compact
         self.list_source = lists
-        self.pc_variant = 0
         self.device = torch.device(device)
         periodic = bool(w["periodic"])
         self.L = float(w["L"]) if periodic else 0.0
@@ -365,9 +364,17 @@ class Evaluation:
 
     def build_lists(self):
         """Rebuild the GPU-built lists from the current node records (after
-        build_moments, e.g. for new positions on the same tree topology)."""
+        build_moments, e.g. for new positions on the same tree topology).
+
+        The new lists live in new device arrays, so everything that holds the
+        old arrays' addresses is dropped: the batched plan (its graph keeps
+        the launch records by value), the fused launch record and the
+        adjoint's list work buffer (sized for the old list lengths)."""
         if self.list_source != "gpu":
             raise ValueError("the lists of this evaluation come from the host walk")
+        self.free_batch()
+        self._fused = None
+        self.d_list_work = None
         self._lists = None
         self._gpu_lists()
 
@@ -513,7 +520,6 @@ class Evaluation:
         a.numBuckets = len(self.t_sizes)
         a.listKind = 0 if pc else (1 if self.pp_compact else 0)
         a.fperiod = self.L
-        a.variant = self.pc_variant if pc else 0
         if self.precise:
             a.tpos64 = self.d_tpos64.data_ptr()
             a.spos64 = self.d_pos64.data_ptr()
@@ -711,9 +717,12 @@ class Evaluation:
         if getattr(self, "d_node_work", None) is None:
             self.d_node_work = torch.empty((tr["n_nodes"], 112), dtype=torch.float64,
                                            device=self.device)
-            ints = gpu.lib().changaGravityAdjointWorkInts(tr["n_nodes"], entries, part_entries, n)
-            if ints == 0:
-                raise RuntimeError("changaGravityAdjointWorkInts failed")
+        ints = gpu.lib().changaGravityAdjointWorkInts(tr["n_nodes"], entries, part_entries, n)
+        if ints == 0:
+            raise RuntimeError("changaGravityAdjointWorkInts failed")
+        work = getattr(self, "d_list_work", None)
+        if work is None or work.numel() < ints:
+            # sized for the current lists (they change with build_lists)
             self.d_list_work = torch.empty(ints, dtype=torch.int32, device=self.device)
         lo = np.ascontiguousarray(tr["level_offsets"], dtype=np.int32)
         a = gpu.ChangaAdjointLaunch()

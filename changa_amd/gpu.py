@@ -206,6 +206,9 @@ def lib():
     L.changaFusedGravityWorkBytes.argtypes = [c_int, c_int, c_int]
     L.changaFusedGravityWorkBytes.restype = c_size_t
     L.changaFusedGravity.argtypes = [c_void_p, c_void_p]
+    # Python callers get a RuntimeError from check() instead of the library's
+    # default abort (the cudaChk convention of HostCUDA.cu:39-47)
+    L.changaGpuSetAbortOnError(0)
     _lib = L
     return L
 

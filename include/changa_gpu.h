@@ -128,7 +128,7 @@ typedef struct {
   int numBuckets;
   int listKind;                          /* 0 ILCell per source, 1 ILPart runs */
   float fperiod;
-  int variant;                           /* kernel variant; 0 = default */
+  int variant;                           /* reserved: must be 0 */
   const double *tpos64;                  /* 3 per target particle, or NULL */
   const double *spos64;                  /* 3 per source particle, or NULL */
   const double *cm64;                    /* 3 per cell, or NULL */

@@ -13,11 +13,9 @@ DUMMY = 0x1000  # never dereferenced: validation fails before any use
 
 @pytest.fixture
 def lib():
-    L = gpu.lib()
-    L.changaGpuSetAbortOnError(0)
+    L = gpu.lib()  # the binding switches abort-on-error off
     L.changaGpuLastError(ctypes.byref(ctypes.c_char_p()))  # clear
     yield L
-    L.changaGpuSetAbortOnError(1)
 
 
 def _last(L):
@@ -95,3 +93,14 @@ def test_add_vars(lib):
     assert lib.changaAddVars(None, None, 0, None) == 0
     assert lib.changaAddVars(None, None, 5, None) != 0
     assert "changaAddVars" in _last(lib)[1]
+
+
+def test_node_gravity_rejects_a_nonzero_variant(lib):
+    a = gpu.ChangaGravityLaunch()
+    a.numBuckets = 4
+    for name in ("targets", "vars", "moments", "list", "marks", "starts", "sizes"):
+        setattr(a, name, DUMMY)
+    a.variant = 1  # reserved field: only the one particle-cell kernel exists
+    assert lib.changaNodeGravity(ctypes.byref(a), None) != 0
+    assert "variant" in _last(lib)[1]
+

@@ -185,22 +185,18 @@ def test_gpu_ewald_overlap_is_bit_identical(device):
 def test_gpu_batched_rejects_bad_arguments(device):
     """Rejected on the host with an error code (no launch happens)."""
     ev = engine.Evaluation(configs.plummer(n=4096), device=device)
-    L = gpu.lib()
-    L.changaGpuSetAbortOnError(0)
-    try:
-        pc, pp = ev._launch(True), ev._launch(False)
-        plan = ctypes.c_void_p()
-        bad = gpu.ChangaBatchOptions(48, 0, 0, 0, 1, 0.0)
-        assert L.changaBatchPlanCreate(ctypes.byref(pc), ctypes.byref(pp), ctypes.byref(bad),
-                                       ctypes.byref(plan), None) != 0
-        assert not plan.value
-        pp2 = ev._launch(False)
-        pp2.numBuckets = pc.numBuckets - 1
-        ok = gpu.ChangaBatchOptions(0, 0, 0, 0, 1, 0.0)
-        assert L.changaBatchPlanCreate(ctypes.byref(pc), ctypes.byref(pp2), ctypes.byref(ok),
-                                       ctypes.byref(plan), None) != 0
-        msg = ctypes.c_char_p()
-        L.changaGpuLastError(ctypes.byref(msg))
-        assert b"differ" in msg.value
-    finally:
-        L.changaGpuSetAbortOnError(1)
+    L = gpu.lib()  # error-return mode (the binding switches abort-on-error off)
+    pc, pp = ev._launch(True), ev._launch(False)
+    plan = ctypes.c_void_p()
+    bad = gpu.ChangaBatchOptions(48, 0, 0, 0, 1, 0.0)
+    assert L.changaBatchPlanCreate(ctypes.byref(pc), ctypes.byref(pp), ctypes.byref(bad),
+                                   ctypes.byref(plan), None) != 0
+    assert not plan.value
+    pp2 = ev._launch(False)
+    pp2.numBuckets = pc.numBuckets - 1
+    ok = gpu.ChangaBatchOptions(0, 0, 0, 0, 1, 0.0)
+    assert L.changaBatchPlanCreate(ctypes.byref(pc), ctypes.byref(pp2), ctypes.byref(ok),
+                                   ctypes.byref(plan), None) != 0
+    msg = ctypes.c_char_p()
+    L.changaGpuLastError(ctypes.byref(msg))
+    assert b"differ" in msg.value

@@ -148,15 +148,11 @@ def test_gpu_mirror_compact_part_lists(device):
 def test_gpu_error_mode_returns(device):
     """With abort-on-error off, a bad launch returns an error code and the
     message names the call (no fault: the library rejects it on the host)."""
-    L = gpu.lib()
-    L.changaGpuSetAbortOnError(0)
-    try:
-        a = gpu.ChangaGravityLaunch()
-        a.numBuckets = 5  # no arrays
-        rc = L.changaNodeGravity(ctypes.byref(a), None)
-        assert rc != 0
-        msg = ctypes.c_char_p()
-        assert L.changaGpuLastError(ctypes.byref(msg)) != 0
-        assert b"missing" in msg.value
-    finally:
-        L.changaGpuSetAbortOnError(1)
+    L = gpu.lib()  # error-return mode (the binding switches abort-on-error off)
+    a = gpu.ChangaGravityLaunch()
+    a.numBuckets = 5  # no arrays
+    rc = L.changaNodeGravity(ctypes.byref(a), None)
+    assert rc != 0
+    msg = ctypes.c_char_p()
+    assert L.changaGpuLastError(ctypes.byref(msg)) != 0
+    assert b"missing" in msg.value
