@@ -10,6 +10,7 @@ Run as `python -m changa_amd.build` or through __graft_entry__.build().
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -25,9 +26,11 @@ GPU_SRC = ["gpu/gravity.hip", "gpu/batch.hip", "gpu/adjoint.hip", "gpu/ewald.hip
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instruction selection stays off for every HIP object.
-HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function"]
+OBJ_DIR = os.path.join(PKG, "_build")
+JOBS = max(1, min(len(GPU_SRC), os.cpu_count() or 1, 16))
 
 
 def _run(cmd):
@@ -42,9 +45,16 @@ def build_host():
 
 
 def build_gpu():
-    srcs = [os.path.join(CSRC, s) for s in GPU_SRC]
-    _run([HIPCC] + HIP_FLAGS + ["-I" + INC, "-I" + os.path.join(CSRC, "gpu"),
-                                "-o", GPU_LIB] + srcs)
+    """One object per source, compiled in parallel, then one shared link."""
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    inc = ["-I" + INC, "-I" + os.path.join(CSRC, "gpu")]
+    jobs = []
+    for src in GPU_SRC:
+        obj = os.path.join(OBJ_DIR, os.path.basename(src).replace(".hip", ".o"))
+        jobs.append((obj, [HIPCC] + HIP_FLAGS + inc + ["-c", os.path.join(CSRC, src), "-o", obj]))
+    with ThreadPoolExecutor(JOBS) as ex:
+        list(ex.map(lambda j: _run(j[1]), jobs))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", GPU_LIB] + [o for o, _ in jobs])
 
 
 def build_oracle():
