@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -432,5 +433,217 @@ extern "C" int changaBatchPlanRun(ChangaBatchPlan* plan, void* stream) {
 
 extern "C" int changaBatchPlanDestroy(ChangaBatchPlan* plan) {
   release(plan);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batched evaluation without allocation. The queue (range starts and bucket
+// ids in tree order), its counters and the partial sums of the range split
+// live in one file-scope device array; host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device. Every call on
+// a GPU rewrites what it reads of the array before reading it.
+// ---------------------------------------------------------------------------
+namespace cgpu {
+
+constexpr int kNaChunk = 4096;  // buckets per partial cost sum
+constexpr int kNaMaxChunks = (CHANGA_NOALLOC_MAX_BUCKETS + kNaChunk - 1) / kNaChunk;
+// [counters][partial sums (float)][queue header][bucket ids]
+constexpr size_t kNaCounters = 0;
+constexpr size_t kNaPartials = kNumXcd * kCounterStride;
+constexpr size_t kNaQueue = kNaPartials + kNaMaxChunks;
+constexpr size_t kNaInts = kNaQueue + kQueueHdr + CHANGA_NOALLOC_MAX_BUCKETS;
+static_assert(kNaQueue % 4 == 0, "queue aligned to 16 bytes");
+
+static __device__ __align__(256) int scratch[kNaInts];
+
+// Per-bucket cost (the estimate of bucket_cost_kernel), stored as float bits
+// in the bucket-id region until the range split has read it.
+template <int KIND>
+__global__ void noalloc_cost_kernel(ChangaGravityLaunch pc, ChangaGravityLaunch pp, int* ws) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= pc.numBuckets) return;
+  const int nc = pc.marks[b + 1] - pc.marks[b];
+  float np = 0.0f;
+  if (KIND == 0) {
+    np = (float)(pp.marks[b + 1] - pp.marks[b]);
+  } else {
+    const ILPart* r = static_cast<const ILPart*>(pp.list);
+    for (int i = pp.marks[b]; i < pp.marks[b + 1]; ++i) np += (float)r[i].num;
+  }
+  const int n = pc.sizes[b];
+  float c = 0.0f;
+  for (int t0 = 0; t0 < n; t0 += kWave) {
+    const int nt = min(kWave, n - t0);
+    c += (130.0f * nc + 25.0f * np) / (float)(kWave / nt) + 400.0f;
+  }
+  reinterpret_cast<float*>(ws + kNaQueue + kQueueHdr)[b] = c;
+}
+
+// Sum of each chunk of kNaChunk costs (one 256-thread block per chunk).
+__global__ __launch_bounds__(256) void noalloc_chunk_sum_kernel(int nb, int* ws) {
+  __shared__ float part[256 / kWave];
+  const float* cost = reinterpret_cast<const float*>(ws + kNaQueue + kQueueHdr);
+  const int c0 = blockIdx.x * kNaChunk, c1 = min(nb, c0 + kNaChunk);
+  float s = 0.0f;
+  for (int b = c0 + (int)threadIdx.x; b < c1; b += 256) s += cost[b];
+  for (int d = kWave / 2; d > 0; d >>= 1) s += __shfl_xor(s, d, kWave);
+  if (threadIdx.x % kWave == 0) part[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    reinterpret_cast<float*>(ws + kNaPartials)[blockIdx.x] =
+        part[0] + part[1] + part[2] + part[3];
+}
+
+// Range starts: range x begins at the first bucket whose preceding cost
+// reaches x/8 of the total (make_queue's rule). Thread x < 8 searches the
+// chunk sums, then the costs inside its chunk.
+__global__ void noalloc_split_kernel(int nb, int nchunks, int* ws) {
+  const float* part = reinterpret_cast<const float*>(ws + kNaPartials);
+  const float* cost = reinterpret_cast<const float*>(ws + kNaQueue + kQueueHdr);
+  int* hdr = ws + kNaQueue;
+  const int x = threadIdx.x;
+  if (x > kNumXcd) return;
+  if (x == 0) {
+    hdr[0] = 0;
+    return;
+  }
+  if (x == kNumXcd) {
+    hdr[kNumXcd] = nb;
+    return;
+  }
+  double total = 0.0;
+  for (int c = 0; c < nchunks; ++c) total += part[c];
+  const double goal = total * x / kNumXcd;
+  double run = 0.0;
+  int c = 0;
+  while (c < nchunks && run + part[c] < goal) run += part[c++];
+  int b = c * kNaChunk;
+  const int bend = min(nb, b + kNaChunk);
+  while (b < bend && run < goal) run += cost[b++];
+  hdr[x] = min(b, nb);
+}
+
+__global__ void noalloc_ids_kernel(int nb, int* ws) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) ws[kNaQueue + kQueueHdr + b] = b;
+}
+
+// Zeroes the results of the particles the buckets cover.
+__global__ void zero_buckets_kernel(const int* starts, const int* sizes, int nb,
+                                    VariablePartData* vars) {
+  const int b = blockIdx.x;
+  if (b >= nb) return;
+  const int s = starts[b], n = sizes[b];
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    vars[s + i] = VariablePartData{{0.f, 0.f, 0.f}, 0.f, 0.f};
+}
+
+}  // namespace cgpu
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_na_mu;
+int* g_na_ptr[kMaxDevices] = {};
+
+// Address of this device's copy of the scratch array, looked up once per
+// device with that device current.
+int na_scratch(int* dev_out, int** out) {
+  int dev = 0;
+  CGPU_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail("changaBatchedGravityNoAlloc: device ordinal", hipErrorInvalidDevice, __FILE__,
+                __LINE__);
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    CGPU_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(cgpu::scratch)));
+    g_na_ptr[dev] = static_cast<int*>(p);
+  }
+  *dev_out = dev;
+  *out = g_na_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int changaBatchedGravityNoAlloc(const ChangaGravityLaunch* cells,
+                                           const ChangaGravityLaunch* parts,
+                                           const ChangaBatchOptions* opts, void* stream) {
+  if (!cells || !parts)
+    return fail("changaBatchedGravityNoAlloc: missing argument", hipErrorInvalidValue, __FILE__,
+                __LINE__);
+  const ChangaGravityLaunch& pc = *cells;
+  const ChangaGravityLaunch& pp = *parts;
+  const int nb = pc.numBuckets;
+  if (nb < 0 || nb > CHANGA_NOALLOC_MAX_BUCKETS)
+    return fail("changaBatchedGravityNoAlloc: numBuckets outside 0..CHANGA_NOALLOC_MAX_BUCKETS "
+                "(the size of the static work queue)",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  if (nb > 0 && (!pc.marks || !pc.starts || !pc.sizes || !pc.vars || !pc.targets ||
+                 !pc.moments || !pc.list || !pp.marks || !pp.list || !pp.sources))
+    return fail("changaBatchedGravityNoAlloc: missing array", hipErrorInvalidValue, __FILE__,
+                __LINE__);
+  if (!same_buckets(pc, pp))
+    return fail("changaBatchedGravityNoAlloc: cell and particle launches differ in their buckets",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  if (pc.tpos64 && (!pc.cm64 || !pp.spos64))
+    return fail("changaBatchedGravityNoAlloc: precision form needs cm64 and spos64",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  if (pp.listKind != 0 && pp.listKind != 1)
+    return fail("changaBatchedGravityNoAlloc: listKind", hipErrorInvalidValue, __FILE__, __LINE__);
+  ChangaBatchOptions o = {64, 256, 1, 1, 0, 0.0f};
+  if (opts) {
+    o = *opts;
+    if (!o.pcTile) o.pcTile = 64;
+    if (!o.ppTile) o.ppTile = 256;
+    if (!o.listsPerGrab) o.listsPerGrab = 1;
+    if (!o.order) o.order = 1;
+  }
+  if ((o.pcTile != 32 && o.pcTile != 64) || (o.ppTile != 128 && o.ppTile != 256) ||
+      o.listsPerGrab < 1 || o.listsPerGrab > kMaxGrab)
+    return fail("changaBatchedGravityNoAlloc: option out of range", hipErrorInvalidValue,
+                __FILE__, __LINE__);
+  if (o.order != 1)
+    return fail("changaBatchedGravityNoAlloc: only tree order (order 1) runs without allocation",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  int dev = 0;
+  int* ws = nullptr;
+  if (int e = na_scratch(&dev, &ws)) return e;
+  if (nb == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const BatchFn fn = pick(pc.tpos64 != nullptr, pp.listKind, o.pcTile, o.ppTile);
+  const int grid = grid_for(fn);
+  if (grid <= 0)
+    return fail("changaBatchedGravityNoAlloc: occupancy", hipErrorInvalidValue, __FILE__,
+                __LINE__);
+  if (o.zeroVars) {
+    hipLaunchKernelGGL(zero_buckets_kernel, dim3(nb), dim3(kWave), 0, s, pc.starts, pc.sizes, nb,
+                       pc.vars);
+    CGPU_CHECK(hipGetLastError());
+  }
+  const int nchunks = (nb + kNaChunk - 1) / kNaChunk;
+  if (pp.listKind)
+    hipLaunchKernelGGL(noalloc_cost_kernel<1>, dim3((nb + 255) / 256), dim3(256), 0, s, pc, pp, ws);
+  else
+    hipLaunchKernelGGL(noalloc_cost_kernel<0>, dim3((nb + 255) / 256), dim3(256), 0, s, pc, pp, ws);
+  CGPU_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(noalloc_chunk_sum_kernel, dim3(nchunks), dim3(256), 0, s, nb, ws);
+  CGPU_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(noalloc_split_kernel, dim3(1), dim3(kWave), 0, s, nb, nchunks, ws);
+  CGPU_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(noalloc_ids_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, nb, ws);
+  CGPU_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kNumXcd * kCounterStride), 0, s,
+                     ws + kNaCounters);
+  CGPU_CHECK(hipGetLastError());
+  BatchArgs a;
+  a.pc = pc;
+  a.pp = pp;
+  a.queue = ws + kNaQueue;
+  a.counters = ws + kNaCounters;
+  a.grab = o.listsPerGrab;
+  void* kargs[] = {&a};
+  CGPU_CHECK(hipLaunchKernel((const void*)fn, dim3(grid), dim3(kWave), kargs, 0, s));
   return 0;
 }

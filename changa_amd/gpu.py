@@ -132,8 +132,11 @@ EXPORTS = [
     "changaGravityAdjoint", "changaGravityAdjointWorkInts", "changaGravityAdjointEwaldWorkDoubles",
     "changaBuildListsWorkBytes", "changaBuildListsCount", "changaBuildListsFill",
     "changaBuildTreeWorkBytes", "changaBuildTree", "changaFusedGravityWorkBytes",
-    "changaFusedGravity",
+    "changaFusedGravity", "changaBatchedGravityNoAlloc",
 ]
+
+# buckets the no-allocation batched launch accepts (its static queue's size)
+NOALLOC_MAX_BUCKETS = 1 << 22
 
 _lib = None
 
@@ -190,6 +193,10 @@ def lib():
         getattr(L, name).restype = c_int
     L.changaBatchPlanDestroy.argtypes = [P]
     L.changaBatchPlanDestroy.restype = c_int
+    L.changaBatchedGravityNoAlloc.argtypes = [ctypes.POINTER(ChangaGravityLaunch),
+                                              ctypes.POINTER(ChangaGravityLaunch),
+                                              ctypes.POINTER(ChangaBatchOptions), P]
+    L.changaBatchedGravityNoAlloc.restype = c_int
     L.changaGravityAdjoint.argtypes = [ctypes.POINTER(ChangaAdjointLaunch), P]
     L.changaGravityAdjoint.restype = c_int
     L.changaGravityAdjointWorkInts.argtypes = [c_int, c_int, c_int, c_int]

@@ -202,6 +202,24 @@ int changaBatchPlanLaunch(ChangaBatchPlan *plan, void *stream);
 int changaBatchPlanRun(ChangaBatchPlan *plan, void *stream);
 int changaBatchPlanDestroy(ChangaBatchPlan *plan);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+   memory (code under stream capture, tight serving loops, harnesses that
+   forbid hipMalloc): one batched evaluation like changaBatchPlanLaunch that
+   calls no allocator. Its work queue (range starts and bucket ids in tree
+   order), queue counters and cost sums live in a static device array of
+   4 x (CHANGA_NOALLOC_MAX_BUCKETS + 1,296) bytes (about 16 MiB), which each
+   GPU the library runs on holds until the process ends; every call rewrites
+   what it reads of it. Calls that overlap on one GPU (two streams, two host
+   threads) race on that array: serialize them. opts may be NULL (cell tile
+   64, particle tile 256, one bucket per claim); fields left at 0 take those
+   values; order must be 1 (tree order: the costliest-first order needs a
+   sort); zeroVars zeroes the results of the particles the buckets cover.
+   More than CHANGA_NOALLOC_MAX_BUCKETS buckets is an error. Results equal
+   changaBatchPlanLaunch's with the same options. */
+#define CHANGA_NOALLOC_MAX_BUCKETS (1 << 22)
+int changaBatchedGravityNoAlloc(const ChangaGravityLaunch *cells, const ChangaGravityLaunch *parts,
+                                const ChangaBatchOptions *opts, void *stream);
+
 /* ======================================================================
  * Part 4: adjoint form
  *

@@ -104,3 +104,28 @@ def test_node_gravity_rejects_a_nonzero_variant(lib):
     assert lib.changaNodeGravity(ctypes.byref(a), None) != 0
     assert "variant" in _last(lib)[1]
 
+
+
+def _noalloc_launches(nb):
+    pc, pp = gpu.ChangaGravityLaunch(), gpu.ChangaGravityLaunch()
+    for side in (pc, pp):
+        side.numBuckets = nb
+        for name in ("targets", "vars", "moments", "sources", "list", "marks", "starts",
+                     "sizes"):
+            setattr(side, name, DUMMY)
+    pp.listKind = 1
+    return pc, pp
+
+
+def test_no_alloc_rejects_more_buckets_than_its_queue_holds(lib):
+    pc, pp = _noalloc_launches(gpu.NOALLOC_MAX_BUCKETS + 1)
+    assert lib.changaBatchedGravityNoAlloc(ctypes.byref(pc), ctypes.byref(pp), None, None) != 0
+    assert "CHANGA_NOALLOC_MAX_BUCKETS" in _last(lib)[1]
+
+
+def test_no_alloc_rejects_the_costliest_first_order(lib):
+    pc, pp = _noalloc_launches(4)
+    o = gpu.ChangaBatchOptions(64, 256, 1, 2, 0, 0.0)
+    assert lib.changaBatchedGravityNoAlloc(ctypes.byref(pc), ctypes.byref(pp), ctypes.byref(o),
+                                           None) != 0
+    assert "tree order" in _last(lib)[1]
