@@ -9,7 +9,8 @@ into TreePieceCellListDataTransfer{Local,Remote,RemoteResume} and
 TreePiecePartListDataTransfer{Local,LocalSmallPhase,Remote,RemoteResume}
 requests (the Resume requests carry their own missed nodes / particles,
 HostCUDA.cu:296-341, 517-572). The gathered accelerations must equal the
-single-process device path up to the FP32 summation order of the split."""
+single-process device path up to the FP32 summation order of the split, and
+the FP64 oracle on the same tree and lists within the plain form's bars."""
 import ctypes
 import os
 import socket
@@ -198,11 +199,14 @@ def _worker(rank, port, out_path):
     res[lo:hi] = torch.from_numpy(out)
     dist.all_reduce(res)
     if rank == 0:
+        from tests.helpers import oracle_forces
         ev.zero()
         ev.run_pc()
         ev.run_pp()
         torch.cuda.synchronize()
-        np.savez(out_path, split=res.numpy(), whole=ev.d_vars.cpu().numpy())
+        oa, op, od = oracle_forces(ev)
+        np.savez(out_path, split=res.numpy(), whole=ev.d_vars.cpu().numpy(), oacc=oa, opot=op,
+                 odtg=od)
     dist.destroy_process_group()
 
 
@@ -219,3 +223,14 @@ def test_gpu_remote_chunks_two_processes(device, tmp_path):
     print(f"remote split: acc max {acc_err.max():.2e} pot max {pot_err.max():.2e}", flush=True)
     assert acc_err.max() < 1e-5 and pot_err.max() < 1e-5
     np.testing.assert_array_equal(split[:, 4], whole[:, 4])  # dtGrav: a max of the same terms
+    # and against the FP64 oracle (nodeBucketForce + partBucketForce) on the
+    # same tree and lists, which the split requests must add up to
+    from tests.helpers import rel_err, summary
+    ea, ep = rel_err(split[:, :3], d["oacc"]), rel_err(split[:, 3], d["opot"])
+    ed = rel_err(split[:, 4], d["odtg"])
+    print(f"remote split vs oracle: acc {summary(ea)} pot {summary(ep)} dtGrav {summary(ed)}",
+          flush=True)
+    # bars at about 1.5x the measured tails (acc median 8.1e-8, max 1.5e-6;
+    # pot 3.5e-8, 1.9e-7; dtGrav 3.5e-7, 5.4e-6)
+    for e, med, mx in ((ea, 1.3e-7, 2.5e-6), (ep, 5.5e-8, 3e-7), (ed, 5.5e-7, 8.5e-6)):
+        assert np.median(e) <= med and e.max() <= mx, summary(e)
