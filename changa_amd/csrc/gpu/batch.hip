@@ -48,7 +48,7 @@ struct BatchArgs {
 template <int PCT, int PPT>
 struct BatchLds {
   static constexpr size_t pc = (size_t)PCT * kPcRec * sizeof(float);
-  static constexpr size_t pp = sizeof(PpLds<PPT>);
+  static constexpr size_t pp = sizeof(PpFwdLds<PPT>);
   static constexpr size_t red = 4 * kWave * sizeof(double) + kWave * sizeof(float);
   static constexpr size_t bytes = pc > pp ? (pc > red ? pc : red) : (pp > red ? pp : red);
   static constexpr size_t f4 = (bytes + 15) / 16;
@@ -65,7 +65,7 @@ __device__ __forceinline__ void batch_bucket(const BatchArgs& a, int b, int lane
   bucket_centre<PRECISE>(a.pc.tpos64, bstart, n, cx0, cy0, cz0);
   double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(lds);
   float* redi = reinterpret_cast<float*>(reinterpret_cast<double*>(lds) + 4 * kWave);
-  PpLds<PPT>& s = *reinterpret_cast<PpLds<PPT>*>(lds);
+  PpFwdLds<PPT>& s = *reinterpret_cast<PpFwdLds<PPT>*>(lds);
   for (int t0 = 0; t0 < n; t0 += kWave) {
     const Lanes l = lanes_for(lane, min(kWave, n - t0));
     const Target t = load_target<PRECISE>(a.pc, bstart + t0 + (l.live ? l.lane - l.slot * l.nt : 0),

@@ -44,7 +44,7 @@ template <int PCT, int PPT>
 struct FusedLds {
   static constexpr size_t stack = kCoopStack * sizeof(unsigned);
   static constexpr size_t pc = (size_t)PCT * kPcRec * sizeof(float);
-  static constexpr size_t pp = sizeof(PpLds<PPT>);
+  static constexpr size_t pp = sizeof(PpFwdLds<PPT>);
   static constexpr size_t red = 4 * kWave * sizeof(double) + kWave * sizeof(float);
   static constexpr size_t m1 = stack > pc ? stack : pc;
   static constexpr size_t m2 = pp > red ? pp : red;
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fu
   unsigned* stack = reinterpret_cast<unsigned*>(lds);
   double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(lds);
   float* redi = reinterpret_cast<float*>(reinterpret_cast<double*>(lds) + 4 * kWave);
-  PpLds<PPT>& ps = *reinterpret_cast<PpLds<PPT>*>(lds);
+  PpFwdLds<PPT>& ps = *reinterpret_cast<PpFwdLds<PPT>*>(lds);
   const int lane = threadIdx.x;
   // this wavefront's scratch: a.pc.list / a.pp.list are the scratch bases
   const int cbase = blockIdx.x * a.maxCells, rbase = blockIdx.x * a.maxRuns;

@@ -42,9 +42,11 @@ __global__ __launch_bounds__(64) void pc_kernel(ChangaGravityLaunch a) {
 // ---------------------------------------------------------------------------
 template <bool PRECISE, int KIND>
 __global__ __launch_bounds__(64) void pp_kernel(ChangaGravityLaunch a) {
-  __shared__ PpLds<kPpTile> s;
-  __shared__ double red[4][kWave];
-  __shared__ float redi[kWave];
+  // staged tiles; reused for the slot reduction at the end of each chunk
+  __shared__ PpFwdLds<kPpTile> s;
+  static_assert(sizeof(s) >= 4 * kWave * sizeof(double) + kWave * sizeof(float), "reduction");
+  double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(&s);
+  float* redi = reinterpret_cast<float*>(reinterpret_cast<double*>(&s) + 4 * kWave);
 
   const int lane = threadIdx.x;
   const int b = xcd_bucket(blockIdx.x, a.numBuckets);

@@ -49,6 +49,14 @@ struct Acc32 {
   float ax, ay, az, pot, idt;
 };
 
+// max of two non-negative floats (dtGrav terms: masses and 1/r^3 are
+// non-negative) on their bit patterns, which order like the values: one
+// integer max, with no NaN canonicalisation of the loop-carried operand.
+__device__ __forceinline__ float max_nonneg(float a, float b) {
+  return __builtin_bit_cast(float, max(__builtin_bit_cast(unsigned, a),
+                                       __builtin_bit_cast(unsigned, b)));
+}
+
 // Hernquist-Katz spline (gravity.h:147-182) in FP32; returns potential kernel
 // a and force kernel b for separation^2 r2 and softening sum twoh.
 __device__ __forceinline__ void spline32(float r2, float twoh, float& a, float& b) {
@@ -84,25 +92,26 @@ __device__ __forceinline__ void pp_interact(float rx, float ry, float rz, float 
     A.ay += ry * bm;
     A.az += rz * bm;
     A.pot -= ms * a;
-    A.idt = fmaxf(A.idt, (mt + ms) * b);
+    A.idt = max_nonneg(A.idt, __builtin_fmaf(mt, b, bm));
   }
 }
 
 // Hexadecapole cell at r = tgt - cm from a staged record (see the enum).
-// With n = r/|r|, V_n = Q_n . n^(n-1) follows from the rank-(n-1) harmonic
-// polynomials of n (the gradients of the harmonic polynomials Q_n . n^n, z
-// eliminated by tracelessness), P_n = V_n . n, and
-//   phi -= dir (m + dir^2 (P2/2 + dir (P3/3 + dir P4/4)))
-//   a   += dir^4 (V2 + dir (V3 + dir V4)) - dir^2 S n,
-//   S = m + dir^2 (5/2 P2 + dir (7/3 P3 + dir 9/4 P4)).
+// With the harmonic polynomials of r (not of the unit vector: the powers of
+// 1/|r| are applied at the end), V_n = Q_n . r^(n-1) (the gradients of the
+// harmonic polynomials Q_n . r^n, z eliminated by tracelessness),
+// P_n = V_n . r, d = 1/|r|, and
+//   phi -= d (m + d^4 (P2/2 + d^2 (P3/3 + d^2 P4/4)))
+//   a   += d^5 (V2 + d^2 (V3 + d^2 V4)) - d^3 S r,
+//   S = m + d^4 (5/2 P2 + d^2 (7/3 P3 + d^2 9/4 P4)).
 // The record's float4 k is rec[k * STRIDE] (STRIDE 1: records stored whole;
 // STRIDE = tile width: component planes).
 template <int STRIDE = 1>
-__device__ __forceinline__ void pc_interact(float rx, float ry, float rz, const float4* rec,
+__device__ __forceinline__ void pc_interact(float x, float y, float z, const float4* rec,
                                             float mt, Acc32& A) {
-  float r2 = rx * rx + ry * ry + rz * rz;
+  const float r2 = x * x + y * y + z * z;
   if (r2 == 0.0f) return;
-  float dir = rsq(r2);
+  const float dir = rsq(r2);
   const float4 c0 = rec[0], c1 = rec[STRIDE], c2 = rec[2 * STRIDE], c3 = rec[3 * STRIDE];
   const float4 c4 = rec[4 * STRIDE], c5 = rec[5 * STRIDE], c6 = rec[6 * STRIDE],
                c7 = rec[7 * STRIDE];
@@ -112,7 +121,6 @@ __device__ __forceinline__ void pc_interact(float rx, float ry, float rz, const 
               qxyz = c4.x, q3a = c4.y, q3b = c4.z;
   const float qxxxx = c4.w, qxyyy = c5.x, qxxxy = c5.y, qyyyy = c5.z, qxxxz = c5.w, qyyyz = c6.x,
               qxxyy = c6.y, qxxyz = c6.z, qxyyz = c6.w, q4a = c7.x, q4b = c7.y, q4c = c7.z;
-  const float x = rx * dir, y = ry * dir, z = rz * dir;
   const float x2 = x * x, y2 = y * y, z2 = z * z;
   // rank-2 harmonic polynomials
   const float fxx = x2 - z2, fyy = y2 - z2;
@@ -124,15 +132,15 @@ __device__ __forceinline__ void pc_interact(float rx, float ry, float rz, const 
   const float fxyy = (3.0f * x) * fyy, fxxy = (3.0f * y) * fxx;
   const float fxxz = z * (3.0f * x2 - z2), fyyz = z * (3.0f * y2 - z2);
   const float fxyz = (3.0f * z) * fxy;
-  // V2 = Q2 . n
+  // V2 = Q2 . r
   const float v2x = qxx * x + qxy * y + qxz * z;
   const float v2y = qxy * x + qyy * y + qyz * z;
   const float v2z = qxz * x + qyz * y + qzz * z;
-  // V3 = Q3 . n n
+  // V3 = Q3 . r r
   const float v3x = qxxx * fxx + qxyy * fyy + qxxy * fxy + qxxz * fxz + qxyz * fyz;
   const float v3y = qxyy * fxy + qxxy * fxx + qyyy * fyy + qyyz * fyz + qxyz * fxz;
   const float v3z = qxxz * fxx + qyyz * fyy + qxyz * fxy + q3a * fxz + q3b * fyz;
-  // V4 = Q4 . n n n
+  // V4 = Q4 . r r r
   const float v4x = qxxxx * fxxx + qxxyy * fxyy + qxxxy * fxxy + qxyyy * fyyy + qxxxz * fxxz +
                     qxyyz * fyyz + qxxyz * fxyz;
   const float v4y = qyyyy * fyyy + qxxyy * fxxy + qxxxy * fxxx + qxyyy * fxyy + qxyyz * fxyz +
@@ -142,15 +150,18 @@ __device__ __forceinline__ void pc_interact(float rx, float ry, float rz, const 
   const float p2 = v2x * x + v2y * y + v2z * z;
   const float p3 = v3x * x + v3y * y + v3z * z;
   const float p4 = v4x * x + v4y * y + v4z * z;
-  const float d2 = dir * dir;
-  const float pin = m + d2 * (0.5f * p2 + dir * ((1.0f / 3.0f) * p3 + (0.25f * dir) * p4));
-  const float S = m + d2 * (2.5f * p2 + dir * ((7.0f / 3.0f) * p3 + (2.25f * dir) * p4));
-  const float e = d2 * d2, f = S * d2;
-  A.pot -= dir * pin;
-  A.ax += e * (v2x + dir * (v3x + dir * v4x)) - f * x;
-  A.ay += e * (v2y + dir * (v3y + dir * v4y)) - f * y;
-  A.az += e * (v2z + dir * (v3z + dir * v4z)) - f * z;
-  A.idt = fmaxf(A.idt, (mt + m) * (d2 * dir));
+  const float d2 = dir * dir, d4 = d2 * d2;
+  const float pin = m + d4 * (0.5f * p2 + d2 * ((1.0f / 3.0f) * p3 + (0.25f * d2) * p4));
+  const float S = m + d4 * (2.5f * p2 + d2 * ((7.0f / 3.0f) * p3 + (2.25f * d2) * p4));
+  const float d3 = d2 * dir, d5 = d3 * d2, d3S = d3 * S;
+  A.pot = __builtin_fmaf(-dir, pin, A.pot);
+  A.ax = __builtin_fmaf(d5, v2x + d2 * (v3x + d2 * v4x), A.ax);
+  A.ay = __builtin_fmaf(d5, v2y + d2 * (v3y + d2 * v4y), A.ay);
+  A.az = __builtin_fmaf(d5, v2z + d2 * (v3z + d2 * v4z), A.az);
+  A.ax = __builtin_fmaf(-d3S, x, A.ax);
+  A.ay = __builtin_fmaf(-d3S, y, A.ay);
+  A.az = __builtin_fmaf(-d3S, z, A.az);
+  A.idt = max_nonneg(A.idt, (mt + m) * d3);
 }
 
 // Slot reduction in a fixed order and the += into the result array.
@@ -374,21 +385,26 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
     load_cell<PRECISE, STREAM>(a, cells, base + TILE + l.lane, stager ? end : 0, w);
     if (l.live) {
       Acc32 A = {0.f, 0.f, 0.f, 0.f, acc.idt};
+      // cell slot + j nslot: full rounds over every slot, then one partial
+      const int full = nc / l.nslot;
+      const float4* r4 = rec4 + l.slot;
       if (!soft_tile) {
-        for (int c = l.slot; c < nc; c += l.nslot) {
-          const float4* r4 = rec4 + c;
-          const float4 h = r4[0];
-          pc_interact<TILE>(t.x - h.x, t.y - h.y, t.z - h.z, r4, t.m, A);
-        }
+        auto cell = [&](const float4* r) {
+          const float4 h = r[0];
+          pc_interact<TILE>(t.x - h.x, t.y - h.y, t.z - h.z, r, t.m, A);
+        };
+        for (int j = 0; j < full; ++j, r4 += l.nslot) cell(r4);
+        if (l.slot < nc - full * l.nslot) cell(r4);
       } else {
-        for (int c = l.slot; c < nc; c += l.nslot) {
-          const float4* r4 = rec4 + c;
-          const float4 h = r4[0];
+        auto cell = [&](const float4* r) {
+          const float4 h = r[0];
           if (h.w < 0.0f)  // softened cell: spline monopole at the cm
-            pp_interact(h.x - t.x, h.y - t.y, h.z - t.z, -h.w, r4[TILE].x, t.m, t.soft, A);
+            pp_interact(h.x - t.x, h.y - t.y, h.z - t.z, -h.w, r[TILE].x, t.m, t.soft, A);
           else
-            pc_interact<TILE>(t.x - h.x, t.y - h.y, t.z - h.z, r4, t.m, A);
-        }
+            pc_interact<TILE>(t.x - h.x, t.y - h.y, t.z - h.z, r, t.m, A);
+        };
+        for (int j = 0; j < full; ++j, r4 += l.nslot) cell(r4);
+        if (l.slot < nc - full * l.nslot) cell(r4);
       }
       acc.ax += A.ax;
       acc.ay += A.ay;
@@ -491,17 +507,184 @@ __device__ __forceinline__ void pp_tiles(const ChangaGravityLaunch& a, int start
   }
 }
 
+// ---------------------------------------------------------------------------
+// Forward particle-particle loop (partBucketForce) with far and near sources
+// kept apart. A source is far from the chunk when its distance from the
+// chunk's centre exceeds the chunk's radius plus both softenings (with a
+// relative margin of 2e-4, far above FP32 rounding): no pair of it with a
+// target of the chunk can need the spline or coincide, so its pairs run a
+// Newtonian loop with no per-pair test. Near sources run the general loop.
+//
+// LDS of a staged tile: rec[kPpFwdCap] float4 (x, y, z, m), far sources from
+// the front, near sources from the back (near source k at rec[cap - 1 - k],
+// its softening at soft[k]). Far sources are consumed slot-major: slot s
+// takes the q consecutive records [s q, s q + q) with q uniform and even,
+// the records past the far count holding null sources (m = 0 at 1e18: b
+// underflows to 0, so they add exactly nothing), so the loop runs a fixed
+// trip count with immediate-offset LDS reads.
+// ---------------------------------------------------------------------------
+template <int TILE>
+struct PpFwdLds {
+  static constexpr int cap = TILE + 2 * kWave;  // far padding is < 2 nslot <= 2 kWave
+  float4 rec[cap];
+  float soft[TILE];
+  int2 map[TILE];  // ILPart runs expanded: (source particle, offsetID) per tile slot
+};
+
+// Chunk sphere for the far test: centre (the chunk's first target; the
+// origin of the staged frame in the precision form), radius + the largest
+// target softening, times the margin.
+struct ChunkSphere {
+  float cx, cy, cz, r;
+};
+
+template <bool PRECISE>
+__device__ __forceinline__ ChunkSphere chunk_sphere(const Target& t, const Lanes& l) {
+  ChunkSphere c;
+  if (PRECISE) {
+    c.cx = c.cy = c.cz = 0.0f;  // the bucket centre, first target of the bucket
+  } else {
+    c.cx = __shfl(t.x, 0, kWave);
+    c.cy = __shfl(t.y, 0, kWave);
+    c.cz = __shfl(t.z, 0, kWave);
+  }
+  const float dx = t.x - c.cx, dy = t.y - c.cy, dz = t.z - c.cz;
+  float d2 = l.live ? dx * dx + dy * dy + dz * dz : 0.0f;
+  float sm = l.live ? t.soft : 0.0f;
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    d2 = fmaxf(d2, __shfl_xor(d2, d, kWave));
+    sm = fmaxf(sm, __shfl_xor(sm, d, kWave));
+  }
+  c.r = (sqrtf(d2) + sm) * 1.0002f;
+  return c;
+}
+
+// Newtonian pair at r = src - tgt (the far loop: no softening, r != 0).
+__device__ __forceinline__ void pp_far(const float4 v, const Target& t, Acc32& A) {
+  const float rx = v.x - t.x, ry = v.y - t.y, rz = v.z - t.z;
+  const float r2 = rx * rx + ry * ry + rz * rz;
+  const float ir = rsq(r2);
+  const float b = ir * ir * ir;
+  const float bm = b * v.w;
+  A.ax += rx * bm;
+  A.ay += ry * bm;
+  A.az += rz * bm;
+  A.pot -= v.w * ir;
+  A.idt = max_nonneg(A.idt, __builtin_fmaf(t.m, b, bm));
+}
+
+// Calls tile(ns, src) for each tile of the sources [start, end) of a.list,
+// where src(i) gives (source particle, offsetID) of tile slot i < ns. KIND
+// 0: one ILCell per source particle (the list ChaNGa's
+// GenericList<ILPart>::serialize produces, Compute.cpp:1118-1218); KIND 1:
+// ILPart runs expanded into map: a wavefront prefix sum places each run,
+// and each lane writes its run's slots of the tile.
+template <int KIND, int TILE, bool STREAM, typename Tile>
+__device__ __forceinline__ void pp_source_tiles(const ChangaGravityLaunch& a, int start, int end,
+                                                int lane, int2* map, Tile&& tile) {
+  if (KIND == 0) {
+    const ILCell* lst = static_cast<const ILCell*>(a.list);
+    for (int base = start; base < end; base += TILE) {
+      tile(min(TILE, end - base), [&](int i) {
+        const ILCell c = load_entry<STREAM>(lst + base + i);
+        return make_int2(c.index, c.offsetID);
+      });
+    }
+  } else {
+    const ILPart* lst = static_cast<const ILPart*>(a.list);
+    for (int rb = start; rb < end; rb += kWave) {
+      const int nr = min(kWave, end - rb);
+      int num = 0, first = 0, oid = 0;
+      if (lane < nr) {
+        const ILPart p = load_entry<STREAM>(lst + rb + lane);
+        first = p.index;
+        oid = p.off;
+        num = p.num;
+      }
+      int incl = num;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        int y = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += y;
+      }
+      const int total = __shfl(incl, kWave - 1, kWave);
+      const int begin = incl - num;
+      for (int sb = 0; sb < total; sb += TILE) {
+        const int ns = min(TILE, total - sb);
+        const int j1 = min(incl, sb + ns);
+        for (int j = max(begin, sb); j < j1; ++j) map[j - sb] = make_int2(first + (j - begin), oid);
+        __syncthreads();
+        tile(ns, [&](int i) { return map[i]; });
+      }
+    }
+  }
+}
+
 // Sources [start, end) of a.list onto the chunk's targets (partBucketForce).
 template <bool PRECISE, int KIND, int TILE, bool STREAM = true>
 __device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int start, int end,
                                            double cx0, double cy0, double cz0, const Target& t,
-                                           const Lanes& l, PpLds<TILE>& s, Acc64& acc) {
-  pp_tiles<PRECISE, KIND, TILE, STREAM>(a, start, end, cx0, cy0, cz0, l.lane, s, [&](int ns) {
+                                           const Lanes& l, PpFwdLds<TILE>& s, Acc64& acc) {
+  constexpr int cap = PpFwdLds<TILE>::cap;
+  const float L = a.fperiod;
+  const int lane = l.lane;
+  const ChunkSphere cs = chunk_sphere<PRECISE>(t, l);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  pp_source_tiles<KIND, TILE, STREAM>(a, start, end, lane, s.map, [&](int ns, auto&& src) {
+    // stage: classify each source, far ones to the front, near ones to the back
+    int nfar = 0, nnear = 0;
+    for (int i0 = 0; i0 < ns; i0 += kWave) {
+      const int i = i0 + lane;
+      const bool valid = i < ns;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      float ss = 0.0f;
+      bool far = false;
+      if (valid) {
+        const int2 e = src(i);
+        const int p = e.x, oid = e.y;
+        const CompactPartData q = a.sources[p];
+        if (PRECISE) {
+          v.x = (float)(a.spos64[3 * (size_t)p] + (double)off_x(oid) * (double)L - cx0);
+          v.y = (float)(a.spos64[3 * (size_t)p + 1] + (double)off_y(oid) * (double)L - cy0);
+          v.z = (float)(a.spos64[3 * (size_t)p + 2] + (double)off_z(oid) * (double)L - cz0);
+        } else {
+          v.x = (float)off_x(oid) * L + q.position.x;
+          v.y = (float)off_y(oid) * L + q.position.y;
+          v.z = (float)off_z(oid) * L + q.position.z;
+        }
+        v.w = q.mass;
+        ss = q.soft;
+        const float dx = v.x - cs.cx, dy = v.y - cs.cy, dz = v.z - cs.cz;
+        const float thr = __builtin_fmaf(ss, 1.0002f, cs.r);
+        far = dx * dx + dy * dy + dz * dz > fmaxf(thr * thr, 1e-30f);
+      }
+      const unsigned long long fm = __ballot(far), nm = __ballot(valid && !far);
+      if (far) {
+        s.rec[nfar + __popcll(fm & below)] = v;
+      } else if (valid) {
+        const int k = nnear + __popcll(nm & below);
+        s.rec[cap - 1 - k] = v;
+        s.soft[k] = ss;
+      }
+      nfar += __popcll(fm);
+      nnear += __popcll(nm);
+    }
+    // slot-major far layout: q records per slot, padded with null sources
+    const int q = ((nfar + l.nslot - 1) / l.nslot + 1) & ~1;
+    for (int k = nfar + lane; k < l.nslot * q; k += kWave)
+      s.rec[k] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
+    __syncthreads();
     if (l.live) {
       Acc32 A = {0.f, 0.f, 0.f, 0.f, acc.idt};
-      for (int k = l.slot; k < ns; k += l.nslot) {
-        float4 v = s.src4[k];
-        pp_interact(v.x - t.x, v.y - t.y, v.z - t.z, v.w, s.srcs[k], t.m, t.soft, A);
+      const float4* r = s.rec + l.slot * q;
+      for (int j = 0; j < q; j += 2, r += 2) {
+        pp_far(r[0], t, A);
+        pp_far(r[1], t, A);
+      }
+      for (int k = l.slot; k < nnear; k += l.nslot) {
+        const float4 v = s.rec[cap - 1 - k];
+        pp_interact(v.x - t.x, v.y - t.y, v.z - t.z, v.w, s.soft[k], t.m, t.soft, A);
       }
       acc.ax += A.ax;
       acc.ay += A.ay;
@@ -509,6 +692,7 @@ __device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int sta
       acc.pot += A.pot;
       acc.idt = A.idt;
     }
+    __syncthreads();
   });
 }
 
