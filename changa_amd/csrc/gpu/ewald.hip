@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
                                                          VariablePartData* vars,
                                                          const int* markers, int first, int n,
                                                          const EwtData* ewt, const double* pos64,
-                                                         EwaldParams p) {
+                                                         const double* root, EwaldParams p) {
   // h-loop table: per entry (k, hC) and (h as integers, hS); dynamic size
   extern __shared__ float4 sh[];
   // lattice offsets (n L, hole flag) that some particle of the wavefront needs
@@ -64,19 +64,24 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
     }
   }
 
+  // root centre of mass, k = 0 term and moments: from the device root record
+  // (changaEwaldRoot) when there is one, else from the parameters
+  const double cmx = root ? root[0] : p.cmx, cmy = root ? root[1] : p.cmy,
+               cmz = root ? root[2] : p.cmz;
+  const double mk1 = root ? root[3] * M_PI / (p.alpha2 * p.L * p.L * p.L) : p.mk1;
   const int gi = blockIdx.x * kEwBlock + tid;
   const bool live = gi < n;
   const int id = live ? (markers ? markers[gi] : first + gi) : (markers ? markers[n - 1] : first + n - 1);
   double dx, dy, dz;
   if (pos64) {
-    dx = pos64[3 * (size_t)id] - p.cmx;
-    dy = pos64[3 * (size_t)id + 1] - p.cmy;
-    dz = pos64[3 * (size_t)id + 2] - p.cmz;
+    dx = pos64[3 * (size_t)id] - cmx;
+    dy = pos64[3 * (size_t)id + 1] - cmy;
+    dz = pos64[3 * (size_t)id + 2] - cmz;
   } else {
     CompactPartData q = parts[id];
-    dx = (double)q.position.x - p.cmx;
-    dy = (double)q.position.y - p.cmy;
-    dz = (double)q.position.z - p.cmz;
+    dx = (double)q.position.x - cmx;
+    dy = (double)q.position.y - cmy;
+    dz = (double)q.position.z - cmz;
   }
   // wavefront bounding box of d (dead lanes hold a live lane's particle)
   float lo[3] = {(float)dx, (float)dy, (float)dz}, hi[3] = {lo[0], lo[1], lo[2]};
@@ -129,11 +134,11 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
     float* qf = &Qf.m;
 #pragma unroll
     for (int k = 0; k < 22; ++k) {
-      qd[k] = p.q[k];
-      qf[k] = (float)p.q[k];
+      qd[k] = root ? root[4 + k] : p.q[k];
+      qf[k] = (float)qd[k];
     }
   }
-  double pot = p.mk1, ax = 0, ay = 0, az = 0;
+  double pot = mk1, ax = 0, ay = 0, az = 0;
   // central replica in FP64
   {
     double g[6];
@@ -228,10 +233,13 @@ __global__ __launch_bounds__(kEwBlock) void ewald_kernel(const CompactPartData* 
   }
 }
 
-// Builds the kernel parameters from the reference's read-only record.
+// Builds the kernel parameters from the reference's read-only record; with
+// d_root (a device record of changaEwaldRoot) the root's centre of mass, mass
+// and moments come from there instead of ro and rootCm64.
 int launch_ewald(const CompactPartData* parts, VariablePartData* vars, const int* markers,
                  int first, int n, const EwaldReadOnlyData* ro, const EwtData* d_ewt,
-                 const double* pos64, const double* rootCm64, hipStream_t s) {
+                 const double* pos64, const double* rootCm64, const double* d_root,
+                 hipStream_t s) {
   if (n <= 0) return 0;
   EwaldParams p = make_params(ro, rootCm64);
   if (p.nEwReps > 3)
@@ -239,7 +247,114 @@ int launch_ewald(const CompactPartData* parts, VariablePartData* vars, const int
   size_t shm = sizeof(float4) * 2 * (size_t)(p.nH > 0 ? p.nH : 1);
   int blocks = (n + kEwBlock - 1) / kEwBlock;
   hipLaunchKernelGGL(ewald_kernel, dim3(blocks), dim3(kEwBlock), shm, s, parts, vars, markers,
-                     first, n, d_ewt, pos64, p);
+                     first, n, d_ewt, pos64, d_root, p);
+  CGPU_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// EwaldInit on the device (Ewald.cpp:285-383): the root record (centre of
+// mass, mass, the 22 reduced moments brought to unit scale, the first 22
+// MOMC components of momRescaleFmomr + momFmomr2Momc) and the h-loop table,
+// from the root's node record node64[0] (cm.xyz, radius, soft, mass,
+// lo.xyz, hi.xyz, 22 radius-scaled reduced moments). One block: each thread
+// takes one lattice vector h of the cube |h_i| <= hreps, the valid ones
+// (0 < |h|^2 <= hcut^2) are numbered in the host table's order (hx
+// outermost, hz innermost) by a block scan, and each writes its entry:
+//   k = 2 pi h / L,  w = exp(-k^2 / 4 alpha^2) / (pi h^2 L),  alpha = 2 / L,
+//   hC = -w (M - (Q2:kk)/2 + (Q4:kkkk)/24),  hS = w (Q3:kkk)/6.
+// ---------------------------------------------------------------------------
+constexpr int kRootBlock = 512;
+
+__global__ __launch_bounds__(kRootBlock) void ewald_root_kernel(const double* node64, double L,
+                                                                double hcut2, int hreps,
+                                                                double* root, EwtData* ewt) {
+  __shared__ double q[22];
+  __shared__ int wave_count[kRootBlock / kWave];
+  __shared__ int base;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  if (tid < 22) {
+    const int order = tid == 0 ? 0 : (tid < 6 ? 2 : (tid < 13 ? 3 : 4));
+    const double u = node64[3];
+    double s = 1.0;
+    for (int k = 0; k < order; ++k) s *= u;
+    q[tid] = node64[12 + tid] * s;
+    root[4 + tid] = q[tid];
+  }
+  if (tid < 3) root[tid] = node64[tid];
+  if (tid == 3) root[3] = node64[5];
+  if (tid == 0) base = 0;
+  __syncthreads();
+  const double m = q[0], xx = q[1], yy = q[2], xy = q[3], xz = q[4], yz = q[5];
+  const double zz = -(xx + yy);
+  const double xxx = q[6], xyy = q[7], xxy = q[8], yyy = q[9], xxz = q[10], yyz = q[11],
+               xyz = q[12];
+  const double xzz = -(xxx + xyy), yzz = -(xxy + yyy), zzz = -(xxz + yyz);
+  const double xxxx = q[13], xyyy = q[14], xxxy = q[15], yyyy = q[16], xxxz = q[17],
+               yyyz = q[18], xxyy = q[19], xxyz = q[20], xyyz = q[21];
+  const double xxzz = -(xxxx + xxyy), xyzz = -(xxxy + xyyy), xzzz = -(xxxz + xyyz),
+               yyzz = -(xxyy + yyyy), yzzz = -(xxyz + yyyz), zzzz = -(xxzz + yyzz);
+  const double alpha = 2.0 / L;
+  const double k4 = M_PI * M_PI / (alpha * alpha * L * L);
+  const double f = 2.0 * M_PI / L;
+  const int W = 2 * hreps + 1, NC = W * W * W;
+  for (int c0 = 0; c0 < NC; c0 += kRootBlock) {
+    const int c = c0 + tid;
+    const int hx = c / (W * W) - hreps, hy = (c / W) % W - hreps, hz = c % W - hreps;
+    const int h2 = hx * hx + hy * hy + hz * hz;
+    const bool valid = c < NC && h2 != 0 && (double)h2 <= hcut2;
+    const unsigned long long mask = __ballot(valid);
+    if (lane == 0) wave_count[wv] = __popcll(mask);
+    __syncthreads();
+    int idx = base + __popcll(mask & ((1ull << lane) - 1ull));
+    for (int k = 0; k < wv; ++k) idx += wave_count[k];
+    if (valid) {
+      const double x = f * hx, y = f * hy, z = f * hz;
+      const double w = exp(-k4 * h2) / (M_PI * h2 * L);
+      const double q2 = xx * x * x + yy * y * y + zz * z * z +
+                        2 * (xy * x * y + xz * x * z + yz * y * z);
+      const double q3 = xxx * x * x * x + yyy * y * y * y + zzz * z * z * z +
+                        3 * (xxy * x * x * y + xxz * x * x * z + xyy * x * y * y +
+                             yyz * y * y * z + xzz * x * z * z + yzz * y * z * z) +
+                        6 * xyz * x * y * z;
+      const double q4 = xxxx * x * x * x * x + yyyy * y * y * y * y + zzzz * z * z * z * z +
+                        4 * (xxxy * x * x * x * y + xxxz * x * x * x * z + xyyy * x * y * y * y +
+                             yyyz * y * y * y * z + xzzz * x * z * z * z + yzzz * y * z * z * z) +
+                        6 * (xxyy * x * x * y * y + xxzz * x * x * z * z + yyzz * y * y * z * z) +
+                        12 * (xxyz * x * x * y * z + xyyz * x * y * y * z + xyzz * x * y * z * z);
+      EwtData e;
+      e.hx = (float)x;
+      e.hy = (float)y;
+      e.hz = (float)z;
+      e.hCfac = (float)(-w * (m - 0.5 * q2 + q4 / 24.0));
+      e.hSfac = (float)(w * q3 / 6.0);
+      ewt[idx] = e;
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (int k = 0; k < kRootBlock / kWave; ++k) base += wave_count[k];
+    __syncthreads();
+  }
+}
+
+int launch_ewald_root(const double* node64, const EwaldReadOnlyData* ro, double hcut,
+                      double* root, EwtData* ewt, hipStream_t s) {
+  if (!node64 || !ro || !root || (!ewt && ro->nEwhLoop > 0) || !(hcut >= 0.0) || ro->L <= 0.0f)
+    return fail("changaEwaldRoot: missing argument", hipErrorInvalidValue, __FILE__, __LINE__);
+  const int hreps = (int)ceil(hcut);
+  const double hcut2 = hcut * hcut;
+  int count = 0;
+  for (int hx = -hreps; hx <= hreps; ++hx)
+    for (int hy = -hreps; hy <= hreps; ++hy)
+      for (int hz = -hreps; hz <= hreps; ++hz) {
+        const int h2 = hx * hx + hy * hy + hz * hz;
+        count += h2 != 0 && (double)h2 <= hcut2;
+      }
+  if (count != ro->nEwhLoop)
+    return fail("changaEwaldRoot: ro->nEwhLoop is not the number of h with 0 < |h|^2 <= hCut^2",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  hipLaunchKernelGGL(ewald_root_kernel, dim3(1), dim3(kRootBlock), 0, s, node64, (double)ro->L,
+                     hcut2, hreps, root, ewt);
   CGPU_CHECK(hipGetLastError());
   return 0;
 }

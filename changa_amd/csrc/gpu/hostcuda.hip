@@ -19,7 +19,10 @@ namespace cgpu {
 
 int launch_ewald(const CompactPartData* parts, VariablePartData* vars, const int* markers,
                  int first, int n, const EwaldReadOnlyData* ro, const EwtData* d_ewt,
-                 const double* pos64, const double* rootCm64, hipStream_t s);
+                 const double* pos64, const double* rootCm64, const double* d_root,
+                 hipStream_t s);
+int launch_ewald_root(const double* node64, const EwaldReadOnlyData* ro, double hcut,
+                      double* root, EwtData* ewt, hipStream_t s);
 
 static std::atomic<int> g_abort{1};
 static std::atomic<changa_gpu_callback_fn> g_handler{nullptr};
@@ -308,7 +311,7 @@ static int ewald_host(CompactPartData* parts, VariablePartData* vars, EwaldData*
       return e;
   if (int e = upload(s, S_EWT, h->ewt, sizeof(EwtData) * (size_t)ro->nEwhLoop, &dewt)) return e;
   if (int e = launch_ewald(parts, vars, (const int*)dmark, largephase ? 0 : h->EwaldRange[0], n, ro,
-                           (const EwtData*)dewt, nullptr, nullptr, s))
+                           (const EwtData*)dewt, nullptr, nullptr, nullptr, s))
     return e;
   return add_callback(s, cb);
 }
@@ -325,7 +328,23 @@ int changaEwald(const CompactPartData* parts, VariablePartData* vars, const int*
   if (n == 0) return 0;  // no active particles
   if (!ro || !ewt || !vars || (!parts && !pos64))
     return fail("changaEwald: missing argument", hipErrorInvalidValue, __FILE__, __LINE__);
-  return launch_ewald(parts, vars, markers, first, n, ro, ewt, pos64, rootCm64,
+  return launch_ewald(parts, vars, markers, first, n, ro, ewt, pos64, rootCm64, nullptr,
+                      (hipStream_t)stream);
+}
+
+int changaEwaldRoot(const double* node64, const EwaldReadOnlyData* ro, double hCut, double* root,
+                    EwtData* ewt, void* stream) {
+  return launch_ewald_root(node64, ro, hCut, root, ewt, (hipStream_t)stream);
+}
+
+int changaEwaldResident(const CompactPartData* parts, VariablePartData* vars, const int* markers,
+                        int first, int n, const EwaldReadOnlyData* ro, const EwtData* ewt,
+                        const double* pos64, const double* root, void* stream) {
+  if (n == 0) return 0;  // no active particles
+  if (!ro || !ewt || !vars || !root || (!parts && !pos64))
+    return fail("changaEwaldResident: missing argument", hipErrorInvalidValue, __FILE__,
+                __LINE__);
+  return launch_ewald(parts, vars, markers, first, n, ro, ewt, pos64, nullptr, root,
                       (hipStream_t)stream);
 }
 

@@ -337,6 +337,45 @@ class Evaluation:
         if self.w["periodic"]:
             self._setup_ewald()
 
+    def move(self, pos):
+        """New particle positions (tree order, [n, 3] FP64) on the same tree
+        topology, interaction lists and device buffers (every address a plan
+        or launch record holds stays valid). The next step's multipole build,
+        Ewald root record and forces use them; the lists are not rebuilt
+        (build_lists does that for GPU-built lists)."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        if pos.shape != (self.n, 3):
+            raise ValueError(f"pos must be [{self.n}, 3] in tree order")
+        d = torch.from_numpy(pos).to(self.device)
+        self.d_pos64.copy_(d)
+        self.d_parts[:, 2:5].copy_(d.to(torch.float32))  # CompactPartData {mass, soft, x, y, z}
+        if self.t_index is not None:
+            ti = self.d_tindex.long()
+            self.d_tpos64.copy_(self.d_pos64[ti])
+            self.d_targets.copy_(self.d_parts[ti])
+        if self._dev is not None:
+            self._dev["pos"] = self.d_pos64
+        self._host["pos"] = pos
+        self._host_root_stale = True
+        self.ewald_count_real = None
+
+    def _refresh_host_root(self):
+        """The host copies of the Ewald root (centre of mass, mass, MOMC) from
+        the device root record, after moves (the adjoint's Ewald term reads
+        them)."""
+        if not (self.w["periodic"] and getattr(self, "_host_root_stale", False)):
+            return
+        r = self.d_ewald_root.cpu().numpy()
+        self.root_cm64 = np.ascontiguousarray(r[0:3])
+        ro = self.ewald_ro
+        ro.mm.totalMass = r[3]
+        ro.mm.cmx, ro.mm.cmy, ro.mm.cmz = r[0:3]
+        q = r[4:26]
+        mc = host.root_momc(q, 1.0)
+        for f, v in zip(gpu.MOMC_FIELDS, mc):
+            setattr(ro.momcRoot, f, float(v))
+        self._host_root_stale = False
+
     def _upload_host_lists(self):
         L = self.t_lists
         if L["cell_marks"][-1] >= 2 ** 31:
@@ -492,7 +531,20 @@ class Evaluation:
         ro.fInner2 = 1.1e-2 * L * L
         self.ewald_ro = ro
         self.d_ewt = self._t(self.ewt64.astype(np.float32))
+        self.d_ewald_root = torch.empty(gpu.EWALD_ROOT_DOUBLES, dtype=torch.float64,
+                                        device=self.device)
+        self.ewald_root()
         self.ewald_count_real = None
+
+    def ewald_root(self):
+        """The Ewald root record and h-loop table rebuilt on the device from
+        the last moment build's root node (changaEwaldRoot): the Ewald pass of
+        a step on moved particles uses the step's own root multipole."""
+        if not self.w["periodic"]:
+            return
+        gpu.check(gpu.lib().changaEwaldRoot(_dp(self.d_node64), ctypes.byref(self.ewald_ro),
+                                            float(self.w["ewh_cut"]), _dp(self.d_ewald_root),
+                                            _dp(self.d_ewt), self._sp()), "changaEwaldRoot")
 
     def ewald_counts(self):
         if not self.w["periodic"]:
@@ -545,26 +597,35 @@ class Evaluation:
         a = self._launch(False)
         gpu.check(gpu.lib().changaPartGravity(ctypes.byref(a), self._sp()), "changaPartGravity")
 
-    def run_ewald(self, vars_=None, stream=None):
+    def run_ewald(self, vars_=None, stream=None, host_record=False):
         """The Ewald correction added into vars_ (default: the results) on
-        stream (default: the evaluation's)."""
+        stream (default: the evaluation's), with the device root record of
+        the last moment build; host_record: with the host EwaldReadOnlyData
+        record instead (changaEwald, the record EwaldHost takes)."""
         if not self.w["periodic"]:
             return
         pos64 = _dp(self.d_tpos64) if self.precise else None
-        cm = self.root_cm64.ctypes.data_as(ctypes.c_void_p) if self.precise else None
         sp = self._sp() if stream is None else ctypes.c_void_p(stream.cuda_stream)
-        rc = gpu.lib().changaEwald(_dp(self.d_targets), _dp(self.d_vars if vars_ is None else vars_),
-                                   _dp(self.ew_markers) if self.ew_markers is not None else None,
-                                   0, self.ewald_ro.n, ctypes.byref(self.ewald_ro), _dp(self.d_ewt),
-                                   pos64, cm, sp)
-        gpu.check(rc, "changaEwald")
+        if host_record:
+            self._refresh_host_root()
+            cm = self.root_cm64.ctypes.data_as(ctypes.c_void_p) if self.precise else None
+            rc = gpu.lib().changaEwald(
+                _dp(self.d_targets), _dp(self.d_vars if vars_ is None else vars_),
+                _dp(self.ew_markers) if self.ew_markers is not None else None, 0,
+                self.ewald_ro.n, ctypes.byref(self.ewald_ro), _dp(self.d_ewt), pos64, cm, sp)
+            gpu.check(rc, "changaEwald")
+            return
+        rc = gpu.lib().changaEwaldResident(
+            _dp(self.d_targets), _dp(self.d_vars if vars_ is None else vars_),
+            _dp(self.ew_markers) if self.ew_markers is not None else None, 0, self.ewald_ro.n,
+            ctypes.byref(self.ewald_ro), _dp(self.d_ewt), pos64, _dp(self.d_ewald_root), sp)
+        gpu.check(rc, "changaEwaldResident")
 
     def _ewald_begin(self):
         """Start the Ewald correction on a second stream into its own result
-        array, so that it overlaps the multipole build and the tree forces;
-        _ewald_end merges it. It uses only the targets' positions and the
-        root multipole (fixed at set-up), so it does not wait for the
-        moments."""
+        array, so that it overlaps the tree forces; _ewald_end merges it. It
+        waits for the work enqueued so far on the evaluation's stream (the
+        moment build and the device root record it reads)."""
         if getattr(self, "_ew_stream", None) is None:
             self._ew_stream = torch.cuda.Stream(self.device)  # default priority
             self._ew_vars = torch.empty_like(self.d_vars)
@@ -585,9 +646,11 @@ class Evaluation:
                                           self._sp()), "changaAddVars")
 
     def step(self, moments=True):
-        """One force evaluation: multipole build, zero, PC, PP, Ewald."""
+        """One force evaluation: multipole build (and the Ewald root record
+        from it), zero, PC, PP, Ewald."""
         if moments:
             self.build_moments()
+            self.ewald_root()
         self.zero()
         self.run_pc()
         self.run_pp()
@@ -618,16 +681,18 @@ class Evaluation:
         gpu.check(fn(self._batch_plan, self._sp()), "changaBatchPlan")
 
     def step_batched(self, moments=True, overlap_ewald=True):
-        """One force evaluation in the batched form: multipole build, the
-        plan's graph (zeroing, PC + PP), Ewald. With overlap_ewald (periodic
-        boxes) the Ewald pass runs on a second stream beside the first two
-        and is merged after them; the results are bit-identical (the
-        merge adds the same FP32 values the direct pass adds)."""
+        """One force evaluation in the batched form: multipole build (and the
+        Ewald root record from it), the plan's graph (zeroing, PC + PP),
+        Ewald. With overlap_ewald (periodic boxes) the Ewald pass runs on a
+        second stream beside the graph and is merged after it; the results
+        are bit-identical (the merge adds the same FP32 values the direct
+        pass adds)."""
         overlap = overlap_ewald and self.w["periodic"]
-        if overlap:
-            self._ewald_begin()
         if moments:
             self.build_moments()
+            self.ewald_root()
+        if overlap:
+            self._ewald_begin()
         self.run_batched()
         if overlap:
             self._ewald_end()
@@ -685,10 +750,11 @@ class Evaluation:
         fused walk and force, Ewald (overlapped on a second stream as in
         step_batched)."""
         overlap = overlap_ewald and self.w["periodic"]
-        if overlap:
-            self._ewald_begin()
         if moments:
             self.build_moments()
+            self.ewald_root()
+        if overlap:
+            self._ewald_begin()
         self.zero()
         self.run_fused()
         if overlap:
@@ -749,6 +815,7 @@ class Evaluation:
         if ewald is None:
             ewald = bool(self.w["periodic"])
         if ewald:
+            self._refresh_host_root()
             if not self.w["periodic"]:
                 raise ValueError("the Ewald term needs a periodic box")
             nd = gpu.lib().changaGravityAdjointEwaldWorkDoubles(self.n_targets,

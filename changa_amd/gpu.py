@@ -132,8 +132,11 @@ EXPORTS = [
     "changaGravityAdjoint", "changaGravityAdjointWorkInts", "changaGravityAdjointEwaldWorkDoubles",
     "changaBuildListsWorkBytes", "changaBuildListsCount", "changaBuildListsFill",
     "changaBuildTreeWorkBytes", "changaBuildTree", "changaFusedGravityWorkBytes",
-    "changaFusedGravity", "changaBatchedGravityNoAlloc",
+    "changaFusedGravity", "changaBatchedGravityNoAlloc", "changaEwaldRoot", "changaEwaldResident",
 ]
+
+# doubles of the device Ewald root record of changaEwaldRoot
+EWALD_ROOT_DOUBLES = 26
 
 # buckets the no-allocation batched launch accepts (its static queue's size)
 NOALLOC_MAX_BUCKETS = 1 << 22
@@ -181,6 +184,11 @@ def lib():
     L.changaAddVars.restype = c_int
     L.changaEwald.argtypes = [P, P, P, c_int, c_int, ctypes.POINTER(EwaldReadOnlyData), P, P, P, P]
     L.changaEwald.restype = c_int
+    L.changaEwaldRoot.argtypes = [P, ctypes.POINTER(EwaldReadOnlyData), ctypes.c_double, P, P, P]
+    L.changaEwaldRoot.restype = c_int
+    L.changaEwaldResident.argtypes = [P, P, P, c_int, c_int, ctypes.POINTER(EwaldReadOnlyData), P,
+                                      P, P, P]
+    L.changaEwaldResident.restype = c_int
     L.changaBuildMoments.argtypes = [c_int, c_int, P, P, P, P, P, P, P, P, P, P, P, P]
     L.changaBuildMoments.restype = c_int
     L.changaBatchPlanCreate.argtypes = [ctypes.POINTER(ChangaGravityLaunch),

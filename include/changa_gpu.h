@@ -150,6 +150,24 @@ int changaEwald(const CompactPartData *parts, VariablePartData *vars, const int 
                 int first, int n, const EwaldReadOnlyData *ro, const EwtData *ewt,
                 const double *pos64, const double *rootCm64, void *stream);
 
+/* TreePiece::EwaldInit (Ewald.cpp:285-383) on the device, so that an
+   evaluation on moved particles needs no host round trip between the
+   multipole build and the Ewald pass: from the root's node record
+   (node64[0] of changaBuildMoments) the root record `root` (device,
+   CHANGA_EWALD_ROOT_DOUBLES doubles: centre of mass x, y, z, mass, then the
+   22 reduced moments at unit scale in FMOMR order, as momRescaleFmomr +
+   momFmomr2Momc give them) and the h-loop table `ewt` (device, the
+   ro->nEwhLoop entries with 0 < |h|^2 <= hCut^2, hx outermost, hz innermost;
+   a different ro->nEwhLoop is an error). Uses ro->L. */
+#define CHANGA_EWALD_ROOT_DOUBLES 26
+int changaEwaldRoot(const double *node64, const EwaldReadOnlyData *ro, double hCut, double *root,
+                    EwtData *ewt, void *stream);
+/* changaEwald with the root of changaEwaldRoot (device) in place of
+   ro->mm, ro->momcRoot and rootCm64. */
+int changaEwaldResident(const CompactPartData *parts, VariablePartData *vars, const int *markers,
+                        int first, int n, const EwaldReadOnlyData *ro, const EwtData *ewt,
+                        const double *pos64, const double *root, void *stream);
+
 /* GPU multipole build (the CPU path is GenericTreeNode::makeBucket,
    GenericTreeNode.h:219-257, and LocalTreeBuilder::doneChildren,
    Compute.cpp:2534-2580). Node topology (breadth-first numbering, children

@@ -126,7 +126,7 @@ def test_gpu_mirror_matches_device_path(device, case):
     ev.zero()
     ev.run_pc()
     ev.run_pp()
-    ev.run_ewald()
+    ev.run_ewald(host_record=True)  # the record EwaldHost was given
     torch.cuda.synchronize()
     ref = ev.d_vars.cpu().numpy()
     np.testing.assert_array_equal(out, ref)
