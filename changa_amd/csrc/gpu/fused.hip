@@ -45,7 +45,7 @@ struct FusedLds {
   static constexpr size_t stack = kCoopStack * sizeof(unsigned);
   static constexpr size_t pc = (size_t)PCT * kPcRec * sizeof(float);
   static constexpr size_t pp = sizeof(PpFwdLds<PPT>);
-  static constexpr size_t red = 4 * kWave * sizeof(double) + kWave * sizeof(float);
+  static constexpr size_t red = reduce_bytes<1>();
   static constexpr size_t m1 = stack > pc ? stack : pc;
   static constexpr size_t m2 = pp > red ? pp : red;
   static constexpr size_t bytes = m1 > m2 ? m1 : m2;
@@ -90,8 +90,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fu
     FusedArgs a) {
   __shared__ float4 lds[FusedLds<PCT, PPT>::f4];
   unsigned* stack = reinterpret_cast<unsigned*>(lds);
-  double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(lds);
-  float* redi = reinterpret_cast<float*>(reinterpret_cast<double*>(lds) + 4 * kWave);
   PpFwdLds<PPT>& ps = *reinterpret_cast<PpFwdLds<PPT>*>(lds);
   const int lane = threadIdx.x;
   // this wavefront's scratch: a.pc.list / a.pp.list are the scratch bases
@@ -124,17 +122,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fu
     double cx0, cy0, cz0;
     bucket_centre<true>(a.pc.tpos64, bstart, n, cx0, cy0, cz0);
     for (int t0 = 0; t0 < n; t0 += kWave) {
-      const Lanes l = lanes_for(lane, min(kWave, n - t0));
-      const Target t = load_target<true>(a.pc, bstart + t0 + (l.live ? l.lane - l.slot * l.nt : 0),
-                                         cx0, cy0, cz0);
-      Acc64 acc = {0.0, 0.0, 0.0, 0.0, 0.0f};
+      // one target per lane: the walk's stack and scratch leave no registers
+      // for more
+      const Lanes l = lanes_for<1>(lane, min(kWave, n - t0));
+      Target t[1];
+      load_targets<true, 1>(a.pc, bstart + t0, l, cx0, cy0, cz0, t);
+      Acc64 acc[1] = {{0.0, 0.0, 0.0, 0.0, 0.0f}};
       // the scratch is re-read per chunk and rewritten per bucket: plain loads
       // keep it in L2
-      pc_cells<true, PCT, false>(a.pc, cbase, cbase + nhex + nsoft, cx0, cy0, cz0, t, l, lds,
-                                 acc);
-      pp_sources<true, 1, PPT, false>(a.pp, rbase, rbase + nruns, cx0, cy0, cz0, t, l, ps, acc);
-      reduce_store(red, redi, lane, l.nt, l.nslot, l.slot, acc.ax, acc.ay, acc.az, acc.pot,
-                   acc.idt, a.pc.vars + bstart + t0 + (lane < l.nt ? lane : 0));
+      pc_cells<true, PCT, false, 1>(a.pc, cbase, cbase + nhex + nsoft, cx0, cy0, cz0, t, l, lds,
+                                    acc);
+      pp_sources<true, 1, PPT, false, 1>(a.pp, rbase, rbase + nruns, cx0, cy0, cz0, t, l, ps,
+                                         acc);
+      reduce_store<1>(lds, l, acc, a.pc.vars + bstart + t0);
     }
   }
 }

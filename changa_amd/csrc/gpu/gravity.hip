@@ -15,9 +15,7 @@ template <bool PRECISE>
 __global__ __launch_bounds__(64) void pc_kernel(ChangaGravityLaunch a) {
   // staged tile; reused for the slot reduction at the end of each chunk
   __shared__ float4 rec4[kPcTile * (kPcRec / 4)];
-  static_assert(sizeof(rec4) >= 4 * kWave * sizeof(double) + kWave * sizeof(float), "reduction");
-  double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(rec4);
-  float* redi = reinterpret_cast<float*>(reinterpret_cast<double*>(rec4) + 4 * kWave);
+  static_assert(sizeof(rec4) >= reduce_bytes<3>(), "reduction");
 
   const int lane = threadIdx.x;
   const int b = xcd_bucket(blockIdx.x, a.numBuckets);
@@ -27,13 +25,18 @@ __global__ __launch_bounds__(64) void pc_kernel(ChangaGravityLaunch a) {
   bucket_centre<PRECISE>(a.tpos64, bstart, n, cx0, cy0, cz0);
 
   for (int t0 = 0; t0 < n; t0 += kWave) {
-    const Lanes l = lanes_for(lane, min(kWave, n - t0));
-    const Target t = load_target<PRECISE>(a, bstart + t0 + (l.live ? l.lane - l.slot * l.nt : 0),
-                                          cx0, cy0, cz0);
-    Acc64 acc = {0.0, 0.0, 0.0, 0.0, 0.0f};
-    pc_cells<PRECISE, kPcTile>(a, start, end, cx0, cy0, cz0, t, l, rec4, acc);
-    reduce_store(red, redi, lane, l.nt, l.nslot, l.slot, acc.ax, acc.ay, acc.az, acc.pot, acc.idt,
-                 a.vars + bstart + t0 + (lane < l.nt ? lane : 0));
+    const int nt = min(kWave, n - t0);
+    with_targets_per_lane(nt, [&](auto kt) {
+      constexpr int K = decltype(kt)::value;
+      const Lanes l = lanes_for<K>(lane, nt);
+      Target t[K];
+      load_targets<PRECISE, K>(a, bstart + t0, l, cx0, cy0, cz0, t);
+      Acc64 acc[K];
+#pragma unroll
+      for (int j = 0; j < K; ++j) acc[j] = {0.0, 0.0, 0.0, 0.0, 0.0f};
+      pc_cells<PRECISE, kPcTile, true, K>(a, start, end, cx0, cy0, cz0, t, l, rec4, acc);
+      reduce_store<K>(rec4, l, acc, a.vars + bstart + t0);
+    });
   }
 }
 
@@ -44,9 +47,7 @@ template <bool PRECISE, int KIND>
 __global__ __launch_bounds__(64) void pp_kernel(ChangaGravityLaunch a) {
   // staged tiles; reused for the slot reduction at the end of each chunk
   __shared__ PpFwdLds<kPpTile> s;
-  static_assert(sizeof(s) >= 4 * kWave * sizeof(double) + kWave * sizeof(float), "reduction");
-  double(*red)[kWave] = reinterpret_cast<double(*)[kWave]>(&s);
-  float* redi = reinterpret_cast<float*>(reinterpret_cast<double*>(&s) + 4 * kWave);
+  static_assert(sizeof(s) >= reduce_bytes<3>(), "reduction");
 
   const int lane = threadIdx.x;
   const int b = xcd_bucket(blockIdx.x, a.numBuckets);
@@ -56,13 +57,18 @@ __global__ __launch_bounds__(64) void pp_kernel(ChangaGravityLaunch a) {
   bucket_centre<PRECISE>(a.tpos64, bstart, n, cx0, cy0, cz0);
 
   for (int t0 = 0; t0 < n; t0 += kWave) {
-    const Lanes l = lanes_for(lane, min(kWave, n - t0));
-    const Target t = load_target<PRECISE>(a, bstart + t0 + (l.live ? l.lane - l.slot * l.nt : 0),
-                                          cx0, cy0, cz0);
-    Acc64 acc = {0.0, 0.0, 0.0, 0.0, 0.0f};
-    pp_sources<PRECISE, KIND, kPpTile>(a, start, end, cx0, cy0, cz0, t, l, s, acc);
-    reduce_store(red, redi, lane, l.nt, l.nslot, l.slot, acc.ax, acc.ay, acc.az, acc.pot, acc.idt,
-                 a.vars + bstart + t0 + (lane < l.nt ? lane : 0));
+    const int nt = min(kWave, n - t0);
+    with_targets_per_lane(nt, [&](auto kt) {
+      constexpr int K = decltype(kt)::value;
+      const Lanes l = lanes_for<K>(lane, nt);
+      Target t[K];
+      load_targets<PRECISE, K>(a, bstart + t0, l, cx0, cy0, cz0, t);
+      Acc64 acc[K];
+#pragma unroll
+      for (int j = 0; j < K; ++j) acc[j] = {0.0, 0.0, 0.0, 0.0, 0.0f};
+      pp_sources<PRECISE, KIND, kPpTile, true, K>(a, start, end, cx0, cy0, cz0, t, l, s, acc);
+      reduce_store<K>(&s, l, acc, a.vars + bstart + t0);
+    });
   }
 }
 

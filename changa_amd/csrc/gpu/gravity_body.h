@@ -105,16 +105,34 @@ __device__ __forceinline__ void pp_interact(float rx, float ry, float rz, float 
 //   a   += d^5 (V2 + d^2 (V3 + d^2 V4)) - d^3 S r,
 //   S = m + d^4 (5/2 P2 + d^2 (7/3 P3 + d^2 9/4 P4)).
 // The record's float4 k is rec[k * STRIDE] (STRIDE 1: records stored whole;
-// STRIDE = tile width: component planes).
+// STRIDE = tile width: component planes), read by load_rec.
+// The staged record of one cell, read once per cell and evaluated for every
+// target a lane holds.
+struct PcRec {
+  float4 c0, c1, c2, c3, c4, c5, c6, c7;
+};
+
 template <int STRIDE = 1>
-__device__ __forceinline__ void pc_interact(float x, float y, float z, const float4* rec,
-                                            float mt, Acc32& A) {
+__device__ __forceinline__ PcRec load_rec(const float4* rec) {
+  PcRec r;
+  r.c0 = rec[0];
+  r.c1 = rec[STRIDE];
+  r.c2 = rec[2 * STRIDE];
+  r.c3 = rec[3 * STRIDE];
+  r.c4 = rec[4 * STRIDE];
+  r.c5 = rec[5 * STRIDE];
+  r.c6 = rec[6 * STRIDE];
+  r.c7 = rec[7 * STRIDE];
+  return r;
+}
+
+__device__ __forceinline__ void pc_eval(float x, float y, float z, const PcRec& R, float mt,
+                                        Acc32& A) {
   const float r2 = x * x + y * y + z * z;
   if (r2 == 0.0f) return;
   const float dir = rsq(r2);
-  const float4 c0 = rec[0], c1 = rec[STRIDE], c2 = rec[2 * STRIDE], c3 = rec[3 * STRIDE];
-  const float4 c4 = rec[4 * STRIDE], c5 = rec[5 * STRIDE], c6 = rec[6 * STRIDE],
-               c7 = rec[7 * STRIDE];
+  const float4 c0 = R.c0, c1 = R.c1, c2 = R.c2, c3 = R.c3, c4 = R.c4, c5 = R.c5, c6 = R.c6,
+               c7 = R.c7;
   const float m = c0.w;
   const float qxx = c1.x, qxy = c1.y, qxz = c1.z, qyy = c1.w, qyz = c2.x, qzz = c2.y;
   const float qxxx = c2.z, qxyy = c2.w, qxxy = c3.x, qyyy = c3.y, qxxz = c3.z, qyyz = c3.w,
@@ -164,37 +182,10 @@ __device__ __forceinline__ void pc_interact(float x, float y, float z, const flo
   A.idt = max_nonneg(A.idt, (mt + m) * d3);
 }
 
-// Slot reduction in a fixed order and the += into the result array.
-__device__ __forceinline__ void reduce_store(double (*red)[kWave], float* redi, int lane, int nt,
-                                             int nslot, int slot, double ax, double ay, double az,
-                                             double pot, float idt, VariablePartData* out) {
-  bool live = slot < nslot;
-  red[0][lane] = live ? ax : 0.0;
-  red[1][lane] = live ? ay : 0.0;
-  red[2][lane] = live ? az : 0.0;
-  red[3][lane] = live ? pot : 0.0;
-  redi[lane] = live ? idt : 0.0f;
-  __syncthreads();
-  if (lane < nt) {
-    double sx = 0, sy = 0, sz = 0, sp = 0;
-    float si = 0.0f;
-    for (int s = 0; s < nslot; ++s) {
-      int l = s * nt + lane;
-      sx += red[0][l];
-      sy += red[1][l];
-      sz += red[2][l];
-      sp += red[3][l];
-      si = fmaxf(si, redi[l]);
-    }
-    VariablePartData v = *out;
-    v.a.x += (float)sx;
-    v.a.y += (float)sy;
-    v.a.z += (float)sz;
-    v.potential += (float)sp;
-    v.dtGrav = fmaxf(v.dtGrav, si);
-    *out = v;
-  }
-  __syncthreads();
+template <int STRIDE = 1>
+__device__ __forceinline__ void pc_interact(float x, float y, float z, const float4* rec,
+                                            float mt, Acc32& A) {
+  pc_eval(x, y, z, load_rec<STRIDE>(rec), mt, A);
 }
 
 // ---------------------------------------------------------------------------
@@ -301,19 +292,68 @@ __device__ __forceinline__ void stage_cell(const ChangaGravityLaunch& a, const W
 // batched kernel. A chunk is up to 64 targets of one bucket; `Lanes` is the
 // slot layout of the wavefront over them.
 // ---------------------------------------------------------------------------
+// Lane layout of a chunk of nt <= 64 targets with K targets per lane: the
+// lanes form nslot = 64 / ng slots of ng = ceil(nt / K) lanes each; lane g of
+// a slot holds the targets g, g + ng, ..., g + (K - 1) ng that are < nt, and
+// every slot takes every nslot-th source. K > 1 packs more slots into the
+// wavefront when nt does not divide 64 well (nt = 17: 3 slots, 51 lanes busy
+// with K = 1; 7 slots of 9 lanes, 119 of 128 target places, with K = 2).
 struct Lanes {
   int lane, nt, nslot, slot;
   bool live;
+  int ng, g;
 };
 
+template <int K = 1>
 __device__ __forceinline__ Lanes lanes_for(int lane, int nt) {
   Lanes l;
   l.lane = lane;
   l.nt = nt;
-  l.nslot = kWave / nt;
-  l.slot = lane / nt;
+  l.ng = (nt + K - 1) / K;
+  l.nslot = kWave / l.ng;
+  l.slot = lane / l.ng;
+  l.g = lane - l.slot * l.ng;
   l.live = l.slot < l.nslot;
   return l;
+}
+
+// Targets per lane (1, 2 or 3) that keep the most target places busy,
+// nslot(K) nt / (64 K), the smaller K on ties.
+__device__ __forceinline__ int targets_per_lane(int nt) {
+  int best = 1;
+  int bnum = (kWave / nt) * nt, bden = 1;  // utilisation bnum / (64 bden)
+  for (int k = 2; k <= 3; ++k) {
+    const int ng = (nt + k - 1) / k;
+    const int num = (kWave / ng) * nt;
+    if (num * bden > bnum * k) {
+      best = k;
+      bnum = num;
+      bden = k;
+    }
+  }
+  return best;
+}
+
+template <int K>
+struct KTag {
+  static constexpr int value = K;
+};
+
+// Calls body(KTag<K>()) with K = targets_per_lane(nt) (nt uniform across the
+// wavefront, so one of the three instantiations runs).
+template <typename Body>
+__device__ __forceinline__ void with_targets_per_lane(int nt, Body&& body) {
+  switch (targets_per_lane(nt)) {
+    case 1:
+      body(KTag<1>());
+      break;
+    case 2:
+      body(KTag<2>());
+      break;
+    default:
+      body(KTag<3>());
+      break;
+  }
 }
 
 struct Target {
@@ -356,6 +396,70 @@ __device__ __forceinline__ Target load_target(const ChangaGravityLaunch& a, int 
   return t;
 }
 
+// The K targets of a lane: target g + j ng of the chunk (a place past nt
+// holds target g again; the reduction ignores it).
+template <bool PRECISE, int K>
+__device__ __forceinline__ void load_targets(const ChangaGravityLaunch& a, int tbase,
+                                             const Lanes& l, double cx0, double cy0, double cz0,
+                                             Target (&t)[K]) {
+  const int g = l.live ? l.g : 0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int tj = g + j * l.ng;
+    t[j] = load_target<PRECISE>(a, tbase + (tj < l.nt ? tj : g), cx0, cy0, cz0);
+  }
+}
+
+// Doubles of LDS the slot reduction of K targets per lane needs (the floats
+// of dtGrav included).
+template <int K>
+constexpr size_t reduce_bytes() {
+  return (size_t)K * kWave * (4 * sizeof(double) + sizeof(float));
+}
+
+// Slot reduction in a fixed order and the += into the result array (+= for
+// a and phi, max for dtGrav): every lane stores the sums of its places,
+// then lane i < nt adds target i's slots in slot order. lds: reduce_bytes<K>().
+template <int K>
+__device__ __forceinline__ void reduce_store(void* lds, const Lanes& l, const Acc64 (&acc)[K],
+                                             VariablePartData* out) {
+  constexpr int W = K * kWave;
+  double* red = static_cast<double*>(lds);
+  float* redi = reinterpret_cast<float*>(red + 4 * W);
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int i = j * kWave + l.lane;
+    const bool v = l.live && l.g + j * l.ng < l.nt;
+    red[i] = v ? acc[j].ax : 0.0;
+    red[W + i] = v ? acc[j].ay : 0.0;
+    red[2 * W + i] = v ? acc[j].az : 0.0;
+    red[3 * W + i] = v ? acc[j].pot : 0.0;
+    redi[i] = v ? acc[j].idt : 0.0f;
+  }
+  __syncthreads();
+  if (l.lane < l.nt) {
+    const int j = l.lane / l.ng, g = l.lane - j * l.ng;
+    double sx = 0, sy = 0, sz = 0, sp = 0;
+    float si = 0.0f;
+    for (int s = 0; s < l.nslot; ++s) {
+      const int i = j * kWave + s * l.ng + g;
+      sx += red[i];
+      sy += red[W + i];
+      sz += red[2 * W + i];
+      sp += red[3 * W + i];
+      si = fmaxf(si, redi[i]);
+    }
+    VariablePartData v = out[l.lane];
+    v.a.x += (float)sx;
+    v.a.y += (float)sy;
+    v.a.z += (float)sz;
+    v.potential += (float)sp;
+    v.dtGrav = fmaxf(v.dtGrav, si);
+    out[l.lane] = v;
+  }
+  __syncthreads();
+}
+
 // Cells [start, end) of a.list onto the chunk's targets. Tiles of TILE <= 64
 // cells: lane c stages cell c of the tile into rec4 (TILE * kPcRec floats of
 // LDS) as component planes: float4 k of cell c at rec4[k * TILE + c], so
@@ -364,10 +468,11 @@ __device__ __forceinline__ Target load_target(const ChangaGravityLaunch& a, int 
 // touches: one per lane when a bucket has a single target). The next tile's
 // wire records are in flight while this one is evaluated. A tile without
 // softened cells (ballot) runs the hexadecapole loop with no per-cell test.
-template <bool PRECISE, int TILE, bool STREAM = true>
+template <bool PRECISE, int TILE, bool STREAM = true, int K = 1>
 __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start, int end,
-                                         double cx0, double cy0, double cz0, const Target& t,
-                                         const Lanes& l, float4* rec4, Acc64& acc) {
+                                         double cx0, double cy0, double cz0,
+                                         const Target (&t)[K], const Lanes& l, float4* rec4,
+                                         Acc64 (&acc)[K]) {
   static_assert(TILE <= kWave, "one staged cell per lane");
   const float L = a.fperiod;
   const ILCell* cells = static_cast<const ILCell*>(a.list);
@@ -384,33 +489,49 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
     // issued after the barrier, whose fence would otherwise wait for them
     load_cell<PRECISE, STREAM>(a, cells, base + TILE + l.lane, stager ? end : 0, w);
     if (l.live) {
-      Acc32 A = {0.f, 0.f, 0.f, 0.f, acc.idt};
-      // cell slot + j nslot: full rounds over every slot, then one partial
+      Acc32 A[K];
+#pragma unroll
+      for (int j = 0; j < K; ++j) A[j] = {0.f, 0.f, 0.f, 0.f, acc[j].idt};
+      // cell slot + i nslot: full rounds over every slot, then one partial
       const int full = nc / l.nslot;
       const float4* r4 = rec4 + l.slot;
+      auto hex = [&](const float4* r) {
+        const PcRec R = load_rec<TILE>(r);
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+          pc_eval(t[j].x - R.c0.x, t[j].y - R.c0.y, t[j].z - R.c0.z, R, t[j].m, A[j]);
+          // one target's evaluation at a time: interleaving them would need
+          // K times the temporaries
+          if (j + 1 < K) __builtin_amdgcn_sched_barrier(0);
+        }
+      };
       if (!soft_tile) {
-        auto cell = [&](const float4* r) {
-          const float4 h = r[0];
-          pc_interact<TILE>(t.x - h.x, t.y - h.y, t.z - h.z, r, t.m, A);
-        };
-        for (int j = 0; j < full; ++j, r4 += l.nslot) cell(r4);
-        if (l.slot < nc - full * l.nslot) cell(r4);
+        for (int i = 0; i < full; ++i, r4 += l.nslot) hex(r4);
+        if (l.slot < nc - full * l.nslot) hex(r4);
       } else {
         auto cell = [&](const float4* r) {
           const float4 h = r[0];
-          if (h.w < 0.0f)  // softened cell: spline monopole at the cm
-            pp_interact(h.x - t.x, h.y - t.y, h.z - t.z, -h.w, r[TILE].x, t.m, t.soft, A);
-          else
-            pc_interact<TILE>(t.x - h.x, t.y - h.y, t.z - h.z, r, t.m, A);
+          if (h.w < 0.0f) {  // softened cell: spline monopole at the cm
+            const float sc = r[TILE].x;
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+              pp_interact(h.x - t[j].x, h.y - t[j].y, h.z - t[j].z, -h.w, sc, t[j].m, t[j].soft,
+                          A[j]);
+          } else {
+            hex(r);
+          }
         };
-        for (int j = 0; j < full; ++j, r4 += l.nslot) cell(r4);
+        for (int i = 0; i < full; ++i, r4 += l.nslot) cell(r4);
         if (l.slot < nc - full * l.nslot) cell(r4);
       }
-      acc.ax += A.ax;
-      acc.ay += A.ay;
-      acc.az += A.az;
-      acc.pot += A.pot;
-      acc.idt = A.idt;
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        acc[j].ax += A[j].ax;
+        acc[j].ay += A[j].ay;
+        acc[j].az += A[j].az;
+        acc[j].pot += A[j].pot;
+        acc[j].idt = A[j].idt;
+      }
     }
     __syncthreads();
   }
@@ -538,19 +659,26 @@ struct ChunkSphere {
   float cx, cy, cz, r;
 };
 
-template <bool PRECISE>
-__device__ __forceinline__ ChunkSphere chunk_sphere(const Target& t, const Lanes& l) {
+template <bool PRECISE, int K>
+__device__ __forceinline__ ChunkSphere chunk_sphere(const Target (&t)[K], const Lanes& l) {
   ChunkSphere c;
   if (PRECISE) {
     c.cx = c.cy = c.cz = 0.0f;  // the bucket centre, first target of the bucket
   } else {
-    c.cx = __shfl(t.x, 0, kWave);
-    c.cy = __shfl(t.y, 0, kWave);
-    c.cz = __shfl(t.z, 0, kWave);
+    c.cx = __shfl(t[0].x, 0, kWave);  // lane 0 holds the chunk's first target
+    c.cy = __shfl(t[0].y, 0, kWave);
+    c.cz = __shfl(t[0].z, 0, kWave);
   }
-  const float dx = t.x - c.cx, dy = t.y - c.cy, dz = t.z - c.cz;
-  float d2 = l.live ? dx * dx + dy * dy + dz * dz : 0.0f;
-  float sm = l.live ? t.soft : 0.0f;
+  float d2 = 0.0f, sm = 0.0f;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const float dx = t[j].x - c.cx, dy = t[j].y - c.cy, dz = t[j].z - c.cz;
+    // a place past nt holds a real target of the chunk again
+    if (l.live) {
+      d2 = fmaxf(d2, dx * dx + dy * dy + dz * dz);
+      sm = fmaxf(sm, t[j].soft);
+    }
+  }
 #pragma unroll
   for (int d = kWave / 2; d > 0; d >>= 1) {
     d2 = fmaxf(d2, __shfl_xor(d2, d, kWave));
@@ -622,14 +750,15 @@ __device__ __forceinline__ void pp_source_tiles(const ChangaGravityLaunch& a, in
 }
 
 // Sources [start, end) of a.list onto the chunk's targets (partBucketForce).
-template <bool PRECISE, int KIND, int TILE, bool STREAM = true>
+template <bool PRECISE, int KIND, int TILE, bool STREAM = true, int K = 1>
 __device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int start, int end,
-                                           double cx0, double cy0, double cz0, const Target& t,
-                                           const Lanes& l, PpFwdLds<TILE>& s, Acc64& acc) {
+                                           double cx0, double cy0, double cz0,
+                                           const Target (&t)[K], const Lanes& l,
+                                           PpFwdLds<TILE>& s, Acc64 (&acc)[K]) {
   constexpr int cap = PpFwdLds<TILE>::cap;
   const float L = a.fperiod;
   const int lane = l.lane;
-  const ChunkSphere cs = chunk_sphere<PRECISE>(t, l);
+  const ChunkSphere cs = chunk_sphere<PRECISE, K>(t, l);
   const unsigned long long below = (1ull << lane) - 1ull;
   pp_source_tiles<KIND, TILE, STREAM>(a, start, end, lane, s.map, [&](int ns, auto&& src) {
     // stage: classify each source, far ones to the front, near ones to the back
@@ -676,21 +805,33 @@ __device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int sta
       s.rec[k] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
     __syncthreads();
     if (l.live) {
-      Acc32 A = {0.f, 0.f, 0.f, 0.f, acc.idt};
+      Acc32 A[K];
+#pragma unroll
+      for (int j = 0; j < K; ++j) A[j] = {0.f, 0.f, 0.f, 0.f, acc[j].idt};
       const float4* r = s.rec + l.slot * q;
-      for (int j = 0; j < q; j += 2, r += 2) {
-        pp_far(r[0], t, A);
-        pp_far(r[1], t, A);
+      for (int i = 0; i < q; i += 2, r += 2) {
+        const float4 r0 = r[0], r1 = r[1];
+#pragma unroll
+        for (int j = 0; j < K; ++j) pp_far(r0, t[j], A[j]);
+        if (K > 1) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < K; ++j) pp_far(r1, t[j], A[j]);
       }
       for (int k = l.slot; k < nnear; k += l.nslot) {
         const float4 v = s.rec[cap - 1 - k];
-        pp_interact(v.x - t.x, v.y - t.y, v.z - t.z, v.w, s.soft[k], t.m, t.soft, A);
+        const float sk = s.soft[k];
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+          pp_interact(v.x - t[j].x, v.y - t[j].y, v.z - t[j].z, v.w, sk, t[j].m, t[j].soft, A[j]);
       }
-      acc.ax += A.ax;
-      acc.ay += A.ay;
-      acc.az += A.az;
-      acc.pot += A.pot;
-      acc.idt = A.idt;
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        acc[j].ax += A[j].ax;
+        acc[j].ay += A[j].ay;
+        acc[j].az += A[j].az;
+        acc[j].pot += A[j].pot;
+        acc[j].idt = A[j].idt;
+      }
     }
     __syncthreads();
   });

@@ -110,16 +110,19 @@ def _oracle_acc(tree, lists, pos, mass, soft):
 
 
 def test_gpu_adjoint_kdk_finite_differences(device):
-    """One kick-drift-kick step, L = u . v1 (every term of its gradient goes
-    through the forces), differentiated through both force evaluations by the
-    autograd wrapper (positions checkpointed, trees and lists rebuilt in
-    backward), against central differences of the same step evaluated by the
-    FP64 oracle on the frozen nominal trees."""
+    """One kick-drift-kick step of a 4,096-particle Plummer sphere, L = u . v1
+    (every term of its gradient goes through the forces), differentiated
+    through both force evaluations by the autograd wrapper (positions
+    checkpointed, trees and lists rebuilt in backward), against central
+    differences (step 1e-4 eps in position, 1e-4 m in mass) of the same step
+    evaluated by the FP64 oracle on the frozen nominal trees, on 64 seeded
+    particles. Bar: 1e-4 of the largest component, beyond three times the
+    differences' own rounding noise (the spread between steps h and 2 h)."""
     import torch
 
     from tests.gravity_autograd import TreeGravity
 
-    cfg = configs.plummer(n=2048)
+    cfg = configs.plummer(n=4096)
     n = cfg["pos"].shape[0]
     rng = np.random.default_rng(8)
     v0 = 0.3 * rng.standard_normal((n, 3))
@@ -165,19 +168,60 @@ def test_gpu_adjoint_kdk_finite_differences(device):
         v1 = vh + 0.5 * dt * a1
         return float((wv * v1).sum())
 
-    for k in rng.choice(n, size=6, replace=False):
-        fd = np.zeros(4)
+    hx = 1e-4 * cfg["soft"][0]  # step 1e-4 eps
+    worst = np.zeros(2)
+    for i, k in enumerate(rng.choice(n, size=64, replace=False)):
+        fd, fd2 = np.zeros(4), np.zeros(3)
         for c in range(3):
-            xp, xm = x0n.copy(), x0n.copy()
-            xp[k, c] += 1e-5
-            xm[k, c] -= 1e-5
-            fd[c] = (loss(xp, cfg["mass"]) - loss(xm, cfg["mass"])) / 2e-5
+            for step, out in ((hx, fd), (2 * hx, fd2)):
+                xp, xm = x0n.copy(), x0n.copy()
+                xp[k, c] += step
+                xm[k, c] -= step
+                out[c] = (loss(xp, cfg["mass"]) - loss(xm, cfg["mass"])) / (2 * step)
         mp, mm = cfg["mass"].copy(), cfg["mass"].copy()
         h = 1e-4 * cfg["mass"][k]
         mp[k] += h
         mm[k] -= h
         fd[3] = (loss(x0n, mp) - loss(x0n, mm)) / (2 * h)
-        ex = np.abs(fd[:3] - gx[k]).max() / np.abs(fd[:3]).max()
+        # the differences' own rounding: L sums 4,096 terms of order one, so
+        # at a step of 1e-6 the quotient carries ~1e-9 of absolute noise,
+        # measured here as the spread between the steps h and 2 h (whose
+        # truncation errors are far below it)
+        noise = np.abs(fd[:3] - fd2).max()
+        ex = max(0.0, np.abs(fd[:3] - gx[k]).max() - 3 * noise) / np.abs(fd[:3]).max()
         em = abs(fd[3] - gm[k]) / abs(fd[3])
-        print(f"kdk particle {k}: dL/dx rel {ex:.2e} dL/dm rel {em:.2e}", flush=True)
+        worst = np.maximum(worst, [ex, em])
+        if i % 16 == 15:
+            print(f"kdk: {i + 1} of 64 particles, worst dL/dx rel {worst[0]:.2e} "
+                  f"dL/dm rel {worst[1]:.2e}", flush=True)
         assert ex <= 1e-4 and em <= 1e-4, (k, fd, gx[k], gm[k])
+
+
+def test_gpu_adjoint_full_size_plummer(device):
+    """The full 2^20-particle Plummer sphere (tree and lists built on the
+    GPU), L = sum w . a with seeded w ~ N(0,1)^3: the GPU gradient against
+    the FP64 oracle's adjoint on the same tree and lists, on 4,096 seeded
+    particles (the oracle computes every particle's gradient: minutes of
+    FP64 work on the host cores)."""
+    import os
+    import time
+
+    import torch
+    ev = engine.Evaluation(configs.workload(4), precise=True, device=device, lists="gpu",
+                           tree="gpu")
+    wv = np.random.default_rng(23).standard_normal(ev.pos.shape)
+    g = ev.adjoint(wv).cpu().numpy()
+    torch.cuda.synchronize()
+    t = time.time()
+    threads = min(16, len(os.sched_getaffinity(0)))
+    geo, mom = O.build_moments(ev.tree, ev.pos, ev.mass, ev.soft)
+    print("adjoint full size: oracle moments done", flush=True)
+    go = O.adjoint_lists(ev.tree, geo, mom, ev.pos, ev.mass, ev.soft, ev.lists, wv, period=0.0,
+                         n_threads=threads)
+    print(f"adjoint full size: oracle adjoint {time.time() - t:.1f} s on {threads} threads",
+          flush=True)
+    samp = np.random.default_rng(24).choice(ev.pos.shape[0], size=4096, replace=False)
+    ex, em = _errors(g[samp], go[samp])
+    print(f"adjoint full size 2^20: dL/dx {summary(ex)} dL/dm {summary(em)}", flush=True)
+    for e in (ex, em):
+        assert np.median(e) <= 2e-5 and np.percentile(e, 99) <= 5e-4, summary(e)
