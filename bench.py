@@ -190,7 +190,8 @@ def main():
     ms = step_ms[args.path]
     value = n_int / (ms * 1e-3) / 1e9
 
-    # per-kernel durations (HIP events on the launch stream), separate pass
+    # per-kernel durations (HIP events on the launch stream, the median of
+    # the passes), separate pass
     def time_fn(fn):
         evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
         for i in range(args.steps):
@@ -198,7 +199,8 @@ def main():
             fn()
             evs[2 * i + 1].record(stream)
         torch.cuda.synchronize()
-        return float(np.mean([evs[2 * i].elapsed_time(evs[2 * i + 1]) for i in range(args.steps)]))
+        return float(np.median([evs[2 * i].elapsed_time(evs[2 * i + 1])
+                                for i in range(args.steps)]))
 
     kern = {"moments": time_fn(ev.build_moments), "zero": time_fn(ev.zero),
             "pc": time_fn(ev.run_pc), "pp": time_fn(ev.run_pp),
@@ -267,11 +269,20 @@ def main():
             "frac": round(achieved * 1e12 / PEAK_FP32, 4),
         },
     }
-    traffic, traffic_kernel = traffic_from_profile(
-        tag, {"batch_kernel": f"batch_kernel<{opts.pcTile},{opts.ppTile}>",
-              "fused_kernel": "fused_kernel<64,256>"}.get(dom, dom))
+    variant = {"batch_kernel": f"batch_kernel<{opts.pcTile},{opts.ppTile}>",
+               "fused_kernel": "fused_kernel<64,256>"}.get(dom, dom)
+    traffic, traffic_kernel = traffic_from_profile(tag, variant)
+    out["roofline"]["kernel"] = variant
+    out["roofline"]["kernel_ms"] = round(dom_ms, 4)
     out["roofline"]["traffic"] = traffic
     out["roofline"]["traffic_kernel"] = traffic_kernel
+    # the same kernel's median duration in the committed rocprofv3 trace of
+    # this workload, and the fraction it gives
+    prof_ms, prof_kernel = duration_from_profile(tag, variant)
+    if prof_ms:
+        out["roofline"]["profile_kernel"] = prof_kernel
+        out["roofline"]["profile_kernel_ms"] = prof_ms
+        out["roofline"]["profile_frac"] = round(dom_flop / (prof_ms * 1e-3) / PEAK_FP32, 4)
 
     log(f"timed: {value:.1f} G interactions/s")
     if not args.no_accuracy:
@@ -507,6 +518,31 @@ def traffic_from_profile(tag, kernel):
         return found[kernel], kernel
     family = [k for k in found if k.split("<")[0] == kernel.split("<")[0]]
     return (found[family[0]], family[0]) if family else (None, None)
+
+
+def duration_from_profile(tag, kernel):
+    """(median ms, profiled kernel) of `kernel` in the committed rocprofv3
+    kernel trace summary of this workload: for the batched kernel the median
+    of the graph replays of that tile variant (the summary's line for the
+    variant the plan chose in the profiled run), else the table's median
+    column."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"kernel_stats_{tag}.md")
+    if not os.path.exists(path):
+        return None, None
+    base = kernel.split("<")[0]
+    want = kernel[len(base) + 1:-1].split(",") if "<" in kernel else None
+    for line in open(path):
+        m = re.match(r"^(.*\)): \d+ graph replays.* median ([0-9.]+) ms", line)
+        if m and base == "batch_kernel" and "batch_kernel<" in m.group(1):
+            targs = m.group(1).split("batch_kernel<")[1].split(">")[0].split(", ")
+            if want and targs[-2:] == want:
+                return float(m.group(2)), m.group(1)
+        elif line.startswith("| ") and base != "batch_kernel" and f"::{base}" in line:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 6:
+                return float(cells[5]), cells[0]
+    return None, None
 
 
 def accuracy(ev, step, n_samples):

@@ -42,6 +42,16 @@ def summary(e):
     return dict(median=float(np.median(e)), p99=float(np.percentile(e, 99)), max=float(e.max()))
 
 
+def check_tails(e, med, p99, mx, label=""):
+    """Median, 99th percentile and max of a per-particle error against bars
+    set at about 1.5x the tails measured for the same case (the cases are
+    seeded, so a run reproduces its tails up to the FP32 order of the
+    partial sums); None skips a bar."""
+    s = summary(e)
+    for key, bar in (("median", med), ("p99", p99), ("max", mx)):
+        assert bar is None or s[key] <= bar, (label, key, s)
+
+
 def deep_chain_workload(n=1500):
     """Particles at radii 2^(-k/25) (k < n) around the box centre: a chain of
     ~60 tree levels, ending in one bucket of ~950 particles whose Morton keys

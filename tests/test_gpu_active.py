@@ -8,7 +8,7 @@ import pytest
 
 from changa_amd import configs, engine
 from oracle import oracle as O
-from tests.helpers import oracle_forces, rel_err, summary
+from tests.helpers import check_tails, oracle_forces, rel_err, summary
 
 pytestmark = pytest.mark.gpu
 
@@ -38,9 +38,11 @@ def test_gpu_active_isolated(device):
     assert ev.n_targets < ev.pos.shape[0]
     ev.step()
     torch.cuda.synchronize()
+    bars = {"acc": (6.6e-8, 7.2e-7, 2.3e-6), "pot": (3.9e-8, 1.35e-7, 1.95e-7),
+            "dtgrav": (1.5e-7, 2.3e-6, 5.1e-6)}
     for name, e in zip(("acc", "pot", "dtgrav"), _active_errors(ev)):
         print(f"active isolated {name} {summary(e)}", flush=True)
-        assert e.max() <= 1e-5, summary(e)
+        check_tails(e, *bars[name], label=f"active isolated {name}")
 
 
 def test_gpu_active_periodic_batched(device):
@@ -57,8 +59,8 @@ def test_gpu_active_periodic_batched(device):
     np.testing.assert_array_equal(v[:, 4], ref[:, 4])
     ea, ep, ed = _active_errors(ev, pot_scale=True)
     print(f"active periodic acc {summary(ea)} pot {summary(ep)}", flush=True)
-    assert np.median(ea) <= 5e-6 and np.percentile(ea, 99) <= 5e-5, summary(ea)
-    assert np.median(ep) <= 5e-6 and np.percentile(ep, 99) <= 5e-5, summary(ep)
+    check_tails(ea, 4.8e-7, 3.5e-6, 1.7e-5, label="active periodic acc")
+    check_tails(ep, 2.5e-6, 7.8e-6, 9.4e-6, label="active periodic pot (over rms)")
     ev.free_batch()
 
 
@@ -78,5 +80,5 @@ def test_gpu_active_adjoint(device):
     ex = np.linalg.norm(g[:, :3] - go[:, :3], axis=1) / np.linalg.norm(go[:, :3], axis=1)
     em = np.abs(g[:, 3] - go[:, 3]) / np.abs(go[:, 3])
     print(f"active adjoint dL/dx {summary(ex)} dL/dm {summary(em)}", flush=True)
-    for e in (ex, em):
-        assert np.median(e) <= 2e-5 and np.percentile(e, 99) <= 5e-4, summary(e)
+    check_tails(ex, 3.5e-7, 2.2e-6, 1e-5, label="active adjoint dL/dx")
+    check_tails(em, 2.4e-7, 1.35e-5, None, label="active adjoint dL/dm")

@@ -7,16 +7,19 @@ Bars. The GPU evaluates every pair term in FP32 on bucket-centred
 coordinates and accumulates in FP64; per particle, dL/dx and dL/dm are sums
 of target-side and source-side terms of either sign. Against the oracle's
 adjoint: median relative error <= 2e-5, 99th percentile <= 5e-4 (per
-particle, relative to that particle's gradient). Against finite differences
-of the FP64 forward: <= 1e-4 relative to the largest gradient component of
-the sampled particle.
+particle, relative to that particle's gradient), at 1.5x the largest tails
+measured over the cases: median 1.5e-6 (measured up to 9.4e-7, the softened
+sphere's dL/dx) and 99th percentile 6e-5 (4.0e-5, its dL/dm); dL/dm has
+zeros, near which its relative error has no useful maximum. Against finite
+differences of the FP64 forward: <= 1e-4 relative to the largest gradient
+component of the sampled particle.
 """
 import numpy as np
 import pytest
 
 from changa_amd import configs, engine
 from oracle import oracle as O
-from tests.helpers import summary
+from tests.helpers import check_tails, summary
 
 pytestmark = pytest.mark.gpu
 
@@ -59,8 +62,8 @@ def test_gpu_adjoint_matches_oracle(device, case):
     go = _oracle_grad(ev, wv)
     ex, em = _errors(g, go)
     print(f"adjoint {case}: dL/dx {summary(ex)} dL/dm {summary(em)}", flush=True)
-    for e in (ex, em):
-        assert np.median(e) <= 2e-5 and np.percentile(e, 99) <= 5e-4, summary(e)
+    for name, e in (("dL/dx", ex), ("dL/dm", em)):
+        check_tails(e, 1.5e-6, 6e-5, None, label=f"adjoint {case} {name}")
     # translation invariance of the tree force
     tot = np.abs(g[:, :3].sum(axis=0)).max() / np.abs(g[:, :3]).sum(axis=0).max()
     assert tot < 1e-5, tot
@@ -85,8 +88,8 @@ def test_gpu_adjoint_ewald_matches_oracle(device, active, reps):
     go = _oracle_grad(ev, wv, ewald=True)
     ex, em = _errors(g, go)
     print(f"adjoint ewald active={active} R={reps}: dL/dx {summary(ex)} dL/dm {summary(em)}", flush=True)
-    for e in (ex, em):
-        assert np.median(e) <= 2e-5 and np.percentile(e, 99) <= 5e-4, summary(e)
+    for name, e in (("dL/dx", ex), ("dL/dm", em)):
+        check_tails(e, 1.5e-6, 6e-5, None, label=f"adjoint ewald {name}")
     tot = np.abs(g[:, :3].sum(axis=0)).max() / np.abs(g[:, :3]).sum(axis=0).max()
     assert tot < 1e-5, tot
 
@@ -223,5 +226,5 @@ def test_gpu_adjoint_full_size_plummer(device):
     samp = np.random.default_rng(24).choice(ev.pos.shape[0], size=4096, replace=False)
     ex, em = _errors(g[samp], go[samp])
     print(f"adjoint full size 2^20: dL/dx {summary(ex)} dL/dm {summary(em)}", flush=True)
-    for e in (ex, em):
-        assert np.median(e) <= 2e-5 and np.percentile(e, 99) <= 5e-4, summary(e)
+    check_tails(ex, 5.8e-7, 6.9e-6, 2.1e-5, label="full-size adjoint dL/dx")
+    check_tails(em, 5.4e-7, 4.4e-5, None, label="full-size adjoint dL/dm")

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from changa_amd import configs, engine, gpu
-from tests.helpers import oracle_forces, rel_err, summary
+from tests.helpers import check_tails, oracle_forces, rel_err, summary
 
 pytestmark = pytest.mark.gpu
 
@@ -106,11 +106,12 @@ def test_gpu_batched_halos_oracle(device):
     _close(v, ref)
     acc, pot, dtg = ev.results()
     oa, op, od = oracle_forces(ev)
+    bars = {"acc": (1.2e-7, 2.4e-6, 4.3e-5), "pot": (3.6e-8, 1.4e-7, 3.4e-7),
+            "dtgrav": (2.7e-7, 3.8e-6, 1.5e-5)}
     for name, e in (("acc", rel_err(acc, oa)), ("pot", rel_err(pot, op)),
                     ("dtgrav", rel_err(dtg, od))):
         print(f"halos batched {name} {summary(e)}", flush=True)
-        assert np.median(e) <= 1e-6 and np.percentile(e, 99) <= 1e-5, summary(e)
-        assert e.max() <= 1e-4, summary(e)
+        check_tails(e, *bars[name], label=f"halos batched {name}")
     ev.free_batch()
     torch.cuda.synchronize()
 
@@ -134,7 +135,7 @@ def test_gpu_batched_periodic_step(device):
     oa, op, od = oracle_forces(ev)
     e = rel_err(acc, oa)
     print(f"periodic batched acc {summary(e)}", flush=True)
-    assert np.median(e) <= 5e-6 and np.percentile(e, 99) <= 5e-5, summary(e)
+    check_tails(e, 2.1e-6, 1.13e-5, 1.45e-4, label="periodic batched acc")
     ev.free_batch()
 
 
@@ -156,8 +157,8 @@ def test_gpu_batched_two_replica_shells(device):
     oa, op, _ = oracle_forces(ev)
     ea, ep = rel_err(acc, oa), rel_err(pot, op)
     print(f"batched R=2: acc {summary(ea)} pot {summary(ep)}", flush=True)
-    for e in (ea, ep):
-        assert np.median(e) <= 5e-6 and np.percentile(e, 99) <= 5e-5, summary(e)
+    check_tails(ea, 2.9e-6, 1.8e-5, 7.3e-5, label="batched R=2 acc")
+    check_tails(ep, 4e-6, 1.45e-5, 2.1e-5, label="batched R=2 pot")
     ev.free_batch()
 
 

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from changa_amd import configs, engine
-from tests.helpers import deep_chain_workload, oracle_forces, rel_err, summary
+from tests.helpers import check_tails, deep_chain_workload, oracle_forces, rel_err, summary
 
 pytestmark = pytest.mark.gpu
 
@@ -90,8 +90,8 @@ def test_gpu_fused_periodic_oracle(device):
     oa, op, _ = oracle_forces(ev)
     ea, ep = rel_err(acc, oa), rel_err(pot, op)
     print(f"fused periodic acc {summary(ea)} pot {summary(ep)}", flush=True)
-    for e in (ea, ep):
-        assert np.median(e) <= 5e-6 and np.percentile(e, 99) <= 5e-5, summary(e)
+    check_tails(ea, 3.1e-6, 1.6e-5, 9.9e-5, label="fused periodic acc")
+    check_tails(ep, 2.1e-6, 7.3e-6, 1.3e-5, label="fused periodic pot")
 
 
 def test_gpu_fused_capacity_error(device):

@@ -1,15 +1,20 @@
 """GPU parity: the HIP kernels through libchanga_gpu.so against the FP64
 oracle on identical trees and interaction lists.
 
-Bars (per particle, relative to the oracle's |a| and |phi|):
+Bars (per particle, relative to the oracle's |a|, |phi| and dtGrav): the
+median, 99th percentile and max of each case at about 1.5x the measured
+tails (check_tails), so a regression of the tail fails. The measured tails:
   - isolated systems, precision form (FP64 bucket-centre subtraction, FP32
-    interactions): acceleration and potential within 1e-5 for every particle;
+    interactions): acceleration median 5e-8, max 7e-6 (the 2^16 Plummer
+    sphere); potential max 2.3e-7;
   - isolated systems, plain form (FP32 absolute coordinates, the reference's
-    wire layout): 99th percentile within 1e-5, max within 1e-4 (close pairs
-    lose digits to absolute FP32 coordinates, as in the reference's GPU path);
+    wire layout and what the HostCUDA mirror returns): acceleration median
+    7e-8, max 1.0e-5 (close pairs lose digits to absolute FP32 coordinates,
+    as in the reference's GPU path);
   - periodic boxes: the tree-with-replicas force and the Ewald correction are
     each ~10x the net force and cancel; FP32 terms are exact to ~1e-7 of
-    their own size, so the bar is median 5e-6, 99th percentile 5e-5 of |a|;
+    their own size, so the acceleration has median 2e-6, 99th percentile
+    1.1e-5 and a max of 5e-5 to 8e-5 where the net force nearly vanishes;
   - dtGrav (a max of FP32 terms): as the acceleration of the same form.
 """
 import numpy as np
@@ -17,12 +22,13 @@ import pytest
 
 from changa_amd import configs, engine
 from oracle import oracle as O
-from tests.helpers import deep_chain_workload, oracle_forces, rel_err, summary
+from tests.helpers import check_tails, deep_chain_workload, oracle_forces, rel_err, summary
 
 pytestmark = pytest.mark.gpu
 
 
-def _check(ev, tol_max, tol_p99, label, tol_med=1e-5):
+def _check(ev, label, bars):
+    """bars: (median, p99, max) for acc, pot and dtGrav."""
     import torch
     ev.step()
     torch.cuda.synchronize()
@@ -30,9 +36,8 @@ def _check(ev, tol_max, tol_p99, label, tol_med=1e-5):
     oa, op, od = oracle_forces(ev)
     ea, ep, ed = rel_err(acc, oa), rel_err(pot, op), rel_err(dtg, od)
     print(f"{label}: acc {summary(ea)} pot {summary(ep)} dtgrav {summary(ed)}", flush=True)
-    for e in (ea, ep, ed):
-        assert np.median(e) <= tol_med and np.percentile(e, 99) <= tol_p99, summary(e)
-        assert e.max() <= tol_max, summary(e)
+    for name, e, b in (("acc", ea, bars[0]), ("pot", ep, bars[1]), ("dtgrav", ed, bars[2])):
+        check_tails(e, *b, label=f"{label} {name}")
 
 
 @pytest.mark.parametrize("case", ["plummer", "deep_chain_gpu_tree"])
@@ -60,16 +65,19 @@ def test_gpu_moments_match_oracle(device, case):
 def test_gpu_plummer_65536(device, precise):
     ev = engine.Evaluation(configs.plummer(n=65536), precise=precise, device=device)
     if precise:
-        _check(ev, 1e-5, 1e-5, "plummer precise")
+        _check(ev, "plummer precise", ((8.3e-8, 5e-7, 1.1e-5), (4.2e-8, 1.7e-7, 3.5e-7),
+                                       (2.1e-7, 2.9e-6, 1.1e-5)))
     else:
-        _check(ev, 1e-4, 1e-5, "plummer plain")
+        _check(ev, "plummer plain", ((1.1e-7, 1.3e-6, 1.5e-5), (4.2e-8, 1.7e-7, 3.5e-7),
+                                     (1.1e-6, 9.2e-6, 2.7e-5)))
 
 
 def test_gpu_softened_cells(device):
     w = configs.plummer(n=4096, eps=0.5)
     ev = engine.Evaluation(w, precise=True, device=device)
     assert ev.counts[1] > 0, "workload must exercise softened cells"
-    _check(ev, 1e-5, 1e-5, "softened")
+    _check(ev, "softened", ((3.8e-7, 7.3e-7, 8.4e-7), (8.1e-8, 2.2e-7, 2.7e-7),
+                            (8.3e-8, 2.2e-6, 4.3e-6)))
 
 
 def test_gpu_pp_list_formats_agree(device):
@@ -90,7 +98,12 @@ def test_gpu_pp_list_formats_agree(device):
 def test_gpu_periodic_ewald(device, precise):
     w = configs.zeldovich(ng=24, rms=0.02)
     ev = engine.Evaluation(w, precise=precise, device=device)
-    _check(ev, 5e-4, 5e-5, "periodic precise" if precise else "periodic plain", tol_med=5e-6)
+    if precise:
+        _check(ev, "periodic precise", ((3.2e-6, 1.7e-5, 1.3e-4), (2.2e-6, 7.9e-6, 1.25e-5),
+                                        (1.9e-7, 8.2e-7, 1.6e-6)))
+    else:
+        _check(ev, "periodic plain", ((3.3e-6, 1.7e-5, 9.8e-5), (2.2e-6, 7.9e-6, 1.2e-5),
+                                      (1.5e-6, 6.9e-6, 1.3e-5)))
 
 
 @pytest.mark.parametrize("reps", [2, 3])
@@ -111,9 +124,10 @@ def test_gpu_periodic_more_replica_shells(device, reps):
     oa, op, _ = oracle_forces(ev)
     ea, ep = rel_err(acc, oa), rel_err(pot, op)
     print(f"periodic R={reps}: acc {summary(ea)} pot {summary(ep)}", flush=True)
-    assert np.median(ea) <= 5e-6 and np.percentile(ea, 99) <= 5e-5, summary(ea)
-    assert np.median(ep) <= (5e-6 if reps == 2 else 1e-5) and np.percentile(ep, 99) <= 5e-5, \
-        summary(ep)
+    bars = {2: ((2.8e-6, 1.6e-5, 5e-5), (4.4e-6, 1.6e-5, 2.1e-5)),
+            3: ((3.3e-6, 2.2e-5, 4.6e-5), (8.8e-6, 3e-5, 3.7e-5))}[reps]
+    check_tails(ea, *bars[0], label=f"R={reps} acc")
+    check_tails(ep, *bars[1], label=f"R={reps} pot")
 
 
 @pytest.mark.parametrize("cfg", [2, 3, 4, 5])
@@ -122,11 +136,10 @@ def test_gpu_full_size_sampled_oracle(device, cfg):
     2^20 Plummer sphere, the 256^3 box with a 10% active rung), tree and
     lists built on the GPU, through the batched step (with the overlapped
     Ewald pass for the boxes): the targets of 128 seeded buckets against the
-    FP64 oracle on the same lists, at the periodic bars for the boxes and
-    median 1e-6, 99th percentile 1e-5 for the isolated ones. The periodic
+    FP64 oracle on the same lists, at 1.5x the measured tails. The periodic
     potential (zero mean over the box) crosses zero among the clustered
-    box's particles, so the potential's error is taken relative to the
-    sample's rms potential rather than per particle."""
+    box's particles, so the potential's bars are on its error relative to the
+    sample's rms potential; the per-particle error is printed beside it."""
     import torch
     ev = engine.Evaluation(configs.workload(cfg), precise=True, device=device, lists="gpu",
                            tree="gpu")
@@ -143,9 +156,14 @@ def test_gpu_full_size_sampled_oracle(device, cfg):
     if ev.active is not None:
         idx = idx[ev.active[idx]]
     ea = rel_err(acc[idx], oa[idx])
+    epp = rel_err(pot[idx], op[idx])
     ep = np.abs(pot[idx] - op[idx]) / np.sqrt(np.mean(op[idx] ** 2))
-    print(f"full size {cfg}: {len(idx)} targets, acc {summary(ea)} pot {summary(ep)}", flush=True)
-    med, p99 = (5e-6, 5e-5) if ev.w["periodic"] else (1e-6, 1e-5)
-    for e in (ea, ep):
-        assert np.median(e) <= med and np.percentile(e, 99) <= p99, summary(e)
+    print(f"full size {cfg}: {len(idx)} targets, acc {summary(ea)} pot per particle "
+          f"{summary(epp)} pot over rms {summary(ep)}", flush=True)
+    bars = {2: ((3e-6, 2.2e-5, 9.6e-5), (2e-6, 6.9e-6, 1.03e-5)),
+            3: ((1.04e-7, 5.5e-7, 1.3e-6), (3.6e-8, 1.4e-7, 1.85e-7)),
+            4: ((7.3e-8, 2.5e-7, 3.8e-7), (2.8e-8, 1.04e-7, 1.3e-7)),
+            5: ((4e-7, 2.1e-6, 4.3e-6), (2.6e-6, 8.1e-6, 8.7e-6))}[cfg]
+    check_tails(ea, *bars[0], label=f"full size {cfg} acc")
+    check_tails(ep, *bars[1], label=f"full size {cfg} pot")
     ev.free_batch()

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from changa_amd import configs, engine, host
-from tests.helpers import deep_chain_workload, oracle_forces, rel_err, summary
+from tests.helpers import check_tails, deep_chain_workload, oracle_forces, rel_err, summary
 
 pytestmark = pytest.mark.gpu
 
@@ -64,7 +64,7 @@ def test_gpu_lists_active_rung(device):
 
 def test_gpu_lists_forces(device):
     """A periodic box evaluated through GPU-built lists, against the FP64
-    oracle on the same lists (the bars of the periodic parity test)."""
+    oracle on the same lists (bars at 1.5x the measured tails)."""
     import torch
     ev = engine.Evaluation(configs.zeldovich(ng=24, rms=0.02), precise=True, device=device,
                            lists="gpu")
@@ -74,8 +74,8 @@ def test_gpu_lists_forces(device):
     oa, op, _ = oracle_forces(ev)
     ea, ep = rel_err(acc, oa), rel_err(pot, op)
     print(f"gpu lists periodic acc {summary(ea)} pot {summary(ep)}", flush=True)
-    for e in (ea, ep):
-        assert np.median(e) <= 5e-6 and np.percentile(e, 99) <= 5e-5, summary(e)
+    check_tails(ea, 3.1e-6, 1.6e-5, 8.1e-5, label="gpu lists periodic acc")
+    check_tails(ep, 2.2e-6, 7.7e-6, 1.25e-5, label="gpu lists periodic pot")
 
 
 def test_tipsy_evaluate(device, tmp_path):
