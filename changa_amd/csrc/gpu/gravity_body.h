@@ -418,8 +418,10 @@ constexpr size_t reduce_bytes() {
 }
 
 // Slot reduction in a fixed order and the += into the result array (+= for
-// a and phi, max for dtGrav): every lane stores the sums of its places,
-// then lane i < nt adds target i's slots in slot order. lds: reduce_bytes<K>().
+// a and phi, max for dtGrav): every lane stores the sums of its places, the
+// slots are added pairwise (slot s += slot s + h, h = ceil(n / 2), while
+// n > 1 slots are left: log2(nslot) rounds, all lanes at once), then lane
+// i < nt adds target i's total, now in slot 0. lds: reduce_bytes<K>().
 template <int K>
 __device__ __forceinline__ void reduce_store(void* lds, const Lanes& l, const Acc64 (&acc)[K],
                                              VariablePartData* out) {
@@ -437,24 +439,31 @@ __device__ __forceinline__ void reduce_store(void* lds, const Lanes& l, const Ac
     redi[i] = v ? acc[j].idt : 0.0f;
   }
   __syncthreads();
-  if (l.lane < l.nt) {
-    const int j = l.lane / l.ng, g = l.lane - j * l.ng;
-    double sx = 0, sy = 0, sz = 0, sp = 0;
-    float si = 0.0f;
-    for (int s = 0; s < l.nslot; ++s) {
-      const int i = j * kWave + s * l.ng + g;
-      sx += red[i];
-      sy += red[W + i];
-      sz += red[2 * W + i];
-      sp += red[3 * W + i];
-      si = fmaxf(si, redi[i]);
+  for (int n = l.nslot; n > 1;) {
+    const int h = (n + 1) >> 1;
+    if (l.slot < n - h) {
+      const int d = h * l.ng;  // the partner lane: same g, slot + h
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const int i = j * kWave + l.lane;
+        red[i] += red[i + d];
+        red[W + i] += red[W + i + d];
+        red[2 * W + i] += red[2 * W + i + d];
+        red[3 * W + i] += red[3 * W + i + d];
+        redi[i] = fmaxf(redi[i], redi[i + d]);
+      }
     }
+    n = h;
+    __syncthreads();
+  }
+  if (l.lane < l.nt) {
+    const int j = l.lane / l.ng, i = j * kWave + (l.lane - j * l.ng);
     VariablePartData v = out[l.lane];
-    v.a.x += (float)sx;
-    v.a.y += (float)sy;
-    v.a.z += (float)sz;
-    v.potential += (float)sp;
-    v.dtGrav = fmaxf(v.dtGrav, si);
+    v.a.x += (float)red[i];
+    v.a.y += (float)red[W + i];
+    v.a.z += (float)red[2 * W + i];
+    v.potential += (float)red[3 * W + i];
+    v.dtGrav = fmaxf(v.dtGrav, redi[i]);
     out[l.lane] = v;
   }
   __syncthreads();
@@ -639,14 +648,14 @@ __device__ __forceinline__ void pp_tiles(const ChangaGravityLaunch& a, int start
 // LDS of a staged tile: rec[kPpFwdCap] float4 (x, y, z, m), far sources from
 // the front, near sources from the back (near source k at rec[cap - 1 - k],
 // its softening at soft[k]). Far sources are consumed slot-major: slot s
-// takes the q consecutive records [s q, s q + q) with q uniform and even,
+// takes the q consecutive records [s q, s q + q) with q uniform,
 // the records past the far count holding null sources (m = 0 at 1e18: b
 // underflows to 0, so they add exactly nothing), so the loop runs a fixed
-// trip count with immediate-offset LDS reads.
+// trip count with immediate-offset LDS reads (pairs, then one odd record).
 // ---------------------------------------------------------------------------
 template <int TILE>
 struct PpFwdLds {
-  static constexpr int cap = TILE + 2 * kWave;  // far padding is < 2 nslot <= 2 kWave
+  static constexpr int cap = TILE + kWave;  // far padding is < nslot <= kWave
   float4 rec[cap];
   float soft[TILE];
   int2 map[TILE];  // ILPart runs expanded: (source particle, offsetID) per tile slot
@@ -800,7 +809,7 @@ __device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int sta
       nnear += __popcll(nm);
     }
     // slot-major far layout: q records per slot, padded with null sources
-    const int q = ((nfar + l.nslot - 1) / l.nslot + 1) & ~1;
+    const int q = (nfar + l.nslot - 1) / l.nslot;
     for (int k = nfar + lane; k < l.nslot * q; k += kWave)
       s.rec[k] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
     __syncthreads();
@@ -809,13 +818,18 @@ __device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int sta
 #pragma unroll
       for (int j = 0; j < K; ++j) A[j] = {0.f, 0.f, 0.f, 0.f, acc[j].idt};
       const float4* r = s.rec + l.slot * q;
-      for (int i = 0; i < q; i += 2, r += 2) {
+      for (int i = 0; i + 1 < q; i += 2, r += 2) {
         const float4 r0 = r[0], r1 = r[1];
 #pragma unroll
         for (int j = 0; j < K; ++j) pp_far(r0, t[j], A[j]);
         if (K > 1) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < K; ++j) pp_far(r1, t[j], A[j]);
+      }
+      if (q & 1) {
+        const float4 r0 = r[0];
+#pragma unroll
+        for (int j = 0; j < K; ++j) pp_far(r0, t[j], A[j]);
       }
       for (int k = l.slot; k < nnear; k += l.nslot) {
         const float4 v = s.rec[cap - 1 - k];
