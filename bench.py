@@ -254,6 +254,7 @@ def main():
         "batched_options": {"pcTile": opts.pcTile, "ppTile": opts.ppTile,
                             "listsPerGrab": opts.listsPerGrab,
                             "order": {1: "tree", 2: "costliest first"}[opts.order],
+                            "targetsPerLane": opts.targetsPerLane,
                             "tuned_ms": round(opts.tunedMs, 4), "tune_s": round(tune_s, 2)},
         "fp32_valu_frac": round(flops / (ms * 1e-3) / PEAK_FP32, 4),
         "algorithmic_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
@@ -269,7 +270,7 @@ def main():
             "frac": round(achieved * 1e12 / PEAK_FP32, 4),
         },
     }
-    variant = {"batch_kernel": f"batch_kernel<{opts.pcTile},{opts.ppTile}>",
+    variant = {"batch_kernel": f"batch_kernel<{opts.pcTile},{opts.ppTile},{opts.targetsPerLane}>",
                "fused_kernel": "fused_kernel<64,256>"}.get(dom, dom)
     traffic, traffic_kernel = traffic_from_profile(tag, variant)
     out["roofline"]["kernel"] = variant
@@ -501,7 +502,8 @@ def grad_accuracy(ev, wt, w, n_samples=4096):
 
 def traffic_from_profile(tag, kernel):
     """(HBM bytes per launch, the profiled kernel) of `kernel` (for the
-    batched kernel, of its variant, e.g. "batch_kernel<64,256>"), from the
+    batched kernel, of its variant, e.g. "batch_kernel<64,256,3>": cell tile,
+    particle tile, targets per lane), from the
     committed rocprofv3 --pmc summary of this workload (FETCH_SIZE and
     WRITE_SIZE, copy-calibrated). The autotuner may have picked another
     tile variant in the profiled run: then that variant's figure is returned,
@@ -536,7 +538,7 @@ def duration_from_profile(tag, kernel):
         m = re.match(r"^(.*\)): \d+ graph replays.* median ([0-9.]+) ms", line)
         if m and base == "batch_kernel" and "batch_kernel<" in m.group(1):
             targs = m.group(1).split("batch_kernel<")[1].split(">")[0].split(", ")
-            if want and targs[-2:] == want:
+            if want and targs[-len(want):] == want:
                 return float(m.group(2)), m.group(1)
         elif line.startswith("| ") and base != "batch_kernel" and f"::{base}" in line:
             cells = [c.strip() for c in line.strip().strip("|").split("|")]

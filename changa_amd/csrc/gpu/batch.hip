@@ -56,7 +56,9 @@ struct BatchLds {
 
 // One bucket: cells, then particles, then one update of its results. Every
 // phase that uses LDS ends with a barrier, so the phases share one buffer.
-template <bool PRECISE, int KIND, int PCT, int PPT>
+// KMAX 1: one target per lane; 3: one to three, chosen per chunk by its size
+// (targets_per_lane).
+template <bool PRECISE, int KIND, int PCT, int PPT, int KMAX>
 __device__ __forceinline__ void batch_bucket(const BatchArgs& a, int b, int lane, float4* lds) {
   const int pcs = a.pc.marks[b], pce = a.pc.marks[b + 1];
   const int pps = a.pp.marks[b], ppe = a.pp.marks[b + 1];
@@ -66,7 +68,7 @@ __device__ __forceinline__ void batch_bucket(const BatchArgs& a, int b, int lane
   PpFwdLds<PPT>& s = *reinterpret_cast<PpFwdLds<PPT>*>(lds);
   for (int t0 = 0; t0 < n; t0 += kWave) {
     const int nt = min(kWave, n - t0);
-    with_targets_per_lane(nt, [&](auto kt) {
+    auto chunk = [&](auto kt) {
       constexpr int K = decltype(kt)::value;
       const Lanes l = lanes_for<K>(lane, nt);
       Target t[K];
@@ -77,7 +79,9 @@ __device__ __forceinline__ void batch_bucket(const BatchArgs& a, int b, int lane
       pc_cells<PRECISE, PCT, true, K>(a.pc, pcs, pce, cx0, cy0, cz0, t, l, lds, acc);
       pp_sources<PRECISE, KIND, PPT, true, K>(a.pp, pps, ppe, cx0, cy0, cz0, t, l, s, acc);
       reduce_store<K>(lds, l, acc, a.pc.vars + bstart + t0);
-    });
+    };
+    if constexpr (KMAX == 1) chunk(KTag<1>());
+    else with_targets_per_lane(nt, chunk);
   }
 }
 
@@ -85,10 +89,13 @@ __device__ __forceinline__ void batch_bucket(const BatchArgs& a, int b, int lane
 // exhausted. Each wave visits the 8 queues once, starting at its own XCD's,
 // and stays on a queue until a claim comes back past its end, so every wave
 // exits after finitely many claims and no wave waits on another.
-template <bool PRECISE, int KIND, int PCT, int PPT>
 // Three wavefronts per SIMD: up to 168 VGPRs for the three-targets-per-lane
 // loops (measured faster on the 2^22 halos than one target per lane at four
-// wavefronts).
+// wavefronts). With KMAX 1 the kernel holds one chunk loop instead of three
+// and needs 139-159 VGPRs, without spills; on a sparse rung (one to three
+// targets in most buckets) it measured 16% faster than KMAX 3, on the dense
+// halos 7% slower.
+template <bool PRECISE, int KIND, int PCT, int PPT, int KMAX>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void batch_kernel(
     BatchArgs a) {
   __shared__ float4 lds[BatchLds<PCT, PPT>::f4];
@@ -107,7 +114,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ba
       continue;
     }
     const int kend = min(k + a.grab, qn);
-    for (; k < kend; ++k) batch_bucket<PRECISE, KIND, PCT, PPT>(a, ids[q0 + k], lane, lds);
+    for (; k < kend; ++k) batch_bucket<PRECISE, KIND, PCT, PPT, KMAX>(a, ids[q0 + k], lane, lds);
   }
 }
 
@@ -141,15 +148,22 @@ __global__ void bucket_cost_kernel(ChangaGravityLaunch pc, ChangaGravityLaunch p
 
 using BatchFn = void (*)(BatchArgs);
 
-template <bool PR, int KIND>
+template <bool PR, int KIND, int KM>
 static BatchFn pick_tiles(int pct, int ppt) {
-  if (pct == 32) return ppt == 128 ? batch_kernel<PR, KIND, 32, 128> : batch_kernel<PR, KIND, 32, 256>;
-  return ppt == 128 ? batch_kernel<PR, KIND, 64, 128> : batch_kernel<PR, KIND, 64, 256>;
+  if (pct == 32)
+    return ppt == 128 ? batch_kernel<PR, KIND, 32, 128, KM> : batch_kernel<PR, KIND, 32, 256, KM>;
+  return ppt == 128 ? batch_kernel<PR, KIND, 64, 128, KM> : batch_kernel<PR, KIND, 64, 256, KM>;
 }
 
-static BatchFn pick(bool pr, int kind, int pct, int ppt) {
-  if (pr) return kind ? pick_tiles<true, 1>(pct, ppt) : pick_tiles<true, 0>(pct, ppt);
-  return kind ? pick_tiles<false, 1>(pct, ppt) : pick_tiles<false, 0>(pct, ppt);
+template <bool PR, int KIND>
+static BatchFn pick_kmax(int pct, int ppt, int kmax) {
+  return kmax == 1 ? pick_tiles<PR, KIND, 1>(pct, ppt) : pick_tiles<PR, KIND, 3>(pct, ppt);
+}
+
+static BatchFn pick(bool pr, int kind, const ChangaBatchOptions& o) {
+  const int pct = o.pcTile, ppt = o.ppTile, km = o.targetsPerLane;
+  if (pr) return kind ? pick_kmax<true, 1>(pct, ppt, km) : pick_kmax<true, 0>(pct, ppt, km);
+  return kind ? pick_kmax<false, 1>(pct, ppt, km) : pick_kmax<false, 0>(pct, ppt, km);
 }
 
 }  // namespace cgpu
@@ -191,7 +205,7 @@ int enqueue(const ChangaBatchPlan* p, const BatchArgs& args, const ChangaBatchOp
   hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kNumXcd * kCounterStride), 0, s,
                      args.counters);
   CGPU_CHECK(hipGetLastError());
-  BatchFn fn = pick(p->precise, p->kind, o.pcTile, o.ppTile);
+  BatchFn fn = pick(p->precise, p->kind, o);
   BatchArgs a = args;
   a.grab = o.listsPerGrab;
   void* kargs[] = {&a};
@@ -264,7 +278,8 @@ extern "C" int changaBatchPlanCreate(const ChangaGravityLaunch* cells,
   ChangaBatchOptions o = *opts;
   if ((o.pcTile != 0 && o.pcTile != 32 && o.pcTile != 64) ||
       (o.ppTile != 0 && o.ppTile != 128 && o.ppTile != 256) || o.listsPerGrab < 0 ||
-      o.listsPerGrab > kMaxGrab || o.order < 0 || o.order > 2)
+      o.listsPerGrab > kMaxGrab || o.order < 0 || o.order > 2 ||
+      (o.targetsPerLane != 0 && o.targetsPerLane != 1 && o.targetsPerLane != 3))
     return fail("changaBatchPlanCreate: option out of range", hipErrorInvalidValue, __FILE__,
                 __LINE__);
   hipStream_t s = (hipStream_t)stream;
@@ -330,7 +345,8 @@ extern "C" int changaBatchPlanCreate(const ChangaGravityLaunch* cells,
   p->args.counters = p->d_counters;
 
   // autotune whatever the caller left at 0, on a scratch result array
-  const bool tune = o.pcTile == 0 || o.ppTile == 0 || o.listsPerGrab == 0 || o.order == 0;
+  const bool tune = o.pcTile == 0 || o.ppTile == 0 || o.listsPerGrab == 0 || o.order == 0 ||
+                    o.targetsPerLane == 0;
   o.tunedMs = 0.0f;
   if (tune && nb > 0) {
     VariablePartData* scratch = nullptr;
@@ -342,25 +358,27 @@ extern "C" int changaBatchPlanCreate(const ChangaGravityLaunch* cells,
     if (e == hipSuccess) e = hipEventCreate(&e1);
     float best = -1.0f;
     ChangaBatchOptions bo = o;
-    const int pcts[2] = {32, 64}, ppts[2] = {128, 256}, grabs[2] = {1, 4}, orders[2] = {1, 2};
-    for (int i0 = 0; i0 < 2 && e == hipSuccess; ++i0)
-      for (int i1 = 0; i1 < 2 && e == hipSuccess; ++i1)
-        for (int i2 = 0; i2 < 2 && e == hipSuccess; ++i2)
-          for (int i3 = 0; i3 < 2 && e == hipSuccess; ++i3) {
+    const int pcts[2] = {32, 64}, ppts[2] = {128, 256}, grabs[2] = {1, 4}, orders[2] = {1, 2},
+              kmaxs[2] = {3, 1};
+    for (int i = 0; i < 32 && e == hipSuccess; ++i) {
+            const int i0 = i & 1, i1 = (i >> 1) & 1, i2 = (i >> 2) & 1, i3 = (i >> 3) & 1,
+                      i4 = i >> 4;
             ChangaBatchOptions c = o;
             c.pcTile = o.pcTile ? o.pcTile : pcts[i0];
             c.ppTile = o.ppTile ? o.ppTile : ppts[i1];
             c.listsPerGrab = o.listsPerGrab ? o.listsPerGrab : grabs[i2];
             c.order = o.order ? o.order : orders[i3];
+            c.targetsPerLane = o.targetsPerLane ? o.targetsPerLane : kmaxs[i4];
             c.zeroVars = 0;
             // skip duplicates of a fixed option
-            if ((o.pcTile && i0) || (o.ppTile && i1) || (o.listsPerGrab && i2) || (o.order && i3))
+            if ((o.pcTile && i0) || (o.ppTile && i1) || (o.listsPerGrab && i2) || (o.order && i3) ||
+                (o.targetsPerLane && i4))
               continue;
             BatchArgs a = p->args;
             a.pc.vars = scratch;
             a.pp.vars = scratch;
             a.queue = d_q[c.order - 1];
-            const int grid = grid_for(pick(p->precise, p->kind, c.pcTile, c.ppTile));
+            const int grid = grid_for(pick(p->precise, p->kind, c));
             if (grid <= 0) {
               e = hipErrorInvalidValue;
               break;
@@ -398,6 +416,7 @@ extern "C" int changaBatchPlanCreate(const ChangaGravityLaunch* cells,
     if (!o.ppTile) o.ppTile = 256;
     if (!o.listsPerGrab) o.listsPerGrab = 1;
     if (!o.order) o.order = 1;
+    if (!o.targetsPerLane) o.targetsPerLane = 3;
   }
   p->opt = o;
   p->d_queue = d_q[o.order - 1];
@@ -405,7 +424,7 @@ extern "C" int changaBatchPlanCreate(const ChangaGravityLaunch* cells,
   p->args.queue = p->d_queue;
   p->args.grab = o.listsPerGrab;
   if (nb > 0) {
-    p->grid = grid_for(pick(p->precise, p->kind, o.pcTile, o.ppTile));
+    p->grid = grid_for(pick(p->precise, p->kind, o));
     if (p->grid <= 0) return bail(fail("batch plan: occupancy", hipErrorInvalidValue, __FILE__,
                                        __LINE__));
     // capture reset (+ zeroing) + persistent kernel; every node is a kernel
@@ -599,16 +618,18 @@ extern "C" int changaBatchedGravityNoAlloc(const ChangaGravityLaunch* cells,
                 hipErrorInvalidValue, __FILE__, __LINE__);
   if (pp.listKind != 0 && pp.listKind != 1)
     return fail("changaBatchedGravityNoAlloc: listKind", hipErrorInvalidValue, __FILE__, __LINE__);
-  ChangaBatchOptions o = {64, 256, 1, 1, 0, 0.0f};
+  ChangaBatchOptions o = {64, 256, 1, 1, 0, 0.0f, 3};
   if (opts) {
     o = *opts;
     if (!o.pcTile) o.pcTile = 64;
     if (!o.ppTile) o.ppTile = 256;
     if (!o.listsPerGrab) o.listsPerGrab = 1;
     if (!o.order) o.order = 1;
+    if (!o.targetsPerLane) o.targetsPerLane = 3;
   }
   if ((o.pcTile != 32 && o.pcTile != 64) || (o.ppTile != 128 && o.ppTile != 256) ||
-      o.listsPerGrab < 1 || o.listsPerGrab > kMaxGrab)
+      o.listsPerGrab < 1 || o.listsPerGrab > kMaxGrab ||
+      (o.targetsPerLane != 1 && o.targetsPerLane != 3))
     return fail("changaBatchedGravityNoAlloc: option out of range", hipErrorInvalidValue,
                 __FILE__, __LINE__);
   if (o.order != 1)
@@ -619,7 +640,7 @@ extern "C" int changaBatchedGravityNoAlloc(const ChangaGravityLaunch* cells,
   if (int e = na_scratch(&dev, &ws)) return e;
   if (nb == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  const BatchFn fn = pick(pc.tpos64 != nullptr, pp.listKind, o.pcTile, o.ppTile);
+  const BatchFn fn = pick(pc.tpos64 != nullptr, pp.listKind, o);
   const int grid = grid_for(fn);
   if (grid <= 0)
     return fail("changaBatchedGravityNoAlloc: occupancy", hipErrorInvalidValue, __FILE__,

@@ -657,13 +657,15 @@ class Evaluation:
         self.run_ewald()
 
     # ------------------------------------------------------------------ batched form
-    def batch_plan(self, pc_tile=0, pp_tile=0, lists_per_grab=0, order=0, zero_vars=True):
+    def batch_plan(self, pc_tile=0, pp_tile=0, lists_per_grab=0, order=0, zero_vars=True,
+                   targets_per_lane=0):
         """Make (replacing any previous) the batched-form plan over this
         evaluation's lists; options left at 0 are autotuned. Returns the
         chosen ChangaBatchOptions."""
         self._need_lists()
         self.free_batch()
-        o = gpu.ChangaBatchOptions(pc_tile, pp_tile, lists_per_grab, order, int(zero_vars), 0.0)
+        o = gpu.ChangaBatchOptions(pc_tile, pp_tile, lists_per_grab, order, int(zero_vars), 0.0,
+                                   targets_per_lane)
         self._batch_launches = (self._launch(True), self._launch(False))
         plan = ctypes.c_void_p()
         gpu.check(gpu.lib().changaBatchPlanCreate(ctypes.byref(self._batch_launches[0]),

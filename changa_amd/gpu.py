@@ -71,7 +71,7 @@ class ChangaGravityLaunch(ctypes.Structure):
 
 class ChangaBatchOptions(ctypes.Structure):
     _fields_ = [("pcTile", c_int), ("ppTile", c_int), ("listsPerGrab", c_int), ("order", c_int),
-                ("zeroVars", c_int), ("tunedMs", c_float)]
+                ("zeroVars", c_int), ("tunedMs", c_float), ("targetsPerLane", c_int)]
 
 
 class ChangaAdjointLaunch(ctypes.Structure):

@@ -204,6 +204,8 @@ typedef struct {
   int order;         /* 1 tree order, 2 costliest bucket first (per XCD range); 0 = autotune */
   int zeroVars;      /* 1: each run first zeroes vars (particles covered by the buckets) */
   float tunedMs;     /* out: time of one evaluation with the chosen options, or 0 */
+  int targetsPerLane; /* 1: one target per lane; 3: one to three, by bucket size (more slots
+                         busy, but a kernel three chunk loops long); 0 = autotune */
 } ChangaBatchOptions;
 
 /* cells: the particle-cell launch (listKind 0); parts: the particle-particle
@@ -229,8 +231,8 @@ int changaBatchPlanDestroy(ChangaBatchPlan *plan);
    GPU the library runs on holds until the process ends; every call rewrites
    what it reads of it. Calls that overlap on one GPU (two streams, two host
    threads) race on that array: serialize them. opts may be NULL (cell tile
-   64, particle tile 256, one bucket per claim); fields left at 0 take those
-   values; order must be 1 (tree order: the costliest-first order needs a
+   64, particle tile 256, one bucket per claim, up to three targets per
+   lane); fields left at 0 take those values; order must be 1 (tree order: the costliest-first order needs a
    sort); zeroVars zeroes the results of the particles the buckets cover.
    More than CHANGA_NOALLOC_MAX_BUCKETS buckets is an error. Results equal
    changaBatchPlanLaunch's with the same options. */

@@ -51,9 +51,10 @@ def test_gpu_batched_matches_per_bucket(device, precise, compact):
     ref = _plain(ev)
     for pct in (32, 64):
         for ppt in (128, 256):
-            for grab, order in ((1, 1), (4, 2)):
-                o = ev.batch_plan(pct, ppt, grab, order)
-                assert (o.pcTile, o.ppTile, o.listsPerGrab, o.order) == (pct, ppt, grab, order)
+            for grab, order, kmax in ((1, 1, 3), (4, 2, 1)):
+                o = ev.batch_plan(pct, ppt, grab, order, targets_per_lane=kmax)
+                assert (o.pcTile, o.ppTile, o.listsPerGrab, o.order, o.targetsPerLane) == \
+                    (pct, ppt, grab, order, kmax)
                 _close(_batched(ev), ref)
     ev.free_batch()
 
@@ -84,8 +85,9 @@ def test_gpu_batched_autotune_leaves_results_alone(device):
     assert (ev.d_vars.cpu().numpy() == 7.0).all(), "tuning must write only scratch memory"
     assert o.pcTile in (32, 64) and o.ppTile in (128, 256)
     assert o.listsPerGrab in (1, 4) and o.order in (1, 2) and o.tunedMs > 0
+    assert o.targetsPerLane in (1, 3)
     print(f"tuned: pcTile {o.pcTile} ppTile {o.ppTile} grab {o.listsPerGrab} order {o.order} "
-          f"{o.tunedMs:.3f} ms", flush=True)
+          f"targets per lane {o.targetsPerLane} {o.tunedMs:.3f} ms", flush=True)
     _close(_batched(ev), ref)
     ev.free_batch()
 
@@ -189,10 +191,11 @@ def test_gpu_batched_rejects_bad_arguments(device):
     L = gpu.lib()  # error-return mode (the binding switches abort-on-error off)
     pc, pp = ev._launch(True), ev._launch(False)
     plan = ctypes.c_void_p()
-    bad = gpu.ChangaBatchOptions(48, 0, 0, 0, 1, 0.0)
-    assert L.changaBatchPlanCreate(ctypes.byref(pc), ctypes.byref(pp), ctypes.byref(bad),
-                                   ctypes.byref(plan), None) != 0
-    assert not plan.value
+    for bad in (gpu.ChangaBatchOptions(48, 0, 0, 0, 1, 0.0),
+                gpu.ChangaBatchOptions(64, 256, 1, 1, 1, 0.0, 2)):
+        assert L.changaBatchPlanCreate(ctypes.byref(pc), ctypes.byref(pp), ctypes.byref(bad),
+                                       ctypes.byref(plan), None) != 0
+        assert not plan.value
     pp2 = ev._launch(False)
     pp2.numBuckets = pc.numBuckets - 1
     ok = gpu.ChangaBatchOptions(0, 0, 0, 0, 1, 0.0)

@@ -19,9 +19,9 @@ pytestmark = pytest.mark.gpu
 OPTS = (64, 256, 1, 1)
 
 
-def _plan_result(ev):
+def _plan_result(ev, kmax):
     import torch
-    ev.batch_plan(*OPTS, zero_vars=True)
+    ev.batch_plan(*OPTS, zero_vars=True, targets_per_lane=kmax)
     ev.d_vars.fill_(5.0)
     ev.run_batched(graph=False)
     torch.cuda.synchronize()
@@ -30,10 +30,10 @@ def _plan_result(ev):
     return out
 
 
-def _noalloc_result(ev, opts=OPTS):
+def _noalloc_result(ev, kmax):
     import torch
     pc, pp = ev._launch(True), ev._launch(False)
-    o = gpu.ChangaBatchOptions(*opts, 1, 0.0)
+    o = gpu.ChangaBatchOptions(*OPTS, 1, 0.0, kmax)
     ev.d_vars.fill_(float("nan"))  # zeroVars must clear every particle of the buckets
     gpu.check(gpu.lib().changaBatchedGravityNoAlloc(ctypes.byref(pc), ctypes.byref(pp),
                                                     ctypes.byref(o), ev._sp()),
@@ -49,18 +49,18 @@ def _covered(ev):
     return idx
 
 
-@pytest.mark.parametrize("precise,compact", [(True, True), (False, False)])
-def test_no_alloc_equals_plan_two_calls(device, precise, compact):
+@pytest.mark.parametrize("precise,compact,kmax", [(True, True, 3), (False, False, 1)])
+def test_no_alloc_equals_plan_two_calls(device, precise, compact, kmax):
     import torch
     big = engine.Evaluation(configs.halos(n=1 << 16, cfg=3), precise=precise, pp_compact=compact,
                             device=device)
     small = engine.Evaluation(configs.plummer(n=6000), precise=precise, pp_compact=compact,
                               device=device)
     assert len(small.t_sizes) < len(big.t_sizes)
-    ref_big, ref_small = _plan_result(big), _plan_result(small)
-    got_big = _noalloc_result(big)
+    ref_big, ref_small = _plan_result(big, kmax), _plan_result(small, kmax)
+    got_big = _noalloc_result(big, kmax)
     free0 = torch.cuda.mem_get_info()[0]
-    got_small = _noalloc_result(small)
+    got_small = _noalloc_result(small, kmax)
     free1 = torch.cuda.mem_get_info()[0]
     assert free1 == free0, "the second call changed the device's free memory"
     for ev, got, ref in ((big, got_big, ref_big), (small, got_small, ref_small)):
