@@ -124,64 +124,129 @@ __device__ __forceinline__ void emit(int nd, const double* g, const double* C, d
   out[nd] = m;
 }
 
-__global__ __launch_bounds__(128) void bucket_moments_kernel(
+// Sum of v over the 4 lanes of a bucket's group (lanes 4g..4g+3), by an xor
+// butterfly: every lane of the group gets the same bits ((v0 + v1) + (v2 + v3)
+// up to commutation).
+__device__ __forceinline__ double group4_sum(double v) {
+  v += __shfl_xor(v, 1);
+  return v + __shfl_xor(v, 2);
+}
+__device__ __forceinline__ double group4_min(double v) {
+  v = fmin(v, __shfl_xor(v, 1));
+  return fmin(v, __shfl_xor(v, 2));
+}
+__device__ __forceinline__ double group4_max(double v) {
+  v = fmax(v, __shfl_xor(v, 1));
+  return fmax(v, __shfl_xor(v, 2));
+}
+
+constexpr int kMomBlock = 256;
+
+// Buckets (leaf nodes). A block takes kMomBlock consecutive nodes, compacts
+// its leaves into LDS, and gives each leaf a group of 4 lanes: lane j of the
+// group takes particles first + j, first + j + 4, ..., so each group reads its
+// bucket's particles as contiguous 4-particle runs and every cache line it
+// touches is used up within about one step (one lane per bucket, walking its
+// own particles, kept ~10^5 partly read lines live at once and fetched each
+// line several times over). Pass 1: mass, mass-weighted position and
+// softening, bounding box; pass 2 (same particles, now in cache): the raw
+// moments about the centre of mass and the farthest particle, summed over
+// the group. One lane per bucket then projects and writes the records.
+__global__ __launch_bounds__(kMomBlock) void bucket_moments_kernel(
     int nNodes, const int* first, const int* count, const int* child, const double* boxRadius,
     const double* pos, const double* mass, const double* soft, CudaMultipoleMoments* out,
     double* node64, double* cm64) {
-  int nd = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nd >= nNodes || child[2 * nd] >= 0) return;
-  const int f = first[nd], cnt = count[nd];
-  double g[12];
-  double M = 0, sx = 0, sy = 0, sz = 0, ss = 0;
-  g[6] = g[7] = g[8] = HUGE_VAL;
-  g[9] = g[10] = g[11] = -HUGE_VAL;
-  for (int i = f; i < f + cnt; ++i) {
-    double m = mass[i], x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-    M += m;
-    sx += m * x;
-    sy += m * y;
-    sz += m * z;
-    ss += m * soft[i];
-    g[6] = fmin(g[6], x); g[7] = fmin(g[7], y); g[8] = fmin(g[8], z);
-    g[9] = fmax(g[9], x); g[10] = fmax(g[10], y); g[11] = fmax(g[11], z);
+  __shared__ int leaves[kMomBlock];
+  __shared__ int wave_base[kMomBlock / kWave + 1];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int nd0 = blockIdx.x * kMomBlock + tid;
+  const bool leaf = nd0 < nNodes && child[2 * (size_t)nd0] < 0;
+  const unsigned long long bal = __ballot(leaf);
+  if (lane == 0) wave_base[wv + 1] = __popcll(bal);
+  __syncthreads();
+  if (tid == 0) {
+    wave_base[0] = 0;
+    for (int w = 1; w <= kMomBlock / kWave; ++w) wave_base[w] += wave_base[w - 1];
   }
-  g[0] = sx / M;
-  g[1] = sy / M;
-  g[2] = sz / M;
-  g[4] = ss / M;
-  g[5] = M;
-  double T[35];
-#pragma unroll
-  for (int k = 0; k < 35; ++k) T[k] = 0.0;
-  double r2max = 0.0;
-  for (int i = f; i < f + cnt; ++i) {
-    double m = mass[i];
-    double dx = pos[3 * i] - g[0], dy = pos[3 * i + 1] - g[1], dz = pos[3 * i + 2] - g[2];
-    r2max = fmax(r2max, dx * dx + dy * dy + dz * dz);
-    double px[5], py[5], pz[5];
-    px[0] = py[0] = pz[0] = m;
-    py[0] = pz[0] = 1.0;
-#pragma unroll
-    for (int k = 1; k < 5; ++k) {
-      px[k] = px[k - 1] * dx;
-      py[k] = py[k - 1] * dy;
-      pz[k] = pz[k - 1] * dz;
+  __syncthreads();
+  if (leaf) leaves[wave_base[wv] + __popcll(bal & ((1ull << lane) - 1ull))] = nd0;
+  __syncthreads();
+  const int nleaf = wave_base[kMomBlock / kWave];
+  const int sub = tid & 3;
+  // a group's 4 lanes run the same trip counts, so its shuffles always pair
+  // active lanes
+  for (int j = tid >> 2; j < nleaf; j += kMomBlock / 4) {
+    const int nd = leaves[j];
+    const int f = first[nd], cnt = count[nd];
+    double M = 0, sx = 0, sy = 0, sz = 0, ss = 0;
+    double lo0 = HUGE_VAL, lo1 = HUGE_VAL, lo2 = HUGE_VAL;
+    double hi0 = -HUGE_VAL, hi1 = -HUGE_VAL, hi2 = -HUGE_VAL;
+    for (int i = f + sub; i < f + cnt; i += 4) {
+      const double m = mass[i], x = pos[3 * (size_t)i], y = pos[3 * (size_t)i + 1],
+                   z = pos[3 * (size_t)i + 2];
+      M += m;
+      sx += m * x;
+      sy += m * y;
+      sz += m * z;
+      ss += m * soft[i];
+      lo0 = fmin(lo0, x); lo1 = fmin(lo1, y); lo2 = fmin(lo2, z);
+      hi0 = fmax(hi0, x); hi1 = fmax(hi1, y); hi2 = fmax(hi2, z);
     }
-    T[0] += m;
+    double g[12];
+    M = group4_sum(M);
+    g[0] = group4_sum(sx) / M;
+    g[1] = group4_sum(sy) / M;
+    g[2] = group4_sum(sz) / M;
+    g[4] = group4_sum(ss) / M;
+    g[5] = M;
+    g[6] = group4_min(lo0); g[7] = group4_min(lo1); g[8] = group4_min(lo2);
+    g[9] = group4_max(hi0); g[10] = group4_max(hi1); g[11] = group4_max(hi2);
+    double T[35];
+#pragma unroll
+    for (int k = 0; k < 35; ++k) T[k] = 0.0;
+    double r2max = 0.0;
+    for (int i = f + sub; i < f + cnt; i += 4) {
+      const double m = mass[i];
+      const double dx = pos[3 * (size_t)i] - g[0], dy = pos[3 * (size_t)i + 1] - g[1],
+                   dz = pos[3 * (size_t)i + 2] - g[2];
+      r2max = fmax(r2max, dx * dx + dy * dy + dz * dz);
+      double px[5], py[5], pz[5];
+      px[0] = m;
+      py[0] = pz[0] = 1.0;
+#pragma unroll
+      for (int k = 1; k < 5; ++k) {
+        px[k] = px[k - 1] * dx;
+        py[k] = py[k - 1] * dy;
+        pz[k] = pz[k - 1] * dz;
+      }
+#pragma unroll
+      for (int n = 2; n <= 4; ++n)
+#pragma unroll
+        for (int a = 0; a <= n; ++a)
+#pragma unroll
+          for (int b = 0; b <= n - a; ++b) {
+            const int c = n - a - b;
+            T[cidx(a, b, c)] += px[a] * py[b] * pz[c];
+          }
+    }
+    T[0] = M;
 #pragma unroll
     for (int n = 2; n <= 4; ++n)
 #pragma unroll
       for (int a = 0; a <= n; ++a)
 #pragma unroll
         for (int b = 0; b <= n - a; ++b) {
-          const int c = n - a - b;
-          T[cidx(a, b, c)] += px[a] * py[b] * pz[c];
+          const int k = cidx(a, b, n - a - b);
+          T[k] = group4_sum(T[k]);
         }
+    r2max = group4_max(r2max);
+    if (sub == 0) {
+      stf_project(T);
+      const double u = (cnt > 1 && r2max > 0.0) ? sqrt(r2max) : boxRadius[nd];
+      g[3] = u;
+      emit(nd, g, T, u, out, node64, cm64);
+    }
   }
-  stf_project(T);
-  const double u = (cnt > 1 && r2max > 0.0) ? sqrt(r2max) : boxRadius[nd];
-  g[3] = u;
-  emit(nd, g, T, u, out, node64, cm64);
 }
 
 __global__ __launch_bounds__(128) void internal_moments_kernel(int lo, int hi, const int* child,
@@ -246,7 +311,8 @@ extern "C" int changaBuildMoments(int nNodes, int nDepths, const int* levelOffse
       !moments || !node64)
     return fail("changaBuildMoments: missing argument", hipErrorInvalidValue, __FILE__, __LINE__);
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(bucket_moments_kernel, dim3((nNodes + 127) / 128), dim3(128), 0, s, nNodes,
+  hipLaunchKernelGGL(bucket_moments_kernel, dim3((nNodes + kMomBlock - 1) / kMomBlock),
+                     dim3(kMomBlock), 0, s, nNodes,
                      first, count, child, boxRadius, pos, mass, soft, moments, node64, cm64);
   CGPU_CHECK(hipGetLastError());
   for (int d = nDepths - 1; d >= 0; --d) {
