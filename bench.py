@@ -272,7 +272,10 @@ def main():
     }
     variant = {"batch_kernel": f"batch_kernel<{opts.pcTile},{opts.ppTile},{opts.targetsPerLane}>",
                "fused_kernel": "fused_kernel<64,256>"}.get(dom, dom)
-    traffic, traffic_kernel = traffic_from_profile(tag, variant)
+    queue = ({1: "tree", 2: "costliest"}[opts.order] + f"/{opts.listsPerGrab}"
+             if dom == "batch_kernel" else None)
+    traffic, traffic_kernel = traffic_from_profile(
+        tag, variant + (" " + queue if queue else ""))
     out["roofline"]["kernel"] = variant
     out["roofline"]["kernel_ms"] = round(dom_ms, 4)
     out["roofline"]["traffic"] = traffic
@@ -502,24 +505,20 @@ def grad_accuracy(ev, wt, w, n_samples=4096):
 
 def traffic_from_profile(tag, kernel):
     """(HBM bytes per launch, the profiled kernel) of `kernel` (for the
-    batched kernel, of its variant, e.g. "batch_kernel<64,256,3>": cell tile,
-    particle tile, targets per lane), from the
-    committed rocprofv3 --pmc summary of this workload (FETCH_SIZE and
-    WRITE_SIZE, copy-calibrated). The autotuner may have picked another
-    tile variant in the profiled run: then that variant's figure is returned,
-    with its name."""
+    batched kernel, of its variant and queue options, e.g.
+    "batch_kernel<64,256,3> tree/1": cell tile, particle tile, targets per
+    lane, order, buckets per claim), from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, copy-calibrated).
+    (None, None) when the summary holds no figure for exactly that kernel."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.md")
     if not os.path.exists(path):
         return None, None
-    found = {}
     for line in open(path):
         if " traffic_bytes_per_launch:" in line:
             name, v = line.split(" traffic_bytes_per_launch:")
-            found[name.strip()] = int(v)
-    if kernel in found:
-        return found[kernel], kernel
-    family = [k for k in found if k.split("<")[0] == kernel.split("<")[0]]
-    return (found[family[0]], family[0]) if family else (None, None)
+            if name.strip() == kernel:
+                return int(v), kernel
+    return None, None
 
 
 def duration_from_profile(tag, kernel):

@@ -212,7 +212,9 @@ typedef struct {
    launch (listKind 0 or 1) over the same buckets, targets, vars and
    precision-form positions. Options left at 0 are chosen by timing the
    candidates on these inputs (results go to a scratch array; vars is not
-   touched); *opts returns the choice. Synchronizes the stream. */
+   touched): the default (64, 256, 1, tree order, 3) is timed first and
+   another candidate is taken only when it is more than 3% faster. *opts
+   returns the choice. Synchronizes the stream. */
 int changaBatchPlanCreate(const ChangaGravityLaunch *cells, const ChangaGravityLaunch *parts,
                           ChangaBatchOptions *opts, ChangaBatchPlan **plan, void *stream);
 /* One evaluation, enqueued as kernels on `stream` (usable inside the
