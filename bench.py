@@ -95,6 +95,8 @@ def parse():
                     help="tree built on the GPU (default) or on the host")
     ap.add_argument("--adjoint", action="store_true",
                     help="time the adjoint (default for configs 4 and 5)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint leg that configs 4 and 5 run by default")
     ap.add_argument("--kdk", action="store_true",
                     help="one kick-drift-kick step of forward + adjoint, checkpointed")
     ap.add_argument("--grad-accuracy", action="store_true",
@@ -255,6 +257,7 @@ def main():
                             "listsPerGrab": opts.listsPerGrab,
                             "order": {1: "tree", 2: "costliest first"}[opts.order],
                             "targetsPerLane": opts.targetsPerLane,
+                            "cellSource": {1: "records", 2: "wire"}[opts.cellSource],
                             "tuned_ms": round(opts.tunedMs, 4), "tune_s": round(tune_s, 2)},
         "fp32_valu_frac": round(flops / (ms * 1e-3) / PEAK_FP32, 4),
         "algorithmic_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
@@ -270,7 +273,11 @@ def main():
             "frac": round(achieved * 1e12 / PEAK_FP32, 4),
         },
     }
-    variant = {"batch_kernel": f"batch_kernel<{opts.pcTile},{opts.ppTile},{opts.targetsPerLane}>",
+    # the last template arguments of the batched kernel: cell tile, particle
+    # tile, targets per lane, cell records (true) or wire moments (false)
+    rec = "true" if opts.cellSource == 1 else "false"
+    variant = {"batch_kernel": f"batch_kernel<{opts.pcTile},{opts.ppTile},{opts.targetsPerLane},"
+                               f"{rec}>",
                "fused_kernel": "fused_kernel<64,256>"}.get(dom, dom)
     queue = ({1: "tree", 2: "costliest"}[opts.order] + f"/{opts.listsPerGrab}"
              if dom == "batch_kernel" else None)
@@ -293,7 +300,8 @@ def main():
         out["accuracy"] = accuracy(ev, {"batched": ev.step_batched, "fused": ev.step_fused}.get(
             args.path, ev.step), args.samples)
         log("accuracy leg done")
-    if args.adjoint or args.kdk or args.grad_accuracy or args.config in (4, 5):
+    if args.adjoint or args.kdk or args.grad_accuracy or (args.config in (4, 5)
+                                                          and not args.no_adjoint):
         out["adjoint"] = adjoint_leg(ev, w, args)
         log("adjoint leg done")
     if not args.no_cpu:
@@ -506,8 +514,8 @@ def grad_accuracy(ev, wt, w, n_samples=4096):
 def traffic_from_profile(tag, kernel):
     """(HBM bytes per launch, the profiled kernel) of `kernel` (for the
     batched kernel, of its variant and queue options, e.g.
-    "batch_kernel<64,256,3> tree/1": cell tile, particle tile, targets per
-    lane, order, buckets per claim), from the committed rocprofv3 --pmc
+    "batch_kernel<64,256,3,false> tree/1": cell tile, particle tile, targets
+    per lane, cell records, order, buckets per claim), from the committed rocprofv3 --pmc
     summary of this workload (FETCH_SIZE and WRITE_SIZE, copy-calibrated).
     (None, None) when the summary holds no figure for exactly that kernel."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.md")

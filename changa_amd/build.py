@@ -26,7 +26,7 @@ GPU_SRC = ["gpu/gravity.hip", "gpu/batch.hip", "gpu/adjoint.hip", "gpu/ewald.hip
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instruction selection stays off for every HIP object.
-HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function"]
 OBJ_DIR = os.path.join(PKG, "_build")

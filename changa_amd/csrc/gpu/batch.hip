@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -47,7 +49,7 @@ struct BatchArgs {
 
 template <int PCT, int PPT>
 struct BatchLds {
-  static constexpr size_t pc = (size_t)PCT * kPcRec * sizeof(float);
+  static constexpr size_t pc = (size_t)(PCT + 1) * kFwdRec * sizeof(float);
   static constexpr size_t pp = sizeof(PpFwdLds<PPT>);
   static constexpr size_t red = reduce_bytes<3>();
   static constexpr size_t bytes = pc > pp ? (pc > red ? pc : red) : (pp > red ? pp : red);
@@ -57,8 +59,8 @@ struct BatchLds {
 // One bucket: cells, then particles, then one update of its results. Every
 // phase that uses LDS ends with a barrier, so the phases share one buffer.
 // KMAX 1: one target per lane; 3: one to three, chosen per chunk by its size
-// (targets_per_lane).
-template <bool PRECISE, int KIND, int PCT, int PPT, int KMAX>
+// (targets_per_lane). REC: cells from the 128-byte cell records.
+template <bool PRECISE, int KIND, int PCT, int PPT, int KMAX, bool REC>
 __device__ __forceinline__ void batch_bucket(const BatchArgs& a, int b, int lane, float4* lds) {
   const int pcs = a.pc.marks[b], pce = a.pc.marks[b + 1];
   const int pps = a.pp.marks[b], ppe = a.pp.marks[b + 1];
@@ -76,7 +78,7 @@ __device__ __forceinline__ void batch_bucket(const BatchArgs& a, int b, int lane
       Acc64 acc[K];
 #pragma unroll
       for (int j = 0; j < K; ++j) acc[j] = {0.0, 0.0, 0.0, 0.0, 0.0f};
-      pc_cells<PRECISE, PCT, true, K>(a.pc, pcs, pce, cx0, cy0, cz0, t, l, lds, acc);
+      pc_cells<PRECISE, PCT, true, K, REC>(a.pc, pcs, pce, cx0, cy0, cz0, t, l, lds, acc);
       pp_sources<PRECISE, KIND, PPT, true, K>(a.pp, pps, ppe, cx0, cy0, cz0, t, l, s, acc);
       reduce_store<K>(lds, l, acc, a.pc.vars + bstart + t0);
     };
@@ -95,8 +97,8 @@ __device__ __forceinline__ void batch_bucket(const BatchArgs& a, int b, int lane
 // and needs 139-159 VGPRs, without spills; on a sparse rung (one to three
 // targets in most buckets) it measured 16% faster than KMAX 3, on the dense
 // halos 7% slower.
-template <bool PRECISE, int KIND, int PCT, int PPT, int KMAX>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void batch_kernel(
+template <bool PRECISE, int KIND, int PCT, int PPT, int KMAX, bool REC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KMAX == 1 && REC ? 4 : 3))) void batch_kernel(
     BatchArgs a) {
   __shared__ float4 lds[BatchLds<PCT, PPT>::f4];
   const int lane = threadIdx.x;
@@ -114,7 +116,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ba
       continue;
     }
     const int kend = min(k + a.grab, qn);
-    for (; k < kend; ++k) batch_bucket<PRECISE, KIND, PCT, PPT, KMAX>(a, ids[q0 + k], lane, lds);
+    for (; k < kend; ++k)
+      batch_bucket<PRECISE, KIND, PCT, PPT, KMAX, REC>(a, ids[q0 + k], lane, lds);
   }
 }
 
@@ -148,22 +151,31 @@ __global__ void bucket_cost_kernel(ChangaGravityLaunch pc, ChangaGravityLaunch p
 
 using BatchFn = void (*)(BatchArgs);
 
-template <bool PR, int KIND, int KM>
+template <bool PR, int KIND, int KM, bool REC>
 static BatchFn pick_tiles(int pct, int ppt) {
   if (pct == 32)
-    return ppt == 128 ? batch_kernel<PR, KIND, 32, 128, KM> : batch_kernel<PR, KIND, 32, 256, KM>;
-  return ppt == 128 ? batch_kernel<PR, KIND, 64, 128, KM> : batch_kernel<PR, KIND, 64, 256, KM>;
+    return ppt == 128 ? batch_kernel<PR, KIND, 32, 128, KM, REC>
+                      : batch_kernel<PR, KIND, 32, 256, KM, REC>;
+  return ppt == 128 ? batch_kernel<PR, KIND, 64, 128, KM, REC>
+                    : batch_kernel<PR, KIND, 64, 256, KM, REC>;
 }
 
-template <bool PR, int KIND>
+template <bool PR, int KIND, bool REC>
 static BatchFn pick_kmax(int pct, int ppt, int kmax) {
-  return kmax == 1 ? pick_tiles<PR, KIND, 1>(pct, ppt) : pick_tiles<PR, KIND, 3>(pct, ppt);
+  return kmax == 1 ? pick_tiles<PR, KIND, 1, REC>(pct, ppt) : pick_tiles<PR, KIND, 3, REC>(pct, ppt);
 }
 
-static BatchFn pick(bool pr, int kind, const ChangaBatchOptions& o) {
+template <bool REC>
+static BatchFn pick_rec(bool pr, int kind, const ChangaBatchOptions& o) {
   const int pct = o.pcTile, ppt = o.ppTile, km = o.targetsPerLane;
-  if (pr) return kind ? pick_kmax<true, 1>(pct, ppt, km) : pick_kmax<true, 0>(pct, ppt, km);
-  return kind ? pick_kmax<false, 1>(pct, ppt, km) : pick_kmax<false, 0>(pct, ppt, km);
+  if (pr)
+    return kind ? pick_kmax<true, 1, REC>(pct, ppt, km) : pick_kmax<true, 0, REC>(pct, ppt, km);
+  return kind ? pick_kmax<false, 1, REC>(pct, ppt, km) : pick_kmax<false, 0, REC>(pct, ppt, km);
+}
+
+// the kernel for these options; with cell records when the launch has them
+static BatchFn pick(bool pr, int kind, const ChangaBatchOptions& o, bool rec) {
+  return rec ? pick_rec<true>(pr, kind, o) : pick_rec<false>(pr, kind, o);
 }
 
 }  // namespace cgpu
@@ -205,7 +217,7 @@ int enqueue(const ChangaBatchPlan* p, const BatchArgs& args, const ChangaBatchOp
   hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kNumXcd * kCounterStride), 0, s,
                      args.counters);
   CGPU_CHECK(hipGetLastError());
-  BatchFn fn = pick(p->precise, p->kind, o);
+  BatchFn fn = pick(p->precise, p->kind, o, o.cellSource == 1);
   BatchArgs a = args;
   a.grab = o.listsPerGrab;
   void* kargs[] = {&a};
@@ -275,13 +287,22 @@ extern "C" int changaBatchPlanCreate(const ChangaGravityLaunch* cells,
                 hipErrorInvalidValue, __FILE__, __LINE__);
   if (pp.listKind != 0 && pp.listKind != 1)
     return fail("changaBatchPlanCreate: listKind", hipErrorInvalidValue, __FILE__, __LINE__);
+  if (reinterpret_cast<uintptr_t>(pc.cellRecords) % CHANGA_CELL_RECORD_BYTES != 0)
+    return fail("changaBatchPlanCreate: cellRecords not 128-byte aligned", hipErrorInvalidValue,
+                __FILE__, __LINE__);
   ChangaBatchOptions o = *opts;
   if ((o.pcTile != 0 && o.pcTile != 32 && o.pcTile != 64) ||
       (o.ppTile != 0 && o.ppTile != 128 && o.ppTile != 256) || o.listsPerGrab < 0 ||
       o.listsPerGrab > kMaxGrab || o.order < 0 || o.order > 2 ||
-      (o.targetsPerLane != 0 && o.targetsPerLane != 1 && o.targetsPerLane != 3))
+      (o.targetsPerLane != 0 && o.targetsPerLane != 1 && o.targetsPerLane != 3) ||
+      o.cellSource < 0 || o.cellSource > 2)
     return fail("changaBatchPlanCreate: option out of range", hipErrorInvalidValue, __FILE__,
                 __LINE__);
+  if (o.cellSource == 1 && !pc.cellRecords)
+    return fail("changaBatchPlanCreate: cellSource 1 needs cellRecords", hipErrorInvalidValue,
+                __FILE__, __LINE__);
+  // without records only the wire moments can serve
+  if (!pc.cellRecords) o.cellSource = 2;
   hipStream_t s = (hipStream_t)stream;
   ChangaBatchPlan* p = new ChangaBatchPlan();
   std::memset(p, 0, sizeof(*p));
@@ -346,7 +367,7 @@ extern "C" int changaBatchPlanCreate(const ChangaGravityLaunch* cells,
 
   // autotune whatever the caller left at 0, on a scratch result array
   const bool tune = o.pcTile == 0 || o.ppTile == 0 || o.listsPerGrab == 0 || o.order == 0 ||
-                    o.targetsPerLane == 0;
+                    o.targetsPerLane == 0 || o.cellSource == 0;
   o.tunedMs = 0.0f;
   if (tune && nb > 0) {
     VariablePartData* scratch = nullptr;
@@ -356,54 +377,90 @@ extern "C" int changaBatchPlanCreate(const ChangaGravityLaunch* cells,
       e = hipMemsetAsync(scratch, 0, sizeof(VariablePartData) * (size_t)std::max(p->nParticles, 1), s);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    float best = -1.0f;
-    ChangaBatchOptions bo = o;
-    // the default configuration (cell tile 64, particle tile 256, one bucket
-    // per claim, tree order, up to three targets per lane) runs first; another
-    // replaces the best only when it is more than 3% faster, so that variants
-    // of near-equal time do not flip from one plan to the next
+    // CHANGA_BATCH_TUNE_LOG set: each candidate's time on stderr
+    const bool tune_log = std::getenv("CHANGA_BATCH_TUNE_LOG") != nullptr;
+    // mean time of `reps` runs of candidate c (after one untimed run), or -1
+    auto time_config = [&](const ChangaBatchOptions& c, int reps) -> float {
+      BatchArgs a = p->args;
+      a.pc.vars = scratch;
+      a.pp.vars = scratch;
+      a.queue = d_q[c.order - 1];
+      const int grid = grid_for(pick(p->precise, p->kind, c, c.cellSource == 1));
+      if (grid <= 0) {
+        e = hipErrorInvalidValue;
+        return -1.0f;
+      }
+      if (enqueue(p, a, c, grid, s)) {
+        e = hipErrorLaunchFailure;
+        return -1.0f;
+      }
+      e = hipEventRecord(e0, s);
+      for (int r = 0; r < reps && e == hipSuccess; ++r)
+        if (enqueue(p, a, c, grid, s)) e = hipErrorLaunchFailure;
+      if (e == hipSuccess) e = hipEventRecord(e1, s);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) return -1.0f;
+      ms /= (float)reps;
+      if (tune_log)
+        std::fprintf(stderr,
+                     "[batch plan] cell tile %d, particle tile %d, %d per claim, %s, targets "
+                     "per lane %d, %s, %d runs: %.4f ms\n",
+                     c.pcTile, c.ppTile, c.listsPerGrab,
+                     c.order == 1 ? "tree order" : "costliest first", c.targetsPerLane,
+                     c.cellSource == 1 ? "cell records" : "wire moments", reps, ms);
+      return ms;
+    };
+    // Every candidate is timed over 2 runs; the 4 fastest and the default
+    // configuration (cell tile 64, particle tile 256, one bucket per claim,
+    // tree order, up to three targets per lane, wire moments) are timed again
+    // over 6 runs, and the default is kept unless another is more than 2%
+    // faster, so that variants of near-equal time do not flip from one plan
+    // to the next. The default runs twice, untimed, before the sweep.
     const int pcts[2] = {64, 32}, ppts[2] = {256, 128}, grabs[2] = {1, 4}, orders[2] = {1, 2},
-              kmaxs[2] = {3, 1};
-    for (int i = 0; i < 32 && e == hipSuccess; ++i) {
-            const int i0 = i & 1, i1 = (i >> 1) & 1, i2 = (i >> 2) & 1, i3 = (i >> 3) & 1,
-                      i4 = i >> 4;
-            ChangaBatchOptions c = o;
-            c.pcTile = o.pcTile ? o.pcTile : pcts[i0];
-            c.ppTile = o.ppTile ? o.ppTile : ppts[i1];
-            c.listsPerGrab = o.listsPerGrab ? o.listsPerGrab : grabs[i2];
-            c.order = o.order ? o.order : orders[i3];
-            c.targetsPerLane = o.targetsPerLane ? o.targetsPerLane : kmaxs[i4];
-            c.zeroVars = 0;
-            // skip duplicates of a fixed option
-            if ((o.pcTile && i0) || (o.ppTile && i1) || (o.listsPerGrab && i2) || (o.order && i3) ||
-                (o.targetsPerLane && i4))
-              continue;
-            BatchArgs a = p->args;
-            a.pc.vars = scratch;
-            a.pp.vars = scratch;
-            a.queue = d_q[c.order - 1];
-            const int grid = grid_for(pick(p->precise, p->kind, c));
-            if (grid <= 0) {
-              e = hipErrorInvalidValue;
-              break;
-            }
-            if (enqueue(p, a, c, grid, s)) {
-              e = hipErrorLaunchFailure;
-              break;
-            }
-            e = hipEventRecord(e0, s);
-            for (int r = 0; r < 2 && e == hipSuccess; ++r)
-              if (enqueue(p, a, c, grid, s)) e = hipErrorLaunchFailure;
-            if (e == hipSuccess) e = hipEventRecord(e1, s);
-            if (e == hipSuccess) e = hipEventSynchronize(e1);
-            float ms = 0.0f;
-            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-            ms *= 0.5f;
-            if (e == hipSuccess && (best < 0.0f || ms < 0.97f * best)) {
-              best = ms;
-              bo = c;
-            }
-          }
+              kmaxs[2] = {3, 1}, sources[2] = {2, 1};
+    std::vector<std::pair<float, ChangaBatchOptions>> swept;
+    ChangaBatchOptions dflt = o;
+    for (int i = 0; i < 64 && e == hipSuccess; ++i) {
+      const int i0 = i & 1, i1 = (i >> 1) & 1, i2 = (i >> 2) & 1, i3 = (i >> 3) & 1,
+                i4 = (i >> 4) & 1, i5 = i >> 5;
+      // skip duplicates of a fixed option
+      if ((o.pcTile && i0) || (o.ppTile && i1) || (o.listsPerGrab && i2) || (o.order && i3) ||
+          (o.targetsPerLane && i4) || (o.cellSource && i5))
+        continue;
+      ChangaBatchOptions c = o;
+      c.pcTile = o.pcTile ? o.pcTile : pcts[i0];
+      c.ppTile = o.ppTile ? o.ppTile : ppts[i1];
+      c.listsPerGrab = o.listsPerGrab ? o.listsPerGrab : grabs[i2];
+      c.order = o.order ? o.order : orders[i3];
+      c.targetsPerLane = o.targetsPerLane ? o.targetsPerLane : kmaxs[i4];
+      c.cellSource = o.cellSource ? o.cellSource : sources[i5];
+      c.zeroVars = 0;
+      if (i == 0 || swept.empty()) {
+        dflt = c;
+        time_config(c, 1);  // warm-up
+      }
+      const float ms = time_config(c, 2);
+      if (ms >= 0.0f) swept.emplace_back(ms, c);
+    }
+    float best = -1.0f;
+    ChangaBatchOptions bo = dflt;
+    if (e == hipSuccess && !swept.empty()) {
+      std::stable_sort(swept.begin(), swept.end(),
+                       [](const auto& u, const auto& v) { return u.first < v.first; });
+      const float dms = time_config(dflt, 6);
+      best = dms;
+      for (size_t k = 0; k < swept.size() && k < 4 && e == hipSuccess; ++k) {
+        const ChangaBatchOptions& c = swept[k].second;
+        if (std::memcmp(&c, &dflt, sizeof(c)) == 0) continue;
+        const float ms = time_config(c, 6);
+        if (ms >= 0.0f && ms < best && ms < 0.98f * dms) {
+          best = ms;
+          bo = c;
+        }
+      }
+    }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (scratch) (void)hipFree(scratch);
@@ -421,6 +478,7 @@ extern "C" int changaBatchPlanCreate(const ChangaGravityLaunch* cells,
     if (!o.listsPerGrab) o.listsPerGrab = 1;
     if (!o.order) o.order = 1;
     if (!o.targetsPerLane) o.targetsPerLane = 3;
+    if (!o.cellSource) o.cellSource = 2;
   }
   p->opt = o;
   p->d_queue = d_q[o.order - 1];
@@ -428,7 +486,7 @@ extern "C" int changaBatchPlanCreate(const ChangaGravityLaunch* cells,
   p->args.queue = p->d_queue;
   p->args.grab = o.listsPerGrab;
   if (nb > 0) {
-    p->grid = grid_for(pick(p->precise, p->kind, o));
+    p->grid = grid_for(pick(p->precise, p->kind, o, o.cellSource == 1));
     if (p->grid <= 0) return bail(fail("batch plan: occupancy", hipErrorInvalidValue, __FILE__,
                                        __LINE__));
     // capture reset (+ zeroing) + persistent kernel; every node is a kernel
@@ -622,7 +680,10 @@ extern "C" int changaBatchedGravityNoAlloc(const ChangaGravityLaunch* cells,
                 hipErrorInvalidValue, __FILE__, __LINE__);
   if (pp.listKind != 0 && pp.listKind != 1)
     return fail("changaBatchedGravityNoAlloc: listKind", hipErrorInvalidValue, __FILE__, __LINE__);
-  ChangaBatchOptions o = {64, 256, 1, 1, 0, 0.0f, 3};
+  if (reinterpret_cast<uintptr_t>(pc.cellRecords) % CHANGA_CELL_RECORD_BYTES != 0)
+    return fail("changaBatchedGravityNoAlloc: cellRecords not 128-byte aligned",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  ChangaBatchOptions o = {64, 256, 1, 1, 0, 0.0f, 3, 0};
   if (opts) {
     o = *opts;
     if (!o.pcTile) o.pcTile = 64;
@@ -631,6 +692,10 @@ extern "C" int changaBatchedGravityNoAlloc(const ChangaGravityLaunch* cells,
     if (!o.order) o.order = 1;
     if (!o.targetsPerLane) o.targetsPerLane = 3;
   }
+  if (o.cellSource == 0) o.cellSource = pc.cellRecords ? 1 : 2;
+  if (o.cellSource < 1 || o.cellSource > 2 || (o.cellSource == 1 && !pc.cellRecords))
+    return fail("changaBatchedGravityNoAlloc: cellSource (1 needs cellRecords)",
+                hipErrorInvalidValue, __FILE__, __LINE__);
   if ((o.pcTile != 32 && o.pcTile != 64) || (o.ppTile != 128 && o.ppTile != 256) ||
       o.listsPerGrab < 1 || o.listsPerGrab > kMaxGrab ||
       (o.targetsPerLane != 1 && o.targetsPerLane != 3))
@@ -644,7 +709,7 @@ extern "C" int changaBatchedGravityNoAlloc(const ChangaGravityLaunch* cells,
   if (int e = na_scratch(&dev, &ws)) return e;
   if (nb == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  const BatchFn fn = pick(pc.tpos64 != nullptr, pp.listKind, o);
+  const BatchFn fn = pick(pc.tpos64 != nullptr, pp.listKind, o, o.cellSource == 1);
   const int grid = grid_for(fn);
   if (grid <= 0)
     return fail("changaBatchedGravityNoAlloc: occupancy", hipErrorInvalidValue, __FILE__,

@@ -43,7 +43,7 @@ struct FusedArgs {
 template <int PCT, int PPT>
 struct FusedLds {
   static constexpr size_t stack = kCoopStack * sizeof(unsigned);
-  static constexpr size_t pc = (size_t)PCT * kPcRec * sizeof(float);
+  static constexpr size_t pc = (size_t)(PCT + 1) * kFwdRec * sizeof(float);
   static constexpr size_t pp = sizeof(PpFwdLds<PPT>);
   static constexpr size_t red = reduce_bytes<1>();
   static constexpr size_t m1 = stack > pc ? stack : pc;

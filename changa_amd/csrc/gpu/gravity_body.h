@@ -20,20 +20,26 @@
 // deterministic. Accelerations and potentials are added into the
 // VariablePartData array (+=), dtGrav takes the max.
 #pragma once
+#include <utility>
+
 #include "common.h"
 
 namespace cgpu {
 
 constexpr int kPcTile = 64;   // cells per staged tile of the plain kernel
-constexpr int kPcRec = 36;    // floats per staged cell record (9 x float4; stored whole,
-                              // 36 keeps records read together 4 LDS banks apart)
+constexpr int kPcRec = 36;    // floats per staged cell record of the adjoint's target pass
+                              // (9 x float4, stored whole: 36 keeps records read together
+                              // 4 LDS banks apart)
+constexpr int kFwdRec = 32;   // floats per staged cell record of the forward loops
+                              // (8 float4 component planes)
 constexpr int kPpTile = 256;  // source particles per staged tile
 
-// Staged cell record. Moments of order n are multiplied by n c_n radius^n
-// (c_n = (2n-1)!!/n!), the replica offset is applied to the centre, and the
-// combinations the trace-free contraction needs are formed once per cell.
-// A cell whose softening overlaps the bucket is stored with negative mass
-// and its softening in slot 4.
+// Staged cell record of the adjoint's target pass (36 floats). Moments of
+// order n are multiplied by n c_n radius^n (c_n = (2n-1)!!/n!), the replica
+// offset is applied to the centre, and the combinations the trace-free
+// contraction needs are formed once per cell. A cell whose softening
+// overlaps the bucket is stored with negative mass and its softening in
+// slot 4.
 enum {
   R_CX = 0, R_CY, R_CZ, R_M,
   R_Q2 = 4,    // xx xy xz yy yz, then -(xx+yy)
@@ -42,6 +48,29 @@ enum {
                // then -(xxxx+xxyy), -(yyyy+xxyy), -(xxxy+xyyy)
   R_SOFT = 4
 };
+
+// Staged cell record of the forward loops (32 floats, 8 float4): the same
+// scaled moments and trace combinations, with the centre (replica offset
+// applied, relative to the bucket centre in the precision form) in the first
+// three floats, the mass in F_M and, in F_SOFT, the cell's softening when
+// its softening overlaps the bucket (a spline monopole) or -1 for a
+// hexadecapole cell.
+enum {
+  F_CX = 0, F_CY, F_CZ,
+  F_Q4A = 3, F_Q4B, F_Q4C,  // -(xxxx+xxyy), -(yyyy+xxyy), -(xxxy+xyyy)
+  F_M = 6, F_SOFT = 7,
+  F_Q2 = 8,    // xx xy xz yy yz, then -(xx+yy)
+  F_Q3 = 14,   // xxx xyy xxy yyy xxz yyz xyz, then -(xxx+xyy), -(xxy+yyy)
+  F_Q4 = 23    // xxxx xyyy xxxy yyyy xxxz yyyz xxyy xxyz xyyz
+};
+
+// Cell record in device memory (changaCellRecords): 128 bytes, one cache
+// line per cell, so that a tile of cells is fetched as whole lines (eight
+// lanes per record) instead of one record per lane. Layout: the centre of
+// mass as 3 doubles (floats 0-5), mass (6), softening (7), then floats
+// F_Q2.. of the forward record. The three Q4 combinations and the centre
+// relative to the bucket are formed when a tile is staged.
+constexpr int kCellRecFloats = 32;
 
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
@@ -96,7 +125,7 @@ __device__ __forceinline__ void pp_interact(float rx, float ry, float rz, float 
   }
 }
 
-// Hexadecapole cell at r = tgt - cm from a staged record (see the enum).
+// Hexadecapole cell at r = tgt - cm from a staged forward record (F_ enum).
 // With the harmonic polynomials of r (not of the unit vector: the powers of
 // 1/|r| are applied at the end), V_n = Q_n . r^(n-1) (the gradients of the
 // harmonic polynomials Q_n . r^n, z eliminated by tracelessness),
@@ -133,12 +162,13 @@ __device__ __forceinline__ void pc_eval(float x, float y, float z, const PcRec& 
   const float dir = rsq(r2);
   const float4 c0 = R.c0, c1 = R.c1, c2 = R.c2, c3 = R.c3, c4 = R.c4, c5 = R.c5, c6 = R.c6,
                c7 = R.c7;
-  const float m = c0.w;
-  const float qxx = c1.x, qxy = c1.y, qxz = c1.z, qyy = c1.w, qyz = c2.x, qzz = c2.y;
-  const float qxxx = c2.z, qxyy = c2.w, qxxy = c3.x, qyyy = c3.y, qxxz = c3.z, qyyz = c3.w,
-              qxyz = c4.x, q3a = c4.y, q3b = c4.z;
-  const float qxxxx = c4.w, qxyyy = c5.x, qxxxy = c5.y, qyyyy = c5.z, qxxxz = c5.w, qyyyz = c6.x,
-              qxxyy = c6.y, qxxyz = c6.z, qxyyz = c6.w, q4a = c7.x, q4b = c7.y, q4c = c7.z;
+  // forward record: c0 (cx cy cz q4a), c1 (q4b q4c m soft), then Q2, Q3, Q4
+  const float m = c1.z;
+  const float qxx = c2.x, qxy = c2.y, qxz = c2.z, qyy = c2.w, qyz = c3.x, qzz = c3.y;
+  const float qxxx = c3.z, qxyy = c3.w, qxxy = c4.x, qyyy = c4.y, qxxz = c4.z, qyyz = c4.w,
+              qxyz = c5.x, q3a = c5.y, q3b = c5.z;
+  const float qxxxx = c5.w, qxyyy = c6.x, qxxxy = c6.y, qyyyy = c6.z, qxxxz = c6.w, qyyyz = c7.x,
+              qxxyy = c7.y, qxxyz = c7.z, qxyyz = c7.w, q4a = c0.w, q4b = c1.x, q4c = c1.y;
   const float x2 = x * x, y2 = y * y, z2 = z * z;
   // rank-2 harmonic polynomials
   const float fxx = x2 - z2, fyy = y2 - z2;
@@ -228,6 +258,20 @@ __device__ __forceinline__ T load_entry(const T* p) {
   else return *p;
 }
 
+// Wire record of list entry e, for every lane (no branch: a load under a
+// divergent branch makes the compiler wait for it at the branch's end, which
+// would serialize the prefetch): the caller clamps e into the list.
+template <bool STREAM>
+__device__ __forceinline__ void load_cell_all(const ChangaGravityLaunch& a, const ILCell* cells,
+                                              int e, WireCell& w) {
+  const ILCell c = load_entry<STREAM>(cells + e);
+  w.idx = c.index;
+  w.oid = c.offsetID;
+  const float* src = reinterpret_cast<const float*>(a.moments) + (size_t)c.index * 27;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) w.v[k] = src[k];
+}
+
 template <bool PRECISE, bool STREAM = true>
 __device__ __forceinline__ void load_cell(const ChangaGravityLaunch& a, const ILCell* cells, int e,
                                           int end, WireCell& w) {
@@ -284,6 +328,132 @@ __device__ __forceinline__ void stage_cell(const ChangaGravityLaunch& a, const W
 #pragma unroll
   for (int k = 0; k < kPcRec / 4; ++k)
     dst[k * STRIDE] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+}
+
+// Replica-shifted centre of a cell relative to the bucket centre (precision
+// form: FP64 subtraction, then FP32) or absolute (plain form).
+template <bool PRECISE>
+__device__ __forceinline__ void cell_centre(double mx, double my, double mz, float fx, float fy,
+                                            float fz, int oid, float L, double cx0, double cy0,
+                                            double cz0, float& x, float& y, float& z) {
+  // every operation rounded on its own (no contraction into FMAs), so that
+  // both staging paths and the records give the same bits
+#pragma clang fp contract(off)
+  if (PRECISE) {
+    x = (float)(mx + (double)off_x(oid) * (double)L - cx0);
+    y = (float)(my + (double)off_y(oid) * (double)L - cy0);
+    z = (float)(mz + (double)off_z(oid) * (double)L - cz0);
+  } else {
+    x = (float)off_x(oid) * L + fx;
+    y = (float)off_y(oid) * L + fy;
+    z = (float)off_z(oid) * L + fz;
+  }
+}
+
+// wire record -> forward record (in registers), then to LDS (float4 k at
+// dst[k * STRIDE])
+template <bool PRECISE, int STRIDE>
+__device__ __forceinline__ void stage_cell_fwd(const ChangaGravityLaunch& a, const WireCell& w,
+                                               float L, double cx0, double cy0, double cz0,
+                                               float4* dst) {
+#pragma clang fp contract(off)
+  float r[kFwdRec];
+  const int oid = w.oid;
+  double mx = 0.0, my = 0.0, mz = 0.0;
+  if (PRECISE) {
+    const size_t ci = (size_t)w.idx * 3;
+    mx = a.cm64[ci];
+    my = a.cm64[ci + 1];
+    mz = a.cm64[ci + 2];
+  }
+  cell_centre<PRECISE>(mx, my, mz, w.v[3], w.v[4], w.v[5], oid, L, cx0, cy0, cz0, r[F_CX],
+                       r[F_CY], r[F_CZ]);
+  const float u = w.v[0], u2 = u * u;
+  const float s2 = 3.0f * u2, s3 = 7.5f * u2 * u, s4 = 17.5f * u2 * u2;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r[F_Q2 + k] = s2 * w.v[6 + k];
+  r[F_Q2 + 5] = -(r[F_Q2 + 0] + r[F_Q2 + 3]);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) r[F_Q3 + k] = s3 * w.v[11 + k];
+  r[F_Q3 + 7] = -(r[F_Q3 + 0] + r[F_Q3 + 1]);
+  r[F_Q3 + 8] = -(r[F_Q3 + 2] + r[F_Q3 + 3]);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) r[F_Q4 + k] = s4 * w.v[18 + k];
+  r[F_Q4A] = -(r[F_Q4 + 0] + r[F_Q4 + 6]);
+  r[F_Q4B] = -(r[F_Q4 + 3] + r[F_Q4 + 6]);
+  r[F_Q4C] = -(r[F_Q4 + 2] + r[F_Q4 + 1]);
+  r[F_M] = w.v[2];
+  r[F_SOFT] = (oid & CHANGA_ILCELL_SOFTENED) ? w.v[1] : -1.0f;
+#pragma unroll
+  for (int k = 0; k < kFwdRec / 4; ++k)
+    dst[k * STRIDE] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+}
+
+// Cell records (changaCellRecords): one 128-byte line per cell. A tile of
+// TILE cells is fetched by TILE / 8 loads per lane: load j gives lane l the
+// float4 l % 8 of the tile's cell 8 j + l / 8, so eight consecutive lanes
+// read one record's 128 contiguous bytes and every load covers eight whole
+// cache lines. In the component planes (stride PS = TILE + 1 float4) the
+// eight float4 of one cell then land in eight different bank groups.
+// (native vectors: HIP's float4 is a struct whose copies become memcpy calls,
+// which keep the tile in scratch memory instead of registers)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int TILE>
+struct RecTile {
+  f32x4 v[TILE / 8];
+};
+
+// cidx: this lane's cell index (its entry of the tile), valid for lanes < nc;
+// slots past nc fetch the tile's last cell again (no branch around a load)
+template <int J, int TILE>
+__device__ __forceinline__ void load_rec_tile_j(const float4* recs, int cidx, int nc, int k,
+                                                int c0, RecTile<TILE>& f) {
+  if constexpr (J < TILE / 8) {
+    const int idx = __shfl(cidx, min(8 * J + c0, nc - 1), kWave);
+    f.v[J] = reinterpret_cast<const f32x4*>(recs)[(size_t)idx * (kCellRecFloats / 4) + k];
+    load_rec_tile_j<J + 1, TILE>(recs, cidx, nc, k, c0, f);
+  }
+}
+
+// (the tile's slots are indexed at compile time, so that the tile stays in
+// registers)
+template <int TILE>
+__device__ __forceinline__ void load_rec_tile(const float4* recs, int cidx, int nc, int lane,
+                                              RecTile<TILE>& f) {
+  load_rec_tile_j<0, TILE>(recs, cidx, nc, lane & 7, lane >> 3, f);
+}
+
+// Write a fetched tile into the component planes (float4 k of cell c at
+// rec4[k * PS + c]), then form each cell's centre (replica offset, bucket
+// centre), its softened flag and the three Q4 combinations: lane c finishes
+// cell c. Ends with a barrier.
+template <bool PRECISE, int TILE, int PS>
+__device__ __forceinline__ void stage_rec_tile(const RecTile<TILE>& f, int nc, int lane, int oid,
+                                               float L, double cx0, double cy0, double cz0,
+                                               float4* rec4) {
+#pragma clang fp contract(off)
+  f32x4* dst = reinterpret_cast<f32x4*>(rec4) + (lane & 7) * PS + (lane >> 3);
+  [&]<int... J>(std::integer_sequence<int, J...>) {
+    ((dst[8 * J] = f.v[J]), ...);
+  }(std::make_integer_sequence<int, TILE / 8>());
+  __syncthreads();
+  if (lane < nc) {
+    float4* r = rec4 + lane;
+    const float4 p0 = r[0], p1 = r[PS], p5 = r[5 * PS], p6 = r[6 * PS], p7 = r[7 * PS];
+    const double mx = __hiloint2double(__float_as_int(p0.y), __float_as_int(p0.x));
+    const double my = __hiloint2double(__float_as_int(p0.w), __float_as_int(p0.z));
+    const double mz = __hiloint2double(__float_as_int(p1.y), __float_as_int(p1.x));
+    float x, y, z;
+    cell_centre<PRECISE>(mx, my, mz, (float)mx, (float)my, (float)mz, oid, L, cx0, cy0, cz0, x,
+                         y, z);
+    // F_Q4 + 0 = p5.w (xxxx), +1 = p6.x (xyyy), +2 = p6.y (xxxy), +3 = p6.z (yyyy),
+    // +6 = p7.y (xxyy)
+    const float q4a = -(p5.w + p7.y), q4b = -(p6.z + p7.y), q4c = -(p6.y + p6.x);
+    r[0] = make_float4(x, y, z, q4a);
+    r[PS] = make_float4(q4b, q4c, p1.z, (oid & CHANGA_ILCELL_SOFTENED) ? p1.w : -1.0f);
+  }
+  __syncthreads();
 }
 
 
@@ -470,33 +640,65 @@ __device__ __forceinline__ void reduce_store(void* lds, const Lanes& l, const Ac
 }
 
 // Cells [start, end) of a.list onto the chunk's targets. Tiles of TILE <= 64
-// cells: lane c stages cell c of the tile into rec4 (TILE * kPcRec floats of
-// LDS) as component planes: float4 k of cell c at rec4[k * TILE + c], so
-// lanes reading or writing different cells touch consecutive 16-byte words
-// (no bank conflicts however many distinct cells a read instruction
-// touches: one per lane when a bucket has a single target). The next tile's
-// wire records are in flight while this one is evaluated. A tile without
-// softened cells (ballot) runs the hexadecapole loop with no per-cell test.
-template <bool PRECISE, int TILE, bool STREAM = true, int K = 1>
+// cells are staged into rec4 ((TILE + 1) * kFwdRec floats of LDS) as component
+// planes: float4 k of cell c at rec4[k * (TILE + 1) + c], so lanes reading or
+// writing different cells touch consecutive 16-byte words (no bank
+// conflicts however many distinct cells a read instruction touches: one per
+// lane when a bucket has a single target). The next tile's data is in flight
+// while this one is evaluated. REC false: lane c fetches cell c's wire
+// record and stages it; REC true: the tile's 128-byte cell records
+// (a.cellRecords) are fetched as whole lines, eight lanes per record
+// (load_rec_tile, stage_rec_tile). A tile without softened cells (ballot)
+// runs the hexadecapole loop with no per-cell test.
+template <bool PRECISE, int TILE, bool STREAM = true, int K = 1, bool REC = false>
 __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start, int end,
                                          double cx0, double cy0, double cz0,
                                          const Target (&t)[K], const Lanes& l, float4* rec4,
                                          Acc64 (&acc)[K]) {
-  static_assert(TILE <= kWave, "one staged cell per lane");
+  static_assert(TILE <= kWave && TILE % 8 == 0, "one staged cell per lane");
+  constexpr int PS = TILE + 1;  // plane stride (float4)
   const float L = a.fperiod;
   const ILCell* cells = static_cast<const ILCell*>(a.list);
-  const bool stager = l.lane < TILE;
+  // list entry of this lane in the tile at base (clamped into the list)
+  auto entry_at = [&](int base) { return min(base + min(l.lane, TILE - 1), end - 1); };
+  // (end - 1 >= start: the loop below runs only for a non-empty list)
+  // REC false: the next tile's wire records. REC true: the next tile's
+  // record lines, its offsetIDs, and the entries of the tile after it (read
+  // a tile ahead, so that the index shuffle of a fetch never waits on them).
   WireCell w;
-  w.oid = 0;
-  load_cell<PRECISE, STREAM>(a, cells, start + l.lane, stager ? end : 0, w);
+  RecTile<TILE> rt;
+  ILCell enext;
+  int oid_next = 0;
+  const float4* recs = static_cast<const float4*>(a.cellRecords);
+  // every fetch is unconditional (past the list's end it fetches the last
+  // cell again): a load under any branch makes the compiler wait for it where
+  // the branch joins, which would put the fetch's latency before the
+  // evaluation it is meant to overlap
+  auto fetch = [&](int base) {
+    if constexpr (REC) {
+      load_rec_tile<TILE>(recs, enext.index, max(1, min(TILE, end - base)), l.lane, rt);
+      oid_next = enext.offsetID;
+      enext = load_entry<STREAM>(cells + entry_at(base + TILE));
+    } else {
+      load_cell_all<STREAM>(a, cells, entry_at(base), w);
+    }
+  };
+  if (start >= end) return;
+  if constexpr (REC) enext = load_entry<STREAM>(cells + entry_at(start));
+  fetch(start);
   for (int base = start; base < end; base += TILE) {
     const int nc = min(TILE, end - base);
     const bool mine = l.lane < nc;
-    const bool soft_tile = __ballot(mine && (w.oid & CHANGA_ILCELL_SOFTENED)) != 0ull;
-    if (mine) stage_cell<PRECISE, TILE>(a, w, L, cx0, cy0, cz0, rec4 + l.lane);
-    __syncthreads();
+    const int oid = REC ? oid_next : w.oid;
+    const bool soft_tile = __ballot(mine && (oid & CHANGA_ILCELL_SOFTENED)) != 0ull;
+    if constexpr (REC) {
+      stage_rec_tile<PRECISE, TILE, PS>(rt, nc, l.lane, oid, L, cx0, cy0, cz0, rec4);
+    } else {
+      if (mine) stage_cell_fwd<PRECISE, PS>(a, w, L, cx0, cy0, cz0, rec4 + l.lane);
+      __syncthreads();
+    }
     // issued after the barrier, whose fence would otherwise wait for them
-    load_cell<PRECISE, STREAM>(a, cells, base + TILE + l.lane, stager ? end : 0, w);
+    fetch(base + TILE);
     if (l.live) {
       Acc32 A[K];
 #pragma unroll
@@ -505,7 +707,7 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
       const int full = nc / l.nslot;
       const float4* r4 = rec4 + l.slot;
       auto hex = [&](const float4* r) {
-        const PcRec R = load_rec<TILE>(r);
+        const PcRec R = load_rec<PS>(r);
 #pragma unroll
         for (int j = 0; j < K; ++j) {
           pc_eval(t[j].x - R.c0.x, t[j].y - R.c0.y, t[j].z - R.c0.z, R, t[j].m, A[j]);
@@ -519,13 +721,13 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
         if (l.slot < nc - full * l.nslot) hex(r4);
       } else {
         auto cell = [&](const float4* r) {
-          const float4 h = r[0];
-          if (h.w < 0.0f) {  // softened cell: spline monopole at the cm
-            const float sc = r[TILE].x;
+          const float4 p1 = r[PS];
+          if (p1.w >= 0.0f) {  // softened cell: spline monopole at the cm
+            const float4 h = r[0];
 #pragma unroll
             for (int j = 0; j < K; ++j)
-              pp_interact(h.x - t[j].x, h.y - t[j].y, h.z - t[j].z, -h.w, sc, t[j].m, t[j].soft,
-                          A[j]);
+              pp_interact(h.x - t[j].x, h.y - t[j].y, h.z - t[j].z, p1.z, p1.w, t[j].m,
+                          t[j].soft, A[j]);
           } else {
             hex(r);
           }

@@ -152,7 +152,7 @@ class Evaluation:
     always every particle."""
 
     def __init__(self, w, precise=False, pp_compact=True, n_threads=0, device="cuda:0",
-                 lists="host", tree="host"):
+                 lists="host", tree="host", cell_records=True):
         if lists not in ("host", "gpu", "none") or tree not in ("host", "gpu"):
             raise ValueError("lists: 'host', 'gpu' or 'none' (fused form only); tree: 'host' or "
                              "'gpu'")
@@ -161,6 +161,9 @@ class Evaluation:
         self.w = w
         self.precise = precise
         self.pp_compact = pp_compact
+        # 128-byte cell records (changaCellRecords) for the particle-cell
+        # launches, rebuilt with every moment build; False: wire moments
+        self.cell_records = bool(cell_records)
         self.list_source = lists
         self.device = torch.device(device)
         periodic = bool(w["periodic"])
@@ -319,6 +322,8 @@ class Evaluation:
         self.d_moments = torch.zeros((nn, 27), dtype=torch.float32, device=self.device)
         self.d_node64 = torch.zeros((nn, 34), dtype=torch.float64, device=self.device)
         self.d_cm64 = torch.zeros((nn, 3), dtype=torch.float64, device=self.device)
+        self.d_cellrec = (torch.zeros((nn, 32), dtype=torch.float32, device=self.device)
+                          if self.cell_records else None)
         self.d_starts = self._t(self.t_starts)
         self.d_sizes = self._t(self.t_sizes)
         self.stream = torch.cuda.current_stream(self.device)
@@ -501,6 +506,10 @@ class Evaluation:
             _dp(self.d_mass64), _dp(self.d_soft64), _dp(self.d_moments), _dp(self.d_node64),
             _dp(self.d_cm64), self._sp())
         gpu.check(rc, "changaBuildMoments")
+        if self.d_cellrec is not None:
+            gpu.check(gpu.lib().changaCellRecords(_dp(self.d_moments), _dp(self.d_cm64),
+                                                  tr["n_nodes"], _dp(self.d_cellrec),
+                                                  self._sp()), "changaCellRecords")
 
     def _setup_ewald(self):
         w = self.w
@@ -576,6 +585,8 @@ class Evaluation:
             a.tpos64 = self.d_tpos64.data_ptr()
             a.spos64 = self.d_pos64.data_ptr()
             a.cm64 = self.d_cm64.data_ptr()
+        if pc and self.d_cellrec is not None:
+            a.cellRecords = self.d_cellrec.data_ptr()
         return a
 
     def zero(self):
@@ -658,14 +669,15 @@ class Evaluation:
 
     # ------------------------------------------------------------------ batched form
     def batch_plan(self, pc_tile=0, pp_tile=0, lists_per_grab=0, order=0, zero_vars=True,
-                   targets_per_lane=0):
+                   targets_per_lane=0, cell_source=0):
         """Make (replacing any previous) the batched-form plan over this
-        evaluation's lists; options left at 0 are autotuned. Returns the
-        chosen ChangaBatchOptions."""
+        evaluation's lists; options left at 0 are autotuned. cell_source 1:
+        the cell records, 2: the wire moments. Returns the chosen
+        ChangaBatchOptions."""
         self._need_lists()
         self.free_batch()
         o = gpu.ChangaBatchOptions(pc_tile, pp_tile, lists_per_grab, order, int(zero_vars), 0.0,
-                                   targets_per_lane)
+                                   targets_per_lane, cell_source)
         self._batch_launches = (self._launch(True), self._launch(False))
         plan = ctypes.c_void_p()
         gpu.check(gpu.lib().changaBatchPlanCreate(ctypes.byref(self._batch_launches[0]),

@@ -66,12 +66,14 @@ class ChangaGravityLaunch(ctypes.Structure):
                 ("sources", c_void_p), ("list", c_void_p), ("marks", c_void_p),
                 ("starts", c_void_p), ("sizes", c_void_p), ("numBuckets", c_int),
                 ("listKind", c_int), ("fperiod", c_float), ("variant", c_int),
-                ("tpos64", c_void_p), ("spos64", c_void_p), ("cm64", c_void_p)]
+                ("tpos64", c_void_p), ("spos64", c_void_p), ("cm64", c_void_p),
+                ("cellRecords", c_void_p)]
 
 
 class ChangaBatchOptions(ctypes.Structure):
     _fields_ = [("pcTile", c_int), ("ppTile", c_int), ("listsPerGrab", c_int), ("order", c_int),
-                ("zeroVars", c_int), ("tunedMs", c_float), ("targetsPerLane", c_int)]
+                ("zeroVars", c_int), ("tunedMs", c_float), ("targetsPerLane", c_int),
+                ("cellSource", c_int)]
 
 
 class ChangaAdjointLaunch(ctypes.Structure):
@@ -133,6 +135,7 @@ EXPORTS = [
     "changaBuildListsWorkBytes", "changaBuildListsCount", "changaBuildListsFill",
     "changaBuildTreeWorkBytes", "changaBuildTree", "changaFusedGravityWorkBytes",
     "changaFusedGravity", "changaBatchedGravityNoAlloc", "changaEwaldRoot", "changaEwaldResident",
+    "changaCellRecords",
 ]
 
 # doubles of the device Ewald root record of changaEwaldRoot
@@ -176,6 +179,8 @@ def lib():
     L.changaGpuLastError.restype = c_int
     L.changaNodeGravity.argtypes = [ctypes.POINTER(ChangaGravityLaunch), P]
     L.changaNodeGravity.restype = c_int
+    L.changaCellRecords.argtypes = [P, P, c_int, P, P]
+    L.changaCellRecords.restype = c_int
     L.changaPartGravity.argtypes = [ctypes.POINTER(ChangaGravityLaunch), P]
     L.changaPartGravity.restype = c_int
     L.changaZeroVars.argtypes = [P, c_int, P]

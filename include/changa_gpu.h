@@ -132,9 +132,21 @@ typedef struct {
   const double *tpos64;                  /* 3 per target particle, or NULL */
   const double *spos64;                  /* 3 per source particle, or NULL */
   const double *cm64;                    /* 3 per cell, or NULL */
+  const void *cellRecords;               /* 128-byte cell records of changaCellRecords (built
+                                            from these moments and cm64), or NULL: with them a
+                                            tile of cells is fetched as whole cache lines */
 } ChangaGravityLaunch;
 
 int changaNodeGravity(const ChangaGravityLaunch *a, void *stream);
+/* No counterpart in the reference (ChangaGravityLaunch.cellRecords): one
+   128-byte record per cell, built from the wire moments and the FP64 centres
+   of mass (cm64, or NULL: the wire centres), with the moments pre-scaled for
+   the hexadecapole evaluation. `records` holds CHANGA_CELL_RECORD_BYTES x
+   numCells bytes, 128-byte aligned (hipMalloc gives 256). Rebuild after every
+   moment build. */
+#define CHANGA_CELL_RECORD_BYTES 128
+int changaCellRecords(const CudaMultipoleMoments *moments, const double *cm64, int numCells,
+                      void *records, void *stream);
 int changaPartGravity(const ChangaGravityLaunch *a, void *stream);
 int changaZeroVars(VariablePartData *vars, int n, void *stream);
 /* dst += src (a and potential; dtGrav = max): merges a result array filled on
@@ -206,6 +218,9 @@ typedef struct {
   float tunedMs;     /* out: time of one evaluation with the chosen options, or 0 */
   int targetsPerLane; /* 1: one target per lane; 3: one to three, by bucket size (more slots
                          busy, but a kernel three chunk loops long); 0 = autotune */
+  int cellSource;     /* 1: the launch's cellRecords (whole-line fetches: faster when buckets
+                         have few targets); 2: the wire moments (per-lane fetches); 0 =
+                         autotune between the two when the launch has cellRecords, else 2 */
 } ChangaBatchOptions;
 
 /* cells: the particle-cell launch (listKind 0); parts: the particle-particle
@@ -234,8 +249,8 @@ int changaBatchPlanDestroy(ChangaBatchPlan *plan);
    what it reads of it. Calls that overlap on one GPU (two streams, two host
    threads) race on that array: serialize them. opts may be NULL (cell tile
    64, particle tile 256, one bucket per claim, up to three targets per
-   lane); fields left at 0 take those values; order must be 1 (tree order: the costliest-first order needs a
-   sort); zeroVars zeroes the results of the particles the buckets cover.
+   lane, the cell records when the launch has them); fields left at 0 take those values;
+   order must be 1 (tree order: the costliest-first order needs a sort); zeroVars zeroes the results of the particles the buckets cover.
    More than CHANGA_NOALLOC_MAX_BUCKETS buckets is an error. Results equal
    changaBatchPlanLaunch's with the same options. */
 #define CHANGA_NOALLOC_MAX_BUCKETS (1 << 22)

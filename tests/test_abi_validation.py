@@ -129,3 +129,22 @@ def test_no_alloc_rejects_the_costliest_first_order(lib):
     assert lib.changaBatchedGravityNoAlloc(ctypes.byref(pc), ctypes.byref(pp), ctypes.byref(o),
                                            None) != 0
     assert "tree order" in _last(lib)[1]
+
+
+def test_cell_records_reject_misalignment(lib):
+    rc = lib.changaCellRecords(ctypes.c_void_p(DUMMY), None, 2, ctypes.c_void_p(DUMMY + 4), None)
+    assert rc != 0
+    assert "aligned" in _last(lib)[1]
+    rc = lib.changaCellRecords(None, None, 2, ctypes.c_void_p(DUMMY), None)
+    assert rc != 0 and "missing" in _last(lib)[1]
+    assert lib.changaCellRecords(None, None, 0, None, None) == 0
+
+
+def test_node_gravity_rejects_misaligned_records(lib):
+    a = gpu.ChangaGravityLaunch()
+    a.numBuckets = 1
+    for name in ("targets", "vars", "moments", "list", "marks", "starts", "sizes"):
+        setattr(a, name, DUMMY)
+    a.cellRecords = DUMMY + 64
+    assert lib.changaNodeGravity(ctypes.byref(a), None) != 0
+    assert "aligned" in _last(lib)[1]
