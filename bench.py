@@ -279,10 +279,8 @@ def main():
     variant = {"batch_kernel": f"batch_kernel<{opts.pcTile},{opts.ppTile},{opts.targetsPerLane},"
                                f"{rec}>",
                "fused_kernel": "fused_kernel<64,256>"}.get(dom, dom)
-    queue = ({1: "tree", 2: "costliest"}[opts.order] + f"/{opts.listsPerGrab}"
-             if dom == "batch_kernel" else None)
     traffic, traffic_kernel = traffic_from_profile(
-        tag, variant + (" " + queue if queue else ""))
+        tag, variant + (" graph replays" if dom == "batch_kernel" else ""))
     out["roofline"]["kernel"] = variant
     out["roofline"]["kernel_ms"] = round(dom_ms, 4)
     out["roofline"]["traffic"] = traffic
@@ -513,11 +511,12 @@ def grad_accuracy(ev, wt, w, n_samples=4096):
 
 def traffic_from_profile(tag, kernel):
     """(HBM bytes per launch, the profiled kernel) of `kernel` (for the
-    batched kernel, of its variant and queue options, e.g.
-    "batch_kernel<64,256,3,false> tree/1": cell tile, particle tile, targets
-    per lane, cell records, order, buckets per claim), from the committed rocprofv3 --pmc
+    batched kernel, the graph replays of its variant, e.g.
+    "batch_kernel<64,256,3,false> graph replays": cell tile, particle tile,
+    targets per lane, cell records), from the committed rocprofv3 --pmc
     summary of this workload (FETCH_SIZE and WRITE_SIZE, copy-calibrated).
-    (None, None) when the summary holds no figure for exactly that kernel."""
+    (None, None) when the summary holds no figure for exactly that kernel
+    (the profiled plan chose another variant)."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}.md")
     if not os.path.exists(path):
         return None, None

@@ -11,8 +11,13 @@ namespace cgpu {
 // Per-bucket particle-cell kernel: one 64-lane workgroup per bucket, buckets
 // mapped to XCDs in contiguous ranges (xcd_bucket).
 // ---------------------------------------------------------------------------
+// REC: cells from the 128-byte cell records in tiles of 32 (their prefetch
+// then fits three wavefronts per SIMD), else from the wire moments in tiles
+// of kPcTile.
 template <bool PRECISE, bool REC>
-__global__ __launch_bounds__(64) void pc_kernel(ChangaGravityLaunch a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void pc_kernel(
+    ChangaGravityLaunch a) {
+  constexpr int TILE = REC ? 32 : kPcTile;
   // staged tile; reused for the slot reduction at the end of each chunk
   __shared__ float4 rec4[(kPcTile + 1) * (kFwdRec / 4)];
   static_assert(sizeof(rec4) >= reduce_bytes<3>(), "reduction");
@@ -34,7 +39,7 @@ __global__ __launch_bounds__(64) void pc_kernel(ChangaGravityLaunch a) {
       Acc64 acc[K];
 #pragma unroll
       for (int j = 0; j < K; ++j) acc[j] = {0.0, 0.0, 0.0, 0.0, 0.0f};
-      pc_cells<PRECISE, kPcTile, true, K, REC>(a, start, end, cx0, cy0, cz0, t, l, rec4, acc);
+      pc_cells<PRECISE, TILE, true, K, REC>(a, start, end, cx0, cy0, cz0, t, l, rec4, acc);
       reduce_store<K>(rec4, l, acc, a.vars + bstart + t0);
     });
   }

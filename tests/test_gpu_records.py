@@ -6,10 +6,11 @@ particle-cell loops fetch a tile of them as whole lines. The records are
 formed with the same FP32 operations as the per-lane staging of the wire
 moments, and the loops evaluate the same staged values in the same order, so:
   - the records equal, bit for bit, a numpy restatement of that staging;
-  - forces with records equal forces from the wire moments bit for bit, for
-    the per-bucket kernel and every tile variant of the batched kernel, in
-    the precision and the plain form, with softened cells and with periodic
-    replicas.
+  - forces with records equal forces from the wire moments bit for bit for
+    every tile variant of the batched kernel, in the precision and the plain
+    form, with softened cells and with periodic replicas; the per-bucket
+    kernel (tiles of 32 cells with records, 64 without) to one FP32 rounding
+    of its per-tile sums, dtGrav bit for bit.
 """
 import numpy as np
 import pytest
@@ -82,7 +83,14 @@ def test_records_equal_wire_path(device, case, precise):
     ev_w = engine.Evaluation(w, precise=precise, device=device, cell_records=False)
     if case == "softened":
         assert ev_r.counts[1] > 0, "workload must exercise softened cells"
-    np.testing.assert_array_equal(_pc(ev_r), _pc(ev_w))
+    # the per-bucket kernel stages records in tiles of 32 cells and wire
+    # moments in tiles of 64: the FP32 per-tile sums round in a different
+    # order (one rounding of each, 1e-6 of the component's largest
+    # magnitude); dtGrav is a max over the same terms
+    pr, pw = _pc(ev_r), _pc(ev_w)
+    scale = np.abs(pw[:, :4]).max(axis=0)
+    assert (np.abs(pr[:, :4].astype(np.float64) - pw[:, :4]) / scale).max() <= 1e-6
+    np.testing.assert_array_equal(pr[:, 4], pw[:, 4])
     for pct in (32, 64):
         for kmax in (1, 3):
             o = ev_r.batch_plan(pct, 256, 1, 1, targets_per_lane=kmax, cell_source=1)
