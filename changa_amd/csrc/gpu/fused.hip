@@ -16,12 +16,16 @@
 // Force: the bucket's targets then run the cell and particle loops of the
 // list path (gravity_body.h) over the scratch lists, precision form, into
 // FP64 accumulators and one update of the result array per chunk.
+#include <algorithm>
+
 #include "gravity_body.h"
 #include "walk_rec.h"
 
 namespace cgpu {
 
 constexpr int kFusedPcTile = 64;
+constexpr int kFusedRecTile = 32;  // cell tile with the 128-byte records (their prefetch fits
+                                   // the four-wavefront register budget)
 constexpr int kFusedPpTile = 256;
 constexpr int kFusedRanges = kNumXcd;
 constexpr int kFusedCounterStride = 32;
@@ -85,7 +89,9 @@ __device__ bool fused_walk(const FusedArgs& a, int b, int lane, unsigned* stack,
 // Four wavefronts per SIMD (128 VGPRs, a few spills outside the force loops)
 // measured 11-12% faster than the three that 141 VGPRs allow: the walk's load
 // latency is hidden behind more resident wavefronts.
-template <int PCT, int PPT>
+// REC: cells from the 128-byte cell records (a.pc.cellRecords), whole-line
+// fetches in tiles of PCT.
+template <int PCT, int PPT, bool REC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fused_kernel(
     FusedArgs a) {
   __shared__ float4 lds[FusedLds<PCT, PPT>::f4];
@@ -130,8 +136,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void fu
       Acc64 acc[1] = {{0.0, 0.0, 0.0, 0.0, 0.0f}};
       // the scratch is re-read per chunk and rewritten per bucket: plain loads
       // keep it in L2
-      pc_cells<true, PCT, false, 1>(a.pc, cbase, cbase + nhex + nsoft, cx0, cy0, cz0, t, l, lds,
-                                    acc);
+      pc_cells<true, PCT, false, 1, REC>(a.pc, cbase, cbase + nhex + nsoft, cx0, cy0, cz0, t, l,
+                                         lds, acc);
       pp_sources<true, 1, PPT, false, 1>(a.pp, rbase, rbase + nruns, cx0, cy0, cz0, t, l, ps,
                                          acc);
       reduce_store<1>(lds, l, acc, a.pc.vars + bstart + t0);
@@ -146,12 +152,17 @@ __global__ void fused_reset_kernel(int* counters) {
 
 using FusedFn = void (*)(FusedArgs);
 
-static int fused_grid(int* grid) {
+static FusedFn fused_fn(bool rec) {
+  return rec ? fused_kernel<kFusedRecTile, kFusedPpTile, true>
+             : fused_kernel<kFusedPcTile, kFusedPpTile, false>;
+}
+
+static int fused_grid(int* grid, bool rec) {
   int dev = 0, ncu = 0, per = 0;
   CGPU_CHECK(hipGetDevice(&dev));
   CGPU_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  CGPU_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per, (const void*)fused_kernel<kFusedPcTile, kFusedPpTile>, kWave, 0));
+  CGPU_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)fused_fn(rec),
+                                                          kWave, 0));
   *grid = ncu * (per > 0 ? per : 1);
   return 0;
 }
@@ -176,9 +187,12 @@ static FusedWork fused_layout(int grid, int numNodes, int maxCells, int maxRuns)
 using namespace cgpu;
 
 extern "C" size_t changaFusedGravityWorkBytes(int numNodes, int maxCells, int maxRuns) {
-  int grid = 0;
-  if (numNodes <= 0 || maxCells <= 0 || maxRuns <= 0 || fused_grid(&grid)) return 0;
-  return fused_layout(grid, numNodes, maxCells, maxRuns).total;
+  // enough for either cell source (the work buffer does not know which runs)
+  int grid = 0, grid_rec = 0;
+  if (numNodes <= 0 || maxCells <= 0 || maxRuns <= 0 || fused_grid(&grid, false) ||
+      fused_grid(&grid_rec, true))
+    return 0;
+  return fused_layout(std::max(grid, grid_rec), numNodes, maxCells, maxRuns).total;
 }
 
 extern "C" int changaFusedGravity(const ChangaFusedLaunch* f, void* stream) {
@@ -203,8 +217,12 @@ extern "C" int changaFusedGravity(const ChangaFusedLaunch* f, void* stream) {
   if (f->numNodes <= 0 || f->numNodes > (1 << kCoopNodeBits))
     return fail("changaFusedGravity: numNodes in 1..2^23", hipErrorInvalidValue, __FILE__,
                 __LINE__);
+  if (reinterpret_cast<uintptr_t>(pc.cellRecords) % CHANGA_CELL_RECORD_BYTES != 0)
+    return fail("changaFusedGravity: cellRecords not 128-byte aligned", hipErrorInvalidValue,
+                __FILE__, __LINE__);
+  const bool rec = pc.cellRecords != nullptr;
   int grid = 0;
-  if (int e = fused_grid(&grid)) return e;
+  if (int e = fused_grid(&grid, rec)) return e;
   if ((long long)grid * f->maxCells > 0x7fffffffll || (long long)grid * f->maxRuns > 0x7fffffffll)
     return fail("changaFusedGravity: scratch capacities exceed the int32 index range",
                 hipErrorInvalidValue, __FILE__, __LINE__);
@@ -238,7 +256,7 @@ extern "C" int changaFusedGravity(const ChangaFusedLaunch* f, void* stream) {
   CGPU_CHECK(hipGetLastError());
   CGPU_CHECK(walk_records(f->numNodes, f->node64, f->first, f->count, f->child, nullptr,
                           f->theta, const_cast<WalkRec*>(a.rec), s));
-  hipLaunchKernelGGL((fused_kernel<kFusedPcTile, kFusedPpTile>), dim3(grid), dim3(kWave), 0, s, a);
+  hipLaunchKernelGGL(fused_fn(rec), dim3(grid), dim3(kWave), 0, s, a);
   CGPU_CHECK(hipGetLastError());
   if (f->errorFlag) {
     CGPU_CHECK(hipMemcpyAsync(f->errorFlag, a.err, sizeof(int), hipMemcpyDeviceToHost, s));
