@@ -5,7 +5,8 @@ TreePiece::EwaldGPU do: upload the local tree, send the cell and particle
 interaction lists as several requests with host-side lists and per-request
 bucket markers, run EwaldHost, copy the accelerations back, and check that
 each request's callback fires exactly once. The result must equal the
-device-resident path bit for bit (same kernels, same per-bucket lists)."""
+device-resident path on the wire moments bit for bit (same kernels, same
+per-bucket lists)."""
 import ctypes
 
 import numpy as np
@@ -119,7 +120,8 @@ def test_gpu_mirror_matches_device_path(device, case):
         w = configs.plummer(n=3000, eps=0.5)
     else:
         w = dict(configs.zeldovich(ng=16, rms=0.03), n_replicas=2 if case == "periodic_r2" else 1)
-    ev = engine.Evaluation(w, precise=False, pp_compact=False, device=device)
+    ev = engine.Evaluation(w, precise=False, pp_compact=False, device=device,
+                           cell_records=False)
     assert case != "plummer_soft" or ev.counts[1] > 0
     out, fired, expect = _run_mirror(ev, pp_expanded=True)
     assert sorted(fired) == sorted(expect) and len(fired) == len(set(fired))
@@ -135,7 +137,8 @@ def test_gpu_mirror_matches_device_path(device, case):
 def test_gpu_mirror_compact_part_lists(device):
     import torch
     w = configs.plummer(n=3000)
-    ev = engine.Evaluation(w, precise=False, pp_compact=True, device=device)
+    ev = engine.Evaluation(w, precise=False, pp_compact=True, device=device,
+                           cell_records=False)
     out, fired, expect = _run_mirror(ev, pp_expanded=False)
     assert sorted(fired) == sorted(expect)
     ev.zero()
