@@ -94,12 +94,14 @@ __device__ __forceinline__ void batch_bucket(const BatchArgs& a, int b, int lane
 // Three wavefronts per SIMD: up to 168 VGPRs for the three-targets-per-lane
 // loops (measured faster on the 2^22 halos than one target per lane at four
 // wavefronts). With KMAX 1 the kernel holds one chunk loop instead of three
-// and needs 139-159 VGPRs, without spills; on a sparse rung (one to three
+// and needs 139-160 VGPRs, without spills; on a sparse rung (one to three
 // targets in most buckets) it measured 16% faster than KMAX 3, on the dense
-// halos 7% slower.
+// halos 7% slower. With KMAX 1 and cell records it is built for four
+// wavefronts (128 VGPRs: the 32-cell tile spills 2, the 64-cell tile 23),
+// which the sparse rung prefers.
 template <bool PRECISE, int KIND, int PCT, int PPT, int KMAX, bool REC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KMAX == 1 && REC ? 4 : 3))) void batch_kernel(
-    BatchArgs a) {
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(KMAX == 1 && REC ? 4 : 3))) void batch_kernel(BatchArgs a) {
   __shared__ float4 lds[BatchLds<PCT, PPT>::f4];
   const int lane = threadIdx.x;
   const int* qbeg = a.queue;
