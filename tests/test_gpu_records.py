@@ -8,9 +8,10 @@ moments, and the loops evaluate the same staged values in the same order, so:
   - the records equal, bit for bit, a numpy restatement of that staging;
   - forces with records equal forces from the wire moments bit for bit for
     every tile variant of the batched kernel, in the precision and the plain
-    form, with softened cells and with periodic replicas; the per-bucket
+    form, with softened cells, with one and two periodic replica shells and
+    on a 10% active rung; the per-bucket kernel and the fused walk-and-force
     kernel (tiles of 32 cells with records, 64 without) to one FP32 rounding
-    of its per-tile sums, dtGrav bit for bit.
+    of their per-tile sums, dtGrav bit for bit.
 """
 import numpy as np
 import pytest
@@ -55,6 +56,15 @@ def _batched(ev):
     return ev.d_vars.cpu().numpy().copy()
 
 
+def _fused(ev):
+    import torch
+    ev.zero()
+    ev.run_fused()
+    torch.cuda.synchronize()
+    ev.fused_check()
+    return ev.d_vars.cpu().numpy().copy()
+
+
 def test_cell_records_content(device):
     ev = engine.Evaluation(configs.plummer(n=8192), precise=True, device=device)
     import torch
@@ -70,13 +80,15 @@ CASES = {
     "plummer": lambda: configs.plummer(n=32768),
     "softened": lambda: configs.plummer(n=4096, eps=0.5),
     "box": lambda: configs.zeldovich(ng=20, rms=0.05, cfg=2),
+    "box_r2": lambda: dict(configs.zeldovich(ng=16, rms=0.05, cfg=2), n_replicas=2),
+    "rung": lambda: configs.zeldovich(ng=24, rms=0.15, cfg=5, active_frac=0.1),
     "halos": lambda: configs.halos(n=65536),
 }
 
 
 @pytest.mark.parametrize("case,precise", [("plummer", True), ("plummer", False),
                                           ("softened", True), ("box", True), ("box", False),
-                                          ("halos", True)])
+                                          ("box_r2", True), ("rung", True), ("halos", True)])
 def test_records_equal_wire_path(device, case, precise):
     w = CASES[case]()
     ev_r = engine.Evaluation(w, precise=precise, device=device, cell_records=True)
@@ -100,6 +112,13 @@ def test_records_equal_wire_path(device, case, precise):
             np.testing.assert_array_equal(_batched(ev_r), _batched(ev_w))
     ev_r.free_batch()
     ev_w.free_batch()
+    if precise:
+        # the fused walk-and-force launch: record tiles of 32 cells, wire
+        # tiles of 64 (the per-bucket bar above)
+        fr, fw = _fused(ev_r), _fused(ev_w)
+        scale = np.abs(fw[:, :4]).max(axis=0)
+        assert (np.abs(fr[:, :4].astype(np.float64) - fw[:, :4]) / scale).max() <= 1e-6
+        np.testing.assert_array_equal(fr[:, 4], fw[:, 4])
 
 
 def test_plan_tunes_the_cell_source(device):
