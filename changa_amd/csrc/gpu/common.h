@@ -12,6 +12,15 @@ namespace cgpu {
 constexpr int kWave = 64;
 constexpr int kNumXcd = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin
 
+// Float offsets of a wire cell record (CudaMultipoleMoments): radius, soft,
+// mass and cm at 0-5, the 21 moments (xx .. xyyz) from kWireMom on;
+// kWireStride floats per node. 27 and 6 in ChaNGa's default layout, 40 and
+// 19 with the local-tree-walk fields between cm and the moments.
+constexpr int kWireStride = (int)(sizeof(CudaMultipoleMoments) / sizeof(float));
+constexpr int kWireMom = (int)(offsetof(CudaMultipoleMoments, xx) / sizeof(float));
+// float k of the 27 (0-5 header, 6-26 moments) read from a wire record
+__host__ __device__ constexpr int wire_float(int k) { return k < 6 ? k : k - 6 + kWireMom; }
+
 // Record an error (and abort unless the caller switched that off). Returns
 // the non-zero code so launchers can `return fail(...)`.
 int fail(const char* what, hipError_t e, const char* file, int line);

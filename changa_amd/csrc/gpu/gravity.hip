@@ -85,10 +85,10 @@ __global__ void cell_records_kernel(const CudaMultipoleMoments* mom, const doubl
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float* w = reinterpret_cast<const float*>(mom) + (size_t)i * 27;
+  const float* w = reinterpret_cast<const float*>(mom) + (size_t)i * kWireStride;
   float v[27];
 #pragma unroll
-  for (int k = 0; k < 27; ++k) v[k] = w[k];
+  for (int k = 0; k < 27; ++k) v[k] = w[wire_float(k)];
   double c[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) c[k] = cm64 ? cm64[3 * (size_t)i + k] : (double)v[3 + k];

@@ -267,9 +267,9 @@ __device__ __forceinline__ void load_cell_all(const ChangaGravityLaunch& a, cons
   const ILCell c = load_entry<STREAM>(cells + e);
   w.idx = c.index;
   w.oid = c.offsetID;
-  const float* src = reinterpret_cast<const float*>(a.moments) + (size_t)c.index * 27;
+  const float* src = reinterpret_cast<const float*>(a.moments) + (size_t)c.index * kWireStride;
 #pragma unroll
-  for (int k = 0; k < 27; ++k) w.v[k] = src[k];
+  for (int k = 0; k < 27; ++k) w.v[k] = src[wire_float(k)];
 }
 
 template <bool PRECISE, bool STREAM = true>
@@ -279,9 +279,9 @@ __device__ __forceinline__ void load_cell(const ChangaGravityLaunch& a, const IL
     const ILCell c = load_entry<STREAM>(cells + e);
     w.idx = c.index;
     w.oid = c.offsetID;
-    const float* src = reinterpret_cast<const float*>(a.moments) + (size_t)c.index * 27;
+    const float* src = reinterpret_cast<const float*>(a.moments) + (size_t)c.index * kWireStride;
 #pragma unroll
-    for (int k = 0; k < 27; ++k) w.v[k] = src[k];
+    for (int k = 0; k < 27; ++k) w.v[k] = src[wire_float(k)];
   }
 }
 

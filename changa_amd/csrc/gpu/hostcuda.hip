@@ -23,6 +23,17 @@ int launch_ewald(const CompactPartData* parts, VariablePartData* vars, const int
                  hipStream_t s);
 int launch_ewald_root(const double* node64, const EwaldReadOnlyData* ro, double hcut,
                       double* root, EwtData* ewt, hipStream_t s);
+#ifdef GPU_LOCAL_TREE_WALK
+int ltw_work_bytes(int maxCells, int maxRuns, size_t* bytes);
+int ltw_launch(const CudaMultipoleMoments* nodes, int numNodes, const CompactPartData* parts,
+               int numParts, VariablePartData* vars, int first, int last, int root, float theta,
+               float thetaMono, int nReplicas, float fperiod, float fperiodY, float fperiodZ,
+               void* work, size_t workBytes, int maxCells, int maxRuns, int* errOut,
+               hipStream_t s);
+// per-wavefront scratch of a local-tree-walk request: cells and particle runs
+constexpr int kLtwMaxCells = 16384;
+constexpr int kLtwMaxRuns = 8192;
+#endif
 
 static std::atomic<int> g_abort{1};
 static std::atomic<changa_gpu_callback_fn> g_handler{nullptr};
@@ -44,11 +55,12 @@ int fail(const char* what, hipError_t e, const char* file, int line) {
   return g_err_code;
 }
 
-enum Slot { S_LIST = 0, S_MARKS, S_STARTS, S_SIZES, S_MISSED, S_EWT, S_EWMARK, S_NSLOT };
+enum Slot { S_LIST = 0, S_MARKS, S_STARTS, S_SIZES, S_MISSED, S_EWT, S_EWMARK, S_LTW, S_NSLOT };
 
 struct Pool {
   void* buf[S_NSLOT] = {};
   size_t cap[S_NSLOT] = {};
+  int* flag = nullptr;  // pinned: the error flag of the stream's last local tree walk
 };
 
 static std::mutex g_pool_mu;
@@ -73,6 +85,17 @@ static int pool_get(hipStream_t s, int slot, size_t bytes, void** out) {
   *out = p.buf[slot];
   return 0;
 }
+
+#ifdef GPU_LOCAL_TREE_WALK
+static int pool_flag(hipStream_t s, int** out) {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  if (!g_pools) g_pools = new std::unordered_map<hipStream_t, Pool>();
+  Pool& p = (*g_pools)[s];
+  if (!p.flag) CGPU_CHECK(hipHostMalloc((void**)&p.flag, sizeof(int), hipHostMallocDefault));
+  *out = p.flag;
+  return 0;
+}
+#endif
 
 static void host_trampoline(void* cb) {
   changa_gpu_callback_fn fn = g_handler.load();
@@ -145,6 +168,7 @@ void changaGpuReleasePools(void) {
   for (auto& kv : *g_pools) {
     for (int k = 0; k < S_NSLOT; ++k)
       if (kv.second.buf[k]) (void)hipFree(kv.second.buf[k]);
+    if (kv.second.flag) (void)hipHostFree(kv.second.flag);
   }
   g_pools->clear();
 }
@@ -229,9 +253,50 @@ void TransferParticleVarsBack(VariablePartData* hostBuffer, size_t size, void* d
   (void)vars_back(hostBuffer, size, d_varParts, stream, cb);
 }
 
+#ifdef GPU_LOCAL_TREE_WALK
+// runs in stream order after the walk's flag has been copied to host memory
+static void ltw_check(void* p) {
+  const int f = *static_cast<const int*>(p);
+  if (f & 1)
+    fail("TreePieceCellListDataTransferLocal: a local tree walk exceeded its scratch",
+         hipErrorInvalidValue, __FILE__, __LINE__);
+  if (f & 2)
+    fail("TreePieceCellListDataTransferLocal: a node or particle index outside its array",
+         hipErrorInvalidValue, __FILE__, __LINE__);
+}
+
+// The local tree walk of particles [firstParticle, lastParticle]
+// (HostCUDA.cu:190-224): the request's interaction list is a placeholder
+// (Compute.cpp:1437-1530) and is not read.
+static int local_tree_walk(CudaRequest* d) {
+  if (!d) return fail("request: NULL", hipErrorInvalidValue, __FILE__, __LINE__);
+  hipStream_t s = (hipStream_t)d->stream;
+  const long long nn = (long long)(d->sMoments / sizeof(CudaMultipoleMoments));
+  const long long np = (long long)(d->sCompactParts / sizeof(CompactPartData));
+  if (nn > 0x7fffffffll || np > 0x7fffffffll)
+    return fail("TreePieceCellListDataTransferLocal: sMoments / sCompactParts too large",
+                hipErrorInvalidValue, __FILE__, __LINE__);
+  size_t bytes = 0;
+  if (int e = ltw_work_bytes(kLtwMaxCells, kLtwMaxRuns, &bytes)) return e;
+  void* work = nullptr;
+  int* flag = nullptr;
+  if (int e = pool_get(s, S_LTW, bytes, &work)) return e;
+  if (int e = pool_flag(s, &flag)) return e;
+  if (int e = ltw_launch(d->d_localMoments, (int)nn, d->d_localParts, (int)np, d->d_localVars,
+                         d->firstParticle, d->lastParticle, d->rootIdx, (float)d->theta,
+                         (float)d->thetaMono, d->nReplicas, d->fperiod, d->fperiodY,
+                         d->fperiodZ, work, bytes, kLtwMaxCells, kLtwMaxRuns, flag, s))
+    return e;
+  CGPU_CHECK(hipLaunchHostFunc(s, ltw_check, flag));
+  return add_callback(s, d->cb);
+}
+
+void TreePieceCellListDataTransferLocal(CudaRequest* d) { (void)local_tree_walk(d); }
+#else
 void TreePieceCellListDataTransferLocal(CudaRequest* d) {
   (void)list_request(d, true, sizeof(ILCell), 0, d ? d->d_localMoments : nullptr, nullptr);
 }
+#endif
 
 void TreePieceCellListDataTransferRemote(CudaRequest* d) {
   (void)list_request(d, true, sizeof(ILCell), 0, d ? d->d_remoteMoments : nullptr, nullptr);

@@ -61,6 +61,14 @@ class CudaRequest(ctypes.Structure):
                 ("node", ctypes.c_bool), ("remote", ctypes.c_bool)]
 
 
+class CudaRequestLTW(ctypes.Structure):
+    """CudaRequest of the GPU_LOCAL_TREE_WALK layout (the tail of HostCUDA.h:79-88)."""
+    _fields_ = CudaRequest._fields_ + [
+        ("firstParticle", c_int), ("lastParticle", c_int), ("rootIdx", c_int),
+        ("theta", ctypes.c_double), ("thetaMono", ctypes.c_double), ("nReplicas", c_int),
+        ("fperiodY", c_float), ("fperiodZ", c_float)]
+
+
 class ChangaGravityLaunch(ctypes.Structure):
     _fields_ = [("targets", c_void_p), ("vars", c_void_p), ("moments", c_void_p),
                 ("sources", c_void_p), ("list", c_void_p), ("marks", c_void_p),
@@ -136,6 +144,21 @@ EXPORTS = [
     "changaBuildTreeWorkBytes", "changaBuildTree", "changaFusedGravityWorkBytes",
     "changaFusedGravity", "changaBatchedGravityNoAlloc", "changaEwaldRoot", "changaEwaldResident",
     "changaCellRecords",
+]
+
+# the symbols of libchanga_gpu_ltw.so: the HostCUDA / EwaldCUDA boundary and
+# the per-list kernels it calls, on the GPU_LOCAL_TREE_WALK wire layouts
+LTW_EXPORTS = [
+    "allocatePinnedHostMemory", "freePinnedHostMemory", "DataManagerTransferLocalTree",
+    "DataManagerTransferRemoteChunk", "TransferParticleVarsBack",
+    "TreePieceCellListDataTransferLocal", "TreePieceCellListDataTransferRemote",
+    "TreePieceCellListDataTransferRemoteResume", "TreePiecePartListDataTransferLocal",
+    "TreePiecePartListDataTransferLocalSmallPhase", "TreePiecePartListDataTransferRemote",
+    "TreePiecePartListDataTransferRemoteResume", "TreePiecePartListDataTransferLocalCompact",
+    "EwaldHostMemorySetup", "EwaldHostMemoryFree", "EwaldHost", "changaGpuSetCallbackHandler",
+    "changaGpuSetAbortOnError", "changaGpuLastError", "changaGpuReleasePools",
+    "changaNodeGravity", "changaPartGravity", "changaZeroVars", "changaAddVars", "changaEwald",
+    "changaEwaldRoot", "changaEwaldResident", "changaCellRecords",
 ]
 
 # doubles of the device Ewald root record of changaEwaldRoot
@@ -231,6 +254,46 @@ def lib():
     L.changaGpuSetAbortOnError(0)
     _lib = L
     return L
+
+
+_ltw = None
+
+
+def lib_ltw():
+    """libchanga_gpu_ltw.so: the boundary built on the GPU_LOCAL_TREE_WALK
+    layouts (160-byte nodes, 24-byte particles, CudaRequestLTW), whose
+    TreePieceCellListDataTransferLocal runs the local tree walk. Its errors
+    are reported through its own changaGpuLastError (check_ltw)."""
+    global _ltw
+    if _ltw is not None:
+        return _ltw
+    if not os.path.exists(_build.LTW_LIB):
+        raise RuntimeError("libchanga_gpu_ltw.so is not built: run `python -m changa_amd.build`")
+    L = ctypes.CDLL(_build.LTW_LIB)
+    P = c_void_p
+    for name in ["TreePieceCellListDataTransferLocal", "TreePieceCellListDataTransferRemote",
+                 "TreePieceCellListDataTransferRemoteResume", "TreePiecePartListDataTransferLocal",
+                 "TreePiecePartListDataTransferRemote", "TreePiecePartListDataTransferRemoteResume",
+                 "TreePiecePartListDataTransferLocalCompact"]:
+        getattr(L, name).argtypes = [ctypes.POINTER(CudaRequestLTW)]
+    L.TransferParticleVarsBack.argtypes = [P, c_size_t, P, P, P]
+    L.changaGpuSetAbortOnError.argtypes = [c_int]
+    L.changaGpuLastError.argtypes = [ctypes.POINTER(ctypes.c_char_p)]
+    L.changaGpuLastError.restype = c_int
+    L.changaGpuReleasePools.argtypes = []
+    L.changaZeroVars.argtypes = [P, c_int, P]
+    L.changaZeroVars.restype = c_int
+    L.changaGpuSetAbortOnError(0)
+    _ltw = L
+    return L
+
+
+def check_ltw(what):
+    """Raise when the LTW library recorded an error (its calls return void)."""
+    msg = ctypes.c_char_p()
+    rc = lib_ltw().changaGpuLastError(ctypes.byref(msg))
+    if rc != 0:
+        raise RuntimeError(f"{what}: {msg.value.decode() if msg.value else rc}")
 
 
 def check(rc, what):

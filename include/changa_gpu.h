@@ -65,7 +65,17 @@ void TransferParticleVarsBack(VariablePartData *hostBuffer, size_t size, void *d
    bucketMarkers[numBucketsPlusOne], bucketStarts/bucketSizes[numBuckets]
    index d_localParts; results are added into d_localVars (+= for a, phi;
    max for dtGrav). Sources: d_localMoments, d_remoteMoments, or
-   data->missedNodes (CudaMultipoleMoments, data->sMissed bytes). */
+   data->missedNodes (CudaMultipoleMoments, data->sMissed bytes).
+   In the GPU_LOCAL_TREE_WALK build (libchanga_gpu_ltw.so, this header with
+   that macro defined) TreePieceCellListDataTransferLocal is the local tree
+   walk of HostCUDA.cu:210-224 instead: the list fields are not read; every
+   particle in [firstParticle, lastParticle] of d_localParts walks the
+   serialized local tree d_localMoments (sMoments bytes) from rootIdx, per
+   periodic replica, with theta / thetaMono, and gets the PC, softened-cell
+   and PP forces of what the walk accepts (+= / max as above). Periodic
+   requests need fperiodY == fperiodZ == fperiod; a walk beyond the
+   per-wavefront scratch (16,384 cells, 8,192 particle runs) or a child or
+   bucket index outside its array is reported as an error. */
 void TreePieceCellListDataTransferLocal(CudaRequest *data);
 void TreePieceCellListDataTransferRemote(CudaRequest *data);
 void TreePieceCellListDataTransferRemoteResume(CudaRequest *data);
@@ -381,9 +391,11 @@ int changaBuildTree(ChangaTreeBuild *t, void *stream);
 /* ======================================================================
  * Part 7: fused walk and force
  *
- * Replaces gpuLocalTreeWalk (CudaFunctions.h:10-23, HostCUDA.cu:724-942),
- * which the reference launches from TreePieceCellListDataTransferLocal
- * (HostCUDA.cu:210-224) with the CudaRequest fields of HostCUDA.h:80-89.
+ * The device-resident counterpart of gpuLocalTreeWalk (CudaFunctions.h:10-23,
+ * HostCUDA.cu:724-942), which the reference launches from
+ * TreePieceCellListDataTransferLocal (HostCUDA.cu:210-224) with the
+ * CudaRequest fields of HostCUDA.h:80-89; that request route itself is the
+ * GPU_LOCAL_TREE_WALK build of Part 1.
  * One persistent launch in which each wavefront walks the tree for a target
  * bucket (the opening tests of changaBuildLists*, so the same interaction
  * set; the reference kernel walks per particle instead) and evaluates what

@@ -291,3 +291,117 @@ def rinv_derivs(r, nmax=6):
     out = np.zeros(343)
     lib().orc_rinv_derivs(_ptr(np.ascontiguousarray(r, dtype=np.float64)), nmax, _ptr(out))
     return out.reshape(7, 7, 7)
+
+
+# ---------------------------------------------------------------------------
+# The local tree walk of ChaNGa's GPU_LOCAL_TREE_WALK build (numpy)
+# ---------------------------------------------------------------------------
+# CudaNodeType (cuda_typedef.h:87-99)
+_LTW_BUCKET, _LTW_INTERNAL, _LTW_BOUNDARY, _LTW_NONLOCAL_BUCKET = 2, 3, 4, 8
+
+
+def ltw_lists(radius, soft, cm, lo, hi, count, ntype, children, bstart, bsize, bucket_nodes,
+              theta, theta_mono, n_replicas=0, period=0.0, root=0, dtype=np.float32):
+    """Interaction lists of the reference's gpuLocalTreeWalk (HostCUDA.cu:
+    724-942) for each bucket node of `bucket_nodes`: the walk from `root`
+    per periodic replica, CUDA_openCriterionNode (CUDAMoments.cu:197-242)
+    against the bucket's box, CUDA_openSoftening (:180-195) and the node-type
+    actions of CUDA_OptAction (:323-352). The test depends on the particle
+    only through its bucket node, so one list per bucket serves all its
+    particles.
+
+    dtype float32: the reference's arithmetic (cudatype operands, the opening
+    radii formed in double from 2/sqrt(3) and rounded to float); numpy does
+    each float32 operation with one rounding and fuses none. dtype float64:
+    the same walk in double, which on the same geometry must give the host
+    list builder's lists (openCriterionBucket, gravity.h:597-635) -- the
+    check that pins this walk's traversal, actions and replica handling.
+
+    Node arrays are per node (cm, lo, hi: [n, 3]; children: [n, 2], -1 none).
+    Returns the lists in the layout of forces_lists (flat int32): cells
+    (node, offsetID with the replica code in bits 22-30 and bit 31 for a
+    softened cell), parts (bucketStart, offsetID, bucketSize), int64 marks
+    per bucket; within a bucket the entries are sorted."""
+    f = np.dtype(dtype).type
+    kgeom = 2.0 / np.sqrt(3.0)
+    rad = np.asarray(radius, dtype=f)
+    sft = np.asarray(soft, dtype=f)
+    cm = np.asarray(cm, dtype=f)
+    count = np.asarray(count)
+    ntype = np.asarray(ntype)
+    children = np.asarray(children).reshape(-1, 2)
+    bstart = np.asarray(bstart)
+    bsize = np.asarray(bsize)
+    bucket_nodes = np.asarray(bucket_nodes, dtype=np.int64)
+    nb = len(bucket_nodes)
+    bc = cm[bucket_nodes]
+    brb = f(2.0) * sft[bucket_nodes]
+    blo = np.asarray(lo, dtype=f)[bucket_nodes]
+    bhi = np.asarray(hi, dtype=f)[bucket_nodes]
+    th, thm = f(theta), f(theta_mono)
+    R = int(n_replicas)
+    W = 2 * R + 1
+    k = np.arange(W ** 3)
+    rx, ry, rz = k // (W * W) - R, (k // W) % W - R, k % W - R
+    codes = ((rx + 3) | ((ry + 3) << 3) | ((rz + 3) << 6)).astype(np.int64)
+    # frontier of (bucket, node, replica code)
+    fb = np.repeat(np.arange(nb), len(codes))
+    fn = np.full(len(fb), root, dtype=np.int64)
+    fc = np.tile(codes, nb)
+    per = f(period)
+
+    def box_sphere(b, c, r):
+        dsq = np.zeros(len(b), dtype=f)
+        for a in range(3):
+            d1 = blo[b, a] - c[:, a]
+            d2 = c[:, a] - bhi[b, a]
+            e = np.where(d1 > 0, d1, np.where(d2 > 0, d2, f(0)))
+            dsq = dsq + e * e
+        return dsq <= r * r
+
+    def soft_open(b, c, rs):
+        d = bc[b] - c
+        dist = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+        return (dist <= (brb[b] + rs) * (brb[b] + rs)) | box_sphere(b, c, rs)
+
+    out_c, out_p = [], []
+    while len(fb):
+        off = np.stack([((fc & 7) - 3), (((fc >> 3) & 7) - 3), (((fc >> 6) & 7) - 3)], axis=1)
+        c = cm[fn] + off.astype(f) * per
+        r = rad[fn]
+        rs = f(2.0) * sft[fn]
+        ro = (kgeom * r.astype(np.float64) / np.float64(th)).astype(f)
+        ro = np.where(ro < r, r, ro)
+        rm = (kgeom * r.astype(np.float64) / np.float64(thm)).astype(f)
+        so = soft_open(fb, c, rs)
+        opened = (count[fn] <= 6) | box_sphere(fb, c, ro) | (so & box_sphere(fb, c, rm))
+        t = ntype[fn]
+        compute = ~opened & ((t == _LTW_INTERNAL) | (t == _LTW_BUCKET) | (t == _LTW_BOUNDARY) |
+                             (t == _LTW_NONLOCAL_BUCKET))
+        oid = (fc << 22).astype(np.int64)
+        sel = np.nonzero(compute)[0]
+        out_c.append(np.stack([fb[sel], fn[sel],
+                               oid[sel] | np.where(so[sel], 1 << 31, 0)], axis=1))
+        sel = np.nonzero(opened & (t == _LTW_BUCKET) & (bsize[fn] > 0))[0]
+        out_p.append(np.stack([fb[sel], bstart[fn[sel]], oid[sel], bsize[fn[sel]]], axis=1))
+        push = opened & ((t == _LTW_INTERNAL) | (t == _LTW_BOUNDARY))
+        nb_, nn_, nc_ = [], [], []
+        for j in (0, 1):
+            ch = children[fn, j]
+            s = np.nonzero(push & (ch >= 0))[0]
+            nb_.append(fb[s])
+            nn_.append(ch[s])
+            nc_.append(fc[s])
+        fb, fn, fc = np.concatenate(nb_), np.concatenate(nn_).astype(np.int64), np.concatenate(nc_)
+
+    def pack(chunks, width):
+        a = np.concatenate(chunks) if chunks else np.zeros((0, width + 1), dtype=np.int64)
+        a = a[np.lexsort(a[:, ::-1].T)]  # by bucket, then the entry
+        marks = np.zeros(nb + 1, dtype=np.int64)
+        np.add.at(marks, a[:, 0] + 1, 1)
+        entries = a[:, 1:].astype(np.uint32).view(np.int32).reshape(-1)
+        return np.ascontiguousarray(entries), np.cumsum(marks)
+
+    cells, cmarks = pack(out_c, 2)
+    parts, pmarks = pack(out_p, 3)
+    return dict(cells=cells, cell_marks=cmarks, parts=parts, part_marks=pmarks)

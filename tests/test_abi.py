@@ -87,3 +87,53 @@ def test_ctypes_mirrors_match_the_header(tmp_path):
         assert int(out[name]) == ctypes.sizeof(cls), name
         for f in cls._fields_:
             assert int(out[f"{name}.{f[0]}"]) == getattr(cls, f[0]).offset, f"{name}.{f[0]}"
+
+
+def test_ltw_layouts_match_the_header(tmp_path):
+    """The GPU_LOCAL_TREE_WALK layouts: the header compiled with the define
+    gives the node, particle and request records whose offsets the numpy
+    records of changa_amd.ltw and the ctypes CudaRequestLTW use."""
+    import subprocess
+    from changa_amd import ltw
+    node_fields = ["radius", "soft", "totalMass", "cm", "lesser_corner", "greater_corner",
+                   "bucketStart", "bucketSize", "particleCount", "nodeArrayIndex", "children",
+                   "type"]
+    lines = ['  printf("node %zu\\n", sizeof(CudaMultipoleMoments));',
+             '  printf("node.moments %zu\\n", offsetof(CudaMultipoleMoments, xx));',
+             '  printf("part %zu\\n", sizeof(CompactPartData));',
+             '  printf("part.nodeId %zu\\n", offsetof(CompactPartData, nodeId));',
+             '  printf("req %zu\\n", sizeof(CudaRequest));']
+    lines += [f'  printf("node.{f} %zu\\n", offsetof(CudaMultipoleMoments, {f}));'
+              for f in node_fields]
+    lines += [f'  printf("req.{f[0]} %zu\\n", offsetof(CudaRequest, {f[0]}));'
+              for f in gpu.CudaRequestLTW._fields_]
+    src = tmp_path / "ltw_layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "changa_gpu.h"\n'
+                   "int main(void) {\n" + "\n".join(lines) + "\n  return 0;\n}\n")
+    exe = tmp_path / "ltw_layout"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-DGPU_LOCAL_TREE_WALK",
+                    "-I" + os.path.dirname(HDR), str(src), "-o", str(exe)], check=True)
+    out = dict(line.rsplit(" ", 1) for line in
+               subprocess.run([str(exe)], check=True, capture_output=True,
+                              text=True).stdout.splitlines())
+    out = {k: int(v) for k, v in out.items()}
+    assert out["node"] == ltw.NODE_DTYPE.itemsize == 160
+    assert out["node.moments"] == ltw.NODE_DTYPE.fields["moments"][1] == 76
+    for f in node_fields:
+        assert out[f"node.{f}"] == ltw.NODE_DTYPE.fields[f][1], f
+    assert out["part"] == ltw.PART_DTYPE.itemsize == 24
+    assert out["part.nodeId"] == ltw.PART_DTYPE.fields["nodeId"][1] == 20
+    assert out["req"] == ctypes.sizeof(gpu.CudaRequestLTW)
+    for f in gpu.CudaRequestLTW._fields_:
+        assert out[f"req.{f[0]}"] == getattr(gpu.CudaRequestLTW, f[0]).offset, f[0]
+
+
+def test_ltw_library_exports_the_boundary():
+    """libchanga_gpu_ltw.so (the boundary on the GPU_LOCAL_TREE_WALK layouts)
+    loads without a GPU and exports the HostCUDA / EwaldCUDA entry points."""
+    assert os.path.exists(build.LTW_LIB), "build first: python -m changa_amd.build"
+    lib = ctypes.CDLL(build.LTW_LIB)
+    for n in gpu.LTW_EXPORTS:
+        assert hasattr(lib, n), n
+    assert set(gpu.LTW_EXPORTS) <= set(declared())
+    gpu.lib_ltw()

@@ -1,0 +1,159 @@
+"""The GPU_LOCAL_TREE_WALK request route: TreePieceCellListDataTransferLocal
+of libchanga_gpu_ltw.so on serialized local trees (changa_amd.ltw).
+
+  - against the FP64 oracle (nodeBucketForce / partBucketForce) on the lists
+    of the oracle's restatement of the reference walk, computed from the
+    same FP32 node records: per-particle tails at about 1.5x the measured
+    ones (plain form: FP32 absolute coordinates);
+  - on these workloads the walk's lists are the host list builder's, and the
+    forces agree with the list route of libchanga_gpu.so (per-bucket plain
+    kernels on the host lists) to the FP32 order of the sums, dtGrav bit for
+    bit;
+  - node types other than bucket / internal act as in CUDA_OptAction (the
+    oracle walk on the same records);
+  - invalid records and rectangular periods are reported as errors.
+"""
+import numpy as np
+import pytest
+
+from changa_amd import configs, engine, ltw
+from oracle import oracle as O
+from tests.helpers import check_tails, rel_err, summary
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "plummer": lambda: (configs.plummer(n=32768), 0),
+    "softened": lambda: (configs.plummer(n=4096, eps=0.5), 0),
+    "box_r1": lambda: (configs.zeldovich(ng=16, rms=0.05), 1),
+    "box_r2": lambda: (configs.zeldovich(ng=12, rms=0.05), 2),
+    "halos": lambda: (configs.halos(n=32768), 0),
+}
+
+# (median, p99, max) of acc, pot, dtGrav against the oracle, at about 1.5x
+# the measured tails (the walk and the sums are deterministic). Measured:
+# plummer acc 6.4e-8 / 1.2e-6 / 2.8e-5; softened 2.5e-7 / 4.7e-7 / 5.6e-7;
+# box_r1 1.6e-7 / 3.2e-6 / 1.7e-5; box_r2 2.6e-7 / 1.7e-6 / 4.6e-6; halos
+# 8.7e-7 / 3.2e-5 / 2.4e-4 (close pairs in FP32 absolute coordinates, as in
+# the reference's GPU walk); potentials <= 1.3e-6.
+BARS = {
+    "plummer": ((9.5e-8, 1.8e-6, 4.2e-5), (3.4e-8, 1.3e-7, 2.6e-7), (9.3e-7, 7.5e-6, 3.1e-5)),
+    "softened": ((3.8e-7, 7.1e-7, 8.4e-7), (8e-8, 2e-7, 2.4e-7), (8.7e-8, 2.2e-6, 3.9e-6)),
+    "box_r1": ((2.5e-7, 4.7e-6, 2.6e-5), (4e-8, 1.4e-7, 2e-7), (1.1e-6, 9.3e-6, 3.9e-5)),
+    "box_r2": ((3.8e-7, 2.6e-6, 6.9e-6), (2.8e-8, 1e-7, 1.4e-7), (9.8e-7, 5.3e-6, 1.2e-5)),
+    "halos": ((1.3e-6, 4.8e-5, 3.6e-4), (6.5e-8, 6.9e-7, 2e-6), (5e-6, 6e-5, 1.1e-4)),
+    "types": ((9.5e-8, 2.2e-6, 1.9e-5), (3.7e-8, 1.3e-7, 2.5e-7), (6e-7, 6e-6, 2.1e-5)),
+}
+
+
+def _oracle_lists(ev, nodes, R, period):
+    theta = float(ev.w["theta"])
+    return O.ltw_lists(nodes["radius"], nodes["soft"], nodes["cm"], nodes["lesser_corner"],
+                       nodes["greater_corner"], nodes["particleCount"], nodes["type"],
+                       nodes["children"], nodes["bucketStart"], nodes["bucketSize"],
+                       ev.tree["bucket_node"], theta, theta ** 4, R, period, dtype=np.float32)
+
+
+def _against_oracle(ev, nodes, parts, R, period, label):
+    v = ltw.walk(nodes, parts, float(ev.w["theta"]), R, period, device=ev.device)
+    lists = _oracle_lists(ev, nodes, R, period)
+    geo, mom = O.build_moments(ev.tree, ev.pos, ev.mass, ev.soft)
+    oa, op, od, _ = O.forces_lists(ev.tree, geo, mom, ev.pos, ev.mass, ev.soft, lists,
+                                   period=period, soft_flags=True)
+    ea = rel_err(v[:, 0:3], oa)
+    ep = rel_err(v[:, 3], op)
+    ed = rel_err(v[:, 4], od)
+    print(f"ltw {label}: acc {summary(ea)} pot {summary(ep)} dtgrav {summary(ed)}", flush=True)
+    for name, e, b in zip(("acc", "pot", "dtgrav"), (ea, ep, ed), BARS[label]):
+        check_tails(e, *b, label=f"ltw {label} {name}")
+    return v, lists
+
+
+def _entries(lists):
+    c = lists["cells"].reshape(-1, 2)
+    p = lists["parts"].reshape(-1, 3)
+    out = []
+    for b in range(len(lists["cell_marks"]) - 1):
+        cs = c[lists["cell_marks"][b]:lists["cell_marks"][b + 1]].astype(np.int64)
+        ps = p[lists["part_marks"][b]:lists["part_marks"][b + 1]].astype(np.int64)
+        out.append(({(n, (o & 0xffffffff) >> 22) for n, o in cs},
+                    {(a, (o >> 22) & 0x1ff, k) for a, o, k in ps}))
+    return out
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_ltw_matches_oracle_and_list_route(device, case):
+    import torch
+    w, R = CASES[case]()
+    period = float(w["L"]) if w["periodic"] else 0.0
+    ev = engine.Evaluation(dict(w, n_replicas=R), precise=False, device=device,
+                           cell_records=False)
+    torch.cuda.synchronize()
+    nodes, parts = ltw.serialize_evaluation(ev)
+    print(f"ltw {case}: {len(parts)} particles, {len(nodes)} nodes, R={R}", flush=True)
+    v, lists = _against_oracle(ev, nodes, parts, R, period, case)
+    # the walk's lists are the host builder's here: the list route of the
+    # main library on the host lists sums the same terms in another order
+    assert _entries(lists) == _entries(ev.lists)
+    ev.zero()
+    ev.run_pc()
+    ev.run_pp()
+    torch.cuda.synchronize()
+    ref = ev.d_vars.cpu().numpy()
+    scale = np.abs(ref[:, :4]).max(axis=0)
+    d = (np.abs(v[:, :4].astype(np.float64) - ref[:, :4]) / scale).max()
+    print(f"ltw {case}: vs list route {d:.2e} of the largest component", flush=True)
+    assert d <= 2e-6
+    np.testing.assert_array_equal(v[:, 4], ref[:, 4])
+
+
+def test_ltw_node_types(device):
+    """A local tree with a non-local subtree, a non-local bucket and a
+    boundary node: the kernel's actions against the oracle walk's."""
+    import torch
+    w = configs.plummer(n=16384)
+    ev = engine.Evaluation(w, precise=False, device=device, cell_records=False)
+    torch.cuda.synchronize()
+    nodes, parts = ltw.serialize_evaluation(ev)
+    count = nodes["particleCount"]
+    inner = np.nonzero(nodes["type"] == ltw.INTERNAL)[0]
+    x = int(inner[np.argmin(np.abs(count[inner] - len(parts) // 8))])
+    y = int(inner[np.argmin(np.abs(count[inner] - len(parts) // 5))])
+    nodes["type"][x] = ltw.NONLOCAL
+    nodes["type"][y] = ltw.BOUNDARY
+    bk = int(ev.tree["bucket_node"][len(ev.tree["bucket_node"]) // 3])
+    nodes["type"][bk] = ltw.NONLOCAL_BUCKET
+    _against_oracle(ev, nodes, parts, 0, 0.0, "types")
+
+
+def test_ltw_reports_invalid_records(device):
+    import torch
+    ev = engine.Evaluation(configs.plummer(n=4096), precise=False, device=device)
+    torch.cuda.synchronize()
+    nodes, parts = ltw.serialize_evaluation(ev)
+    bad = nodes.copy()
+    bad["children"][0, 1] = len(nodes) + 5  # the root's second child past the array
+    with pytest.raises(RuntimeError, match="index outside its array"):
+        ltw.walk(bad, parts, float(ev.w["theta"]))
+    # the library stays usable: the same request on valid records
+    v = ltw.walk(nodes, parts, float(ev.w["theta"]))
+    assert np.isfinite(v).all() and (np.abs(v[:, 0:3]).sum(axis=1) > 0).all()
+    # one period per axis only (the cell and particle loops share it)
+    import changa_amd.gpu as gpu
+    L = gpu.lib_ltw()
+    r = gpu.CudaRequestLTW()
+    r.stream = torch.cuda.current_stream().cuda_stream
+    d_nodes = torch.from_numpy(nodes.view(np.uint8).copy()).cuda()
+    d_parts = torch.from_numpy(parts.view(np.uint8).copy()).cuda()
+    d_vars = torch.zeros((len(parts), 5), dtype=torch.float32, device="cuda")
+    r.d_localMoments, r.d_localParts, r.d_localVars = (d_nodes.data_ptr(), d_parts.data_ptr(),
+                                                       d_vars.data_ptr())
+    r.sMoments, r.sCompactParts = nodes.nbytes, parts.nbytes
+    r.firstParticle, r.lastParticle, r.rootIdx = 0, len(parts) - 1, 0
+    r.theta, r.thetaMono, r.nReplicas = 0.7, 0.7 ** 4, 1
+    r.fperiod, r.fperiodY, r.fperiodZ = 1.0, 1.0, 2.0
+    import ctypes
+    L.TreePieceCellListDataTransferLocal(ctypes.byref(r))
+    torch.cuda.synchronize()
+    with pytest.raises(RuntimeError, match="one period on all three axes"):
+        gpu.check_ltw("TreePieceCellListDataTransferLocal")
