@@ -77,7 +77,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--form", choices=["precise", "plain"], default="precise")
-    ap.add_argument("--path", choices=["batched", "per-bucket", "fused"], default="batched")
+    ap.add_argument("--path", choices=["batched", "per-bucket", "fused", "ltw"], default="batched",
+                    help="ltw: the GPU_LOCAL_TREE_WALK request route (plain form)")
     ap.add_argument("--config", type=int, choices=[2, 3, 4, 5], default=3)
     ap.add_argument("--ng", type=int, default=128, help="lattice size of the periodic box")
     ap.add_argument("--n", type=int, default=1 << 22, help="particles of the halo workload")
@@ -187,8 +188,19 @@ def main():
         # walk and force in one launch: the list build is inside this step
         step_ms["fused"] = time_steps(ev.step_fused)
         ev.fused_check()
+    elif ev.active is None:
+        # the local-tree-walk request route (libchanga_gpu_ltw.so): the
+        # serialized local tree walked per bucket and evaluated, no lists
+        # (every particle is a target of that route)
+        from changa_amd import ltw
+        nodes, parts = ltw.serialize_evaluation(ev)
+        lw = ltw.LocalWalk(nodes, parts, float(w["theta"]),
+                           int(w["n_replicas"]) if w["periodic"] else 0, ev.L, device=ev.device)
+        step_ms["ltw"] = time_steps(lw.run)
+        lw.results()
     if args.path not in step_ms:
-        raise SystemExit("the fused path runs on the precision form")
+        raise SystemExit("the fused path runs on the precision form, the ltw path on the plain "
+                         "form of a workload without an active rung")
     ms = step_ms[args.path]
     value = n_int / (ms * 1e-3) / 1e9
 
@@ -209,6 +221,9 @@ def main():
             "batched_graph": time_fn(ev.run_batched)}
     if ev.precise:
         kern["fused"] = time_fn(ev.run_fused)
+    elif "ltw" in step_ms:
+        kern["ltw"] = time_fn(lw.run)
+        lw.results()
     if w["periodic"]:
         kern["ewald"] = time_fn(ev.run_ewald)
     pc_flop = FLOP["pc"] * counts["pc"] + FLOP["soft"] * counts["soft"]
@@ -218,6 +233,10 @@ def main():
         dom_flop = pc_flop + FLOP["pp"] * counts["pp"]
     elif args.path == "fused":
         dom, dom_ms = "fused_kernel", kern["fused"]
+        dom_flop = pc_flop + FLOP["pp"] * counts["pp"]
+    elif args.path == "ltw":
+        # the zeroing kernel, the counter reset and ltw_kernel
+        dom, dom_ms = "ltw_kernel", kern["ltw"] - kern["zero"]
         dom_flop = pc_flop + FLOP["pp"] * counts["pp"]
     else:
         dom, dom_ms, dom_flop = "pc_kernel", kern["pc"], pc_flop
@@ -240,9 +259,12 @@ def main():
             "workload": workload,
             "form": args.form,
             "path": args.path,
-            "step": "GPU multipole build + zero + PC + PP" + (" + Ewald" if w["periodic"] else "")
-                    + (" with the tree walk fused in (no lists)" if args.path == "fused"
-                       else " on resident lists"),
+            "step": ("zero + one local-tree-walk request (walk + PC + PP, no lists) on the "
+                     "serialized tree; the multipole build and Ewald are not in this step"
+                     if args.path == "ltw" else
+                     "GPU multipole build + zero + PC + PP" + (" + Ewald" if w["periodic"] else "")
+                     + (" with the tree walk fused in (no lists)" if args.path == "fused"
+                        else " on resident lists")),
             "interactions_per_step": counts,
             "n_buckets": int(ev.tree["n_buckets"]),
             "n_nodes": int(ev.tree["n_nodes"]),
@@ -295,8 +317,15 @@ def main():
 
     log(f"timed: {value:.1f} G interactions/s")
     if not args.no_accuracy:
-        out["accuracy"] = accuracy(ev, {"batched": ev.step_batched, "fused": ev.step_fused}.get(
-            args.path, ev.step), args.samples)
+        def step_ltw():
+            # the request's results into the evaluation's, then the Ewald pass
+            lw.run()
+            ev.d_vars.copy_(lw.d_vars)
+            if w["periodic"]:
+                ev.run_ewald()
+
+        out["accuracy"] = accuracy(ev, {"batched": ev.step_batched, "fused": ev.step_fused,
+                                        "ltw": step_ltw}.get(args.path, ev.step), args.samples)
         log("accuracy leg done")
     if args.adjoint or args.kdk or args.grad_accuracy or (args.config in (4, 5)
                                                           and not args.no_adjoint):

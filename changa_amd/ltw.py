@@ -72,37 +72,62 @@ def serialize_evaluation(ev):
     return serialize(ev.tree, ev.d_moments.cpu().numpy(), ev.node64(), ev.pos, ev.mass, ev.soft)
 
 
+class LocalWalk:
+    """A TreePieceCellListDataTransferLocal request of the local tree walk on
+    device copies of `nodes` and `parts` that stay resident, for repeated
+    runs; theta_mono defaults to theta^4 (TreePiece.cpp:5036)."""
+
+    def __init__(self, nodes, parts, theta, n_replicas=0, period=0.0, first=0, last=None,
+                 root=0, device="cuda:0", theta_mono=None):
+        import torch
+        self.L = gpu.lib_ltw()
+        self.n = len(parts)
+        dev = torch.device(device)
+        self.d_nodes = torch.from_numpy(nodes.view(np.uint8).copy()).to(dev)
+        self.d_parts = torch.from_numpy(parts.view(np.uint8).copy()).to(dev)
+        self.d_vars = torch.zeros((self.n, 5), dtype=torch.float32, device=dev)
+        self.device = dev
+        self.stream = torch.cuda.current_stream(dev)
+        r = gpu.CudaRequestLTW()
+        r.stream = self.stream.cuda_stream
+        r.d_localMoments = self.d_nodes.data_ptr()
+        r.d_localParts = self.d_parts.data_ptr()
+        r.d_localVars = self.d_vars.data_ptr()
+        r.sMoments = nodes.nbytes
+        r.sCompactParts = parts.nbytes
+        r.sVarParts = self.n * 20
+        r.numBucketsPlusOne = 1
+        r.firstParticle, r.rootIdx = first, root
+        r.lastParticle = self.n - 1 if last is None else last
+        r.theta = theta
+        r.thetaMono = theta * theta * theta * theta if theta_mono is None else theta_mono
+        r.nReplicas = n_replicas
+        r.fperiod = r.fperiodY = r.fperiodZ = period
+        r.node, r.remote = True, False
+        self.request = r
+
+    def run(self, zero=True):
+        """Enqueue one request (after zeroing the results unless zero=False)."""
+        if zero:
+            rc = self.L.changaZeroVars(ctypes.c_void_p(self.d_vars.data_ptr()), self.n,
+                                       ctypes.c_void_p(self.stream.cuda_stream))
+            if rc:
+                gpu.check_ltw("changaZeroVars")
+        self.L.TreePieceCellListDataTransferLocal(ctypes.byref(self.request))
+
+    def results(self):
+        """Synchronize, raise on a recorded error, and return the
+        VariablePartData of every particle (float32 [n, 5])."""
+        import torch
+        torch.cuda.synchronize(self.device)
+        gpu.check_ltw("TreePieceCellListDataTransferLocal")
+        return self.d_vars.cpu().numpy()
+
+
 def walk(nodes, parts, theta, n_replicas=0, period=0.0, first=0, last=None, root=0,
          device="cuda:0", theta_mono=None):
-    """One TreePieceCellListDataTransferLocal request of the local tree walk
-    on fresh device copies of `nodes` and `parts`; returns the
-    VariablePartData of every particle (a[3], potential, dtGrav; float32
-    [n, 5]). theta_mono defaults to theta^4 (TreePiece.cpp:5036)."""
-    import torch
-    L = gpu.lib_ltw()
-    n = len(parts)
-    last = n - 1 if last is None else last
-    dev = torch.device(device)
-    d_nodes = torch.from_numpy(nodes.view(np.uint8).copy()).to(dev)
-    d_parts = torch.from_numpy(parts.view(np.uint8).copy()).to(dev)
-    d_vars = torch.zeros((n, 5), dtype=torch.float32, device=dev)
-    stream = torch.cuda.current_stream(dev)
-    r = gpu.CudaRequestLTW()
-    r.stream = stream.cuda_stream
-    r.d_localMoments = d_nodes.data_ptr()
-    r.d_localParts = d_parts.data_ptr()
-    r.d_localVars = d_vars.data_ptr()
-    r.sMoments = nodes.nbytes
-    r.sCompactParts = parts.nbytes
-    r.sVarParts = d_vars.numel() * 4
-    r.numBucketsPlusOne = 1
-    r.firstParticle, r.lastParticle, r.rootIdx = first, last, root
-    r.theta = theta
-    r.thetaMono = theta * theta * theta * theta if theta_mono is None else theta_mono
-    r.nReplicas = n_replicas
-    r.fperiod = r.fperiodY = r.fperiodZ = period
-    r.node, r.remote = True, False
-    L.TreePieceCellListDataTransferLocal(ctypes.byref(r))
-    torch.cuda.synchronize(dev)
-    gpu.check_ltw("TreePieceCellListDataTransferLocal")
-    return d_vars.cpu().numpy()
+    """One request of the local tree walk on fresh device copies; returns
+    the VariablePartData of every particle (a[3], potential, dtGrav)."""
+    w = LocalWalk(nodes, parts, theta, n_replicas, period, first, last, root, device, theta_mono)
+    w.run()
+    return w.results()
