@@ -276,6 +276,14 @@ def lib_ltw():
                  "TreePiecePartListDataTransferRemote", "TreePiecePartListDataTransferRemoteResume",
                  "TreePiecePartListDataTransferLocalCompact"]:
         getattr(L, name).argtypes = [ctypes.POINTER(CudaRequestLTW)]
+    L.TreePiecePartListDataTransferLocalSmallPhase.argtypes = [ctypes.POINTER(CudaRequestLTW), P,
+                                                               c_int]
+    L.allocatePinnedHostMemory.argtypes = [ctypes.POINTER(c_void_p), c_size_t]
+    L.freePinnedHostMemory.argtypes = [P]
+    L.EwaldHostMemorySetup.argtypes = [ctypes.POINTER(EwaldData), c_int, c_int, c_int]
+    L.EwaldHostMemoryFree.argtypes = [ctypes.POINTER(EwaldData), c_int]
+    L.EwaldHost.argtypes = [P, P, ctypes.POINTER(EwaldData), P, P, c_int, c_int]
+    L.changaGpuSetCallbackHandler.argtypes = [CALLBACK]
     L.TransferParticleVarsBack.argtypes = [P, c_size_t, P, P, P]
     L.changaGpuSetAbortOnError.argtypes = [c_int]
     L.changaGpuLastError.argtypes = [ctypes.POINTER(ctypes.c_char_p)]

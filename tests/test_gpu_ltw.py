@@ -11,7 +11,10 @@ of libchanga_gpu_ltw.so on serialized local trees (changa_amd.ltw).
     bit;
   - node types other than bucket / internal act as in CUDA_OptAction (the
     oracle walk on the same records);
-  - invalid records and rectangular periods are reported as errors.
+  - invalid records and rectangular periods are reported as errors;
+  - the library's list requests (remote cells, compact particle lists,
+    EwaldHost) read the 160-byte node and 24-byte particle records and give
+    the main library's results on its own layouts bit for bit.
 """
 import numpy as np
 import pytest
@@ -157,3 +160,74 @@ def test_ltw_reports_invalid_records(device):
     torch.cuda.synchronize()
     with pytest.raises(RuntimeError, match="one period on all three axes"):
         gpu.check_ltw("TreePieceCellListDataTransferLocal")
+
+
+def _list_route(L, Req, ev, nodes, parts):
+    """TreePieceCellListDataTransferRemote (the cells from d_remoteMoments),
+    TreePiecePartListDataTransferLocalCompact and, for a periodic box,
+    EwaldHost, each as one request of the whole host lists."""
+    import ctypes
+    import torch
+    import changa_amd.gpu as gpu
+    n = len(ev.pos)
+    d_nodes = torch.from_numpy(nodes.view(np.uint8).reshape(-1).copy()).cuda()
+    d_parts = torch.from_numpy(parts.view(np.uint8).reshape(-1).copy()).cuda()
+    d_vars = torch.zeros((n, 5), dtype=torch.float32, device="cuda")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    tr, lists = ev.tree, ev.lists
+    keep = []
+    for fn, lst, marks in ((L.TreePieceCellListDataTransferRemote, lists["cells"],
+                            lists["cell_marks"]),
+                           (L.TreePiecePartListDataTransferLocalCompact, lists["parts"],
+                            lists["part_marks"])):
+        r = dict(list=np.ascontiguousarray(lst), marks=np.ascontiguousarray(marks.astype(np.int32)),
+                 starts=np.ascontiguousarray(tr["bucket_start"]),
+                 sizes=np.ascontiguousarray(tr["bucket_size"]))
+        q = Req()
+        q.stream = stream
+        q.d_localMoments = q.d_remoteMoments = d_nodes.data_ptr()
+        q.d_localParts = q.d_remoteParts = d_parts.data_ptr()
+        q.d_localVars = d_vars.data_ptr()
+        q.list = r["list"].ctypes.data
+        q.bucketMarkers = r["marks"].ctypes.data
+        q.bucketStarts = r["starts"].ctypes.data
+        q.bucketSizes = r["sizes"].ctypes.data
+        q.numInteractions = int(marks[-1])
+        q.numBucketsPlusOne = tr["n_buckets"] + 1
+        q.fperiod = ev.L
+        keep.append((q, r))
+        fn(ctypes.byref(q))
+    h = None
+    if ev.w["periodic"]:
+        h = gpu.EwaldData()
+        L.EwaldHostMemorySetup(ctypes.byref(h), n, len(ev.ewt64), 1)
+        for i in range(n):
+            h.EwaldMarkers[i] = i
+        for i, e in enumerate(ev.ewt64.astype(np.float32)):
+            h.ewt[i] = gpu.EwtData(*[float(x) for x in e])
+        ctypes.memmove(h.cachedData, ctypes.byref(ev.ewald_ro), ctypes.sizeof(ev.ewald_ro))
+        h.EwaldRange[0], h.EwaldRange[1] = 0, n - 1
+        L.EwaldHost(ctypes.c_void_p(d_parts.data_ptr()), ctypes.c_void_p(d_vars.data_ptr()),
+                    ctypes.byref(h), stream, None, 0, 1)
+    torch.cuda.synchronize()
+    if h is not None:
+        L.EwaldHostMemoryFree(ctypes.byref(h), 1)
+    msg = ctypes.c_char_p()
+    assert L.changaGpuLastError(ctypes.byref(msg)) == 0, msg.value
+    return d_vars.cpu().numpy()
+
+
+@pytest.mark.parametrize("case", ["plummer", "box_r1"])
+def test_ltw_library_list_requests_equal_main_library(device, case):
+    import torch
+    import changa_amd.gpu as gpu
+    w, R = CASES[case]()
+    ev = engine.Evaluation(dict(w, n_replicas=R), precise=False, device=device,
+                           cell_records=False)
+    torch.cuda.synchronize()
+    nodes, parts = ltw.serialize_evaluation(ev)
+    main = _list_route(gpu.lib(), gpu.CudaRequest, ev, ev.d_moments.cpu().numpy(),
+                       ev.d_parts.cpu().numpy())
+    other = _list_route(gpu.lib_ltw(), gpu.CudaRequestLTW, ev, nodes, parts)
+    assert np.abs(main[:, 0:3]).sum(axis=1).min() > 0
+    np.testing.assert_array_equal(other, main)
