@@ -21,7 +21,7 @@ import pytest
 
 from changa_amd import configs, engine, ltw
 from oracle import oracle as O
-from tests.helpers import check_tails, rel_err, summary
+from tests.helpers import check_tails, deep_chain_workload, rel_err, summary
 
 pytestmark = pytest.mark.gpu
 
@@ -95,8 +95,13 @@ def test_ltw_matches_oracle_and_list_route(device, case):
     nodes, parts = ltw.serialize_evaluation(ev)
     print(f"ltw {case}: {len(parts)} particles, {len(nodes)} nodes, R={R}", flush=True)
     v, lists = _against_oracle(ev, nodes, parts, R, period, case)
-    # the walk's lists are the host builder's here: the list route of the
-    # main library on the host lists sums the same terms in another order
+    _check_list_route(ev, v, lists, case)
+
+
+def _check_list_route(ev, v, lists, label):
+    """The walk's lists are the host builder's: the list route of the main
+    library on the host lists sums the same terms in another order."""
+    import torch
     assert _entries(lists) == _entries(ev.lists)
     ev.zero()
     ev.run_pc()
@@ -105,9 +110,26 @@ def test_ltw_matches_oracle_and_list_route(device, case):
     ref = ev.d_vars.cpu().numpy()
     scale = np.abs(ref[:, :4]).max(axis=0)
     d = (np.abs(v[:, :4].astype(np.float64) - ref[:, :4]) / scale).max()
-    print(f"ltw {case}: vs list route {d:.2e} of the largest component", flush=True)
+    print(f"ltw {label}: vs list route {d:.2e} of the largest component", flush=True)
     assert d <= 2e-6
     np.testing.assert_array_equal(v[:, 4], ref[:, 4])
+
+
+def test_ltw_long_runs(device):
+    """~60 tree levels ending in one bucket of ~950 particles: runs of one
+    nodeId far longer than a wavefront's 64 targets. Against the list route
+    only: in FP32 absolute coordinates (the plain form of this route, as of
+    the reference's) the chain's innermost particles coincide (584 distinct
+    positions of 1,500) and their pairs are skipped as r = 0, where the FP64
+    oracle keeps their softened terms."""
+    import torch
+    ev = engine.Evaluation(deep_chain_workload(), precise=False, device=device,
+                           cell_records=False)
+    torch.cuda.synchronize()
+    assert ev.tree["bucket_size"].max() > 3 * 64
+    nodes, parts = ltw.serialize_evaluation(ev)
+    v = ltw.walk(nodes, parts, float(ev.w["theta"]))
+    _check_list_route(ev, v, _oracle_lists(ev, nodes, 0, 0.0), "deep_chain")
 
 
 def test_ltw_node_types(device):
@@ -231,3 +253,33 @@ def test_ltw_library_list_requests_equal_main_library(device, case):
     other = _list_route(gpu.lib_ltw(), gpu.CudaRequestLTW, ev, nodes, parts)
     assert np.abs(main[:, 0:3]).sum(axis=1).min() > 0
     np.testing.assert_array_equal(other, main)
+
+
+def test_ltw_particle_subrange(device):
+    """A request for particles [first, last] that starts and ends inside
+    buckets and off the 64-particle chunk grid: the particles of the buckets
+    inside the range get the full run's results bit for bit (their walk
+    depends on their bucket node only), those of the two cut buckets the
+    same terms summed over fewer targets per wavefront (FP32 order; dtGrav,
+    a max, bit for bit), the others nothing."""
+    import torch
+    ev = engine.Evaluation(configs.plummer(n=8192), precise=False, device=device,
+                           cell_records=False)
+    torch.cuda.synchronize()
+    nodes, parts = ltw.serialize_evaluation(ev)
+    theta = float(ev.w["theta"])
+    full = ltw.walk(nodes, parts, theta)
+    first, last = 1003, 8192 - 1 - 517
+    assert parts["nodeId"][first] == parts["nodeId"][first - 1]
+    assert parts["nodeId"][last] == parts["nodeId"][last + 1]
+    sub = ltw.walk(nodes, parts, theta, first=first, last=last)
+    nid = parts["nodeId"]
+    cut = (nid == nid[first]) | (nid == nid[last])
+    inside = np.zeros(len(parts), dtype=bool)
+    inside[first:last + 1] = True
+    np.testing.assert_array_equal(sub[inside & ~cut], full[inside & ~cut])
+    scale = np.abs(full[:, :4]).max(axis=0)
+    d = np.abs(sub[inside & cut, :4].astype(np.float64) - full[inside & cut, :4]) / scale
+    assert d.max() <= 2e-6
+    np.testing.assert_array_equal(sub[inside, 4], full[inside, 4])
+    assert not sub[~inside].any()
