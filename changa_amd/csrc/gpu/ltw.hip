@@ -256,7 +256,10 @@ __device__ bool ltw_walk(const LtwArgs& a, const LtwBucket& b, int lane, unsigne
   return true;
 }
 
-// Four wavefronts per SIMD, as the fused kernel (one target per lane).
+// Four wavefronts per SIMD, as the fused kernel, and one target per lane:
+// one to three targets per lane at three wavefronts (168 VGPRs) measured
+// 5-20% slower on the 2^22 halos, the 128^3 box and the 2^20 sphere (the
+// walk's load latency wants the fourth wavefront).
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ltw_kernel(
     LtwArgs a) {
   __shared__ float4 lds[LtwLds<kLtwPcTile, kLtwPpTile>::f4];
