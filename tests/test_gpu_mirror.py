@@ -159,3 +159,51 @@ def test_gpu_error_mode_returns(device):
     msg = ctypes.c_char_p()
     assert L.changaGpuLastError(ctypes.byref(msg)) != 0
     assert b"missing" in msg.value
+
+
+def test_gpu_ewald_host_small_phase(device):
+    """EwaldHost's two phases (HostCUDA.cu:1917-1946, EwaldKernel :1958-1971,
+    filled as Ewald.cpp:416-441 fills them): the large phase evaluates the
+    particles markers[0 .. EwaldRange[1]], the small phase the contiguous
+    particles EwaldRange[0] .. EwaldRange[1]. The same particles through
+    either phase get the same result bit for bit; no other particle is
+    touched."""
+    import torch
+    L = gpu.lib()
+    ev = engine.Evaluation(configs.zeldovich(ng=12, rms=0.05), precise=False, device=device,
+                           cell_records=False)
+    torch.cuda.synchronize()
+    n = ev.pos.shape[0]
+    a, b = 211, n - 307
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    dp = ctypes.c_void_p(ev.d_parts.data_ptr())
+
+    def run(large):
+        h = gpu.EwaldData()
+        L.EwaldHostMemorySetup(ctypes.byref(h), n, len(ev.ewt64), int(large))
+        for i, e in enumerate(ev.ewt64.astype(np.float32)):
+            h.ewt[i] = gpu.EwtData(*[float(x) for x in e])
+        ro = gpu.EwaldReadOnlyData.from_buffer_copy(ev.ewald_ro)
+        if large:
+            for i in range(b - a + 1):
+                h.EwaldMarkers[i] = a + i
+            h.EwaldRange[0], h.EwaldRange[1] = 0, b - a
+            ro.n = b - a + 1
+        else:
+            assert not h.EwaldMarkers
+            h.EwaldRange[0], h.EwaldRange[1] = a, b
+            ro.n = n
+        ctypes.memmove(h.cachedData, ctypes.byref(ro), ctypes.sizeof(ro))
+        out = torch.zeros((n, 5), dtype=torch.float32, device=device)
+        L.EwaldHost(dp, ctypes.c_void_p(out.data_ptr()), ctypes.byref(h), stream, None, 0,
+                    int(large))
+        torch.cuda.synchronize()
+        L.EwaldHostMemoryFree(ctypes.byref(h), int(large))
+        msg = ctypes.c_char_p()
+        assert L.changaGpuLastError(ctypes.byref(msg)) == 0, msg.value
+        return out.cpu().numpy()
+
+    big, small = run(True), run(False)
+    np.testing.assert_array_equal(small, big)
+    assert np.abs(big[a:b + 1, 0:4]).sum(axis=1).min() > 0
+    assert not big[:a].any() and not big[b + 1:].any()
