@@ -24,13 +24,15 @@ int launch_ewald(const CompactPartData* parts, VariablePartData* vars, const int
 int launch_ewald_root(const double* node64, const EwaldReadOnlyData* ro, double hcut,
                       double* root, EwtData* ewt, hipStream_t s);
 #ifdef GPU_LOCAL_TREE_WALK
-int ltw_work_bytes(int maxCells, int maxRuns, size_t* bytes);
+int ltw_work_bytes(int n, int maxCells, int maxRuns, size_t* bytes);
 int ltw_launch(const CudaMultipoleMoments* nodes, int numNodes, const CompactPartData* parts,
                int numParts, VariablePartData* vars, int first, int last, int root, float theta,
                float thetaMono, int nReplicas, float fperiod, float fperiodY, float fperiodZ,
                void* work, size_t workBytes, int maxCells, int maxRuns, int* errOut,
                hipStream_t s);
 // per-wavefront scratch of a local-tree-walk request: cells and particle runs
+// (229 KB a wavefront; a request runs at most one wavefront per 64
+// particles, up to the device's resident count)
 constexpr int kLtwMaxCells = 16384;
 constexpr int kLtwMaxRuns = 8192;
 #endif
@@ -91,7 +93,10 @@ static int pool_flag(hipStream_t s, int** out) {
   std::lock_guard<std::mutex> lk(g_pool_mu);
   if (!g_pools) g_pools = new std::unordered_map<hipStream_t, Pool>();
   Pool& p = (*g_pools)[s];
-  if (!p.flag) CGPU_CHECK(hipHostMalloc((void**)&p.flag, sizeof(int), hipHostMallocDefault));
+  if (!p.flag) {
+    CGPU_CHECK(hipHostMalloc((void**)&p.flag, sizeof(int), hipHostMallocDefault));
+    *p.flag = 0;
+  }
   *out = p.flag;
   return 0;
 }
@@ -276,8 +281,10 @@ static int local_tree_walk(CudaRequest* d) {
   if (nn > 0x7fffffffll || np > 0x7fffffffll)
     return fail("TreePieceCellListDataTransferLocal: sMoments / sCompactParts too large",
                 hipErrorInvalidValue, __FILE__, __LINE__);
+  if (d->lastParticle < d->firstParticle) return add_callback(s, d->cb);  // no particles
   size_t bytes = 0;
-  if (int e = ltw_work_bytes(kLtwMaxCells, kLtwMaxRuns, &bytes)) return e;
+  const int n = d->lastParticle - d->firstParticle + 1;
+  if (int e = ltw_work_bytes(n, kLtwMaxCells, kLtwMaxRuns, &bytes)) return e;
   void* work = nullptr;
   int* flag = nullptr;
   if (int e = pool_get(s, S_LTW, bytes, &work)) return e;

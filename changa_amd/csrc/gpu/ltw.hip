@@ -351,13 +351,17 @@ __global__ void ltw_reset_kernel(int* counters) {
   if (i < kLtwRanges * kLtwCounterStride + 1) counters[i] = 0;  // counters, then err
 }
 
-static int ltw_grid(int* grid) {
+// Wavefronts of a request for n particles: the resident maximum, but no
+// more than its 64-particle chunks (each wavefront holds its own scratch).
+static int ltw_grid(int n, int* grid) {
   int dev = 0, ncu = 0, per = 0;
   CGPU_CHECK(hipGetDevice(&dev));
   CGPU_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   CGPU_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)ltw_kernel, kWave,
                                                           0));
-  *grid = ncu * (per > 0 ? per : 1);
+  const int full = ncu * (per > 0 ? per : 1);
+  const int nchunk = n > 0 ? (n - 1) / kWave + 1 : 1;
+  *grid = full < nchunk ? full : nchunk;
   return 0;
 }
 
@@ -375,9 +379,10 @@ static LtwWork ltw_layout(int grid, int maxCells, int maxRuns) {
   return w;
 }
 
-int ltw_work_bytes(int maxCells, int maxRuns, size_t* bytes) {
+// Work bytes of a request for n particles.
+int ltw_work_bytes(int n, int maxCells, int maxRuns, size_t* bytes) {
   int grid = 0;
-  if (int e = ltw_grid(&grid)) return e;
+  if (int e = ltw_grid(n, &grid)) return e;
   if ((long long)grid * maxCells > 0x7fffffffll || (long long)grid * maxRuns > 0x7fffffffll)
     return fail("local tree walk: scratch capacities exceed the int32 index range",
                 hipErrorInvalidValue, __FILE__, __LINE__);
@@ -407,10 +412,11 @@ int ltw_launch(const CudaMultipoleMoments* nodes, int numNodes, const CompactPar
   if (nReplicas > 0 && (!(fperiod > 0.0f) || fperiodY != fperiod || fperiodZ != fperiod))
     return fail("local tree walk: periodic replicas need one period on all three axes",
                 hipErrorInvalidValue, __FILE__, __LINE__);
+  const int n = last - first + 1;
   int grid = 0;
-  if (int e = ltw_grid(&grid)) return e;
+  if (int e = ltw_grid(n, &grid)) return e;
   size_t need = 0;
-  if (int e = ltw_work_bytes(maxCells, maxRuns, &need)) return e;
+  if (int e = ltw_work_bytes(n, maxCells, maxRuns, &need)) return e;
   if (workBytes < need)
     return fail("local tree walk: work buffer too small", hipErrorInvalidValue, __FILE__,
                 __LINE__);

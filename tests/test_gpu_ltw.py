@@ -255,6 +255,27 @@ def test_ltw_library_list_requests_equal_main_library(device, case):
     np.testing.assert_array_equal(other, main)
 
 
+def test_ltw_small_and_empty_requests(device):
+    """A request for one bucket's particles runs one wavefront (its scratch
+    sized to the request) and equals the full run there bit for bit; an
+    empty request (lastParticle < firstParticle) changes nothing and reports
+    no error."""
+    import torch
+    ev = engine.Evaluation(configs.plummer(n=4096), precise=False, device=device,
+                           cell_records=False)
+    torch.cuda.synchronize()
+    nodes, parts = ltw.serialize_evaluation(ev)
+    theta = float(ev.w["theta"])
+    full = ltw.walk(nodes, parts, theta)
+    b = len(ev.tree["bucket_node"]) // 2
+    st, sz = int(ev.tree["bucket_start"][b]), int(ev.tree["bucket_size"][b])
+    one = ltw.walk(nodes, parts, theta, first=st, last=st + sz - 1)
+    np.testing.assert_array_equal(one[st:st + sz], full[st:st + sz])
+    assert not one[:st].any() and not one[st + sz:].any()
+    empty = ltw.walk(nodes, parts, theta, first=10, last=9)
+    assert not empty.any()
+
+
 def test_ltw_particle_subrange(device):
     """A request for particles [first, last] that starts and ends inside
     buckets and off the 64-particle chunk grid: the particles of the buckets
