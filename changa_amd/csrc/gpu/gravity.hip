@@ -180,6 +180,9 @@ extern "C" int changaCellRecords(const CudaMultipoleMoments* moments, const doub
                                  int numCells, void* records, void* stream) {
   if (numCells < 0 || (numCells > 0 && (!moments || !records)))
     return fail("changaCellRecords: missing argument", hipErrorInvalidValue, __FILE__, __LINE__);
+  if (numCells > CHANGA_CELL_RECORD_MAX)
+    return fail("changaCellRecords: more than CHANGA_CELL_RECORD_MAX cells", hipErrorInvalidValue,
+                __FILE__, __LINE__);
   if (reinterpret_cast<uintptr_t>(records) % CHANGA_CELL_RECORD_BYTES != 0)
     return fail("changaCellRecords: records not 128-byte aligned", hipErrorInvalidValue,
                 __FILE__, __LINE__);

@@ -330,23 +330,44 @@ __device__ __forceinline__ void stage_cell(const ChangaGravityLaunch& a, const W
     dst[k * STRIDE] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
 }
 
+// Replica code of the central image (x, y, z offsets 0) in offsetID bits
+// 22-30, and whether any of a tile's cells (mine: this lane's is one) comes
+// from another image: a wavefront-uniform answer, so that tiles of central
+// cells skip the offset arithmetic (adding a zero offset changes no bit).
+constexpr int kCentralCode = 3 | (3 << 3) | (3 << 6);
+__device__ __forceinline__ bool tile_shifted(bool mine, int oid) {
+  return __ballot(mine && ((oid >> 22) & 0x1ff) != kCentralCode) != 0ull;
+}
+
 // Replica-shifted centre of a cell relative to the bucket centre (precision
-// form: FP64 subtraction, then FP32) or absolute (plain form).
+// form: FP64 subtraction, then FP32) or absolute (plain form). shifted
+// (wavefront-uniform) false: every cell of the tile is in the central image.
 template <bool PRECISE>
 __device__ __forceinline__ void cell_centre(double mx, double my, double mz, float fx, float fy,
-                                            float fz, int oid, float L, double cx0, double cy0,
-                                            double cz0, float& x, float& y, float& z) {
+                                            float fz, int oid, float L, bool shifted, double cx0,
+                                            double cy0, double cz0, float& x, float& y,
+                                            float& z) {
   // every operation rounded on its own (no contraction into FMAs), so that
   // both staging paths and the records give the same bits
 #pragma clang fp contract(off)
   if (PRECISE) {
-    x = (float)(mx + (double)off_x(oid) * (double)L - cx0);
-    y = (float)(my + (double)off_y(oid) * (double)L - cy0);
-    z = (float)(mz + (double)off_z(oid) * (double)L - cz0);
-  } else {
+    if (shifted) {
+      x = (float)(mx + (double)off_x(oid) * (double)L - cx0);
+      y = (float)(my + (double)off_y(oid) * (double)L - cy0);
+      z = (float)(mz + (double)off_z(oid) * (double)L - cz0);
+    } else {
+      x = (float)(mx - cx0);
+      y = (float)(my - cy0);
+      z = (float)(mz - cz0);
+    }
+  } else if (shifted) {
     x = (float)off_x(oid) * L + fx;
     y = (float)off_y(oid) * L + fy;
     z = (float)off_z(oid) * L + fz;
+  } else {
+    x = fx;
+    y = fy;
+    z = fz;
   }
 }
 
@@ -354,8 +375,8 @@ __device__ __forceinline__ void cell_centre(double mx, double my, double mz, flo
 // dst[k * STRIDE])
 template <bool PRECISE, int STRIDE>
 __device__ __forceinline__ void stage_cell_fwd(const ChangaGravityLaunch& a, const WireCell& w,
-                                               float L, double cx0, double cy0, double cz0,
-                                               float4* dst) {
+                                               float L, bool shifted, double cx0, double cy0,
+                                               double cz0, float4* dst) {
 #pragma clang fp contract(off)
   float r[kFwdRec];
   const int oid = w.oid;
@@ -366,8 +387,8 @@ __device__ __forceinline__ void stage_cell_fwd(const ChangaGravityLaunch& a, con
     my = a.cm64[ci + 1];
     mz = a.cm64[ci + 2];
   }
-  cell_centre<PRECISE>(mx, my, mz, w.v[3], w.v[4], w.v[5], oid, L, cx0, cy0, cz0, r[F_CX],
-                       r[F_CY], r[F_CZ]);
+  cell_centre<PRECISE>(mx, my, mz, w.v[3], w.v[4], w.v[5], oid, L, shifted, cx0, cy0, cz0,
+                       r[F_CX], r[F_CY], r[F_CZ]);
   const float u = w.v[0], u2 = u * u;
   const float s2 = 3.0f * u2, s3 = 7.5f * u2 * u, s4 = 17.5f * u2 * u2;
 #pragma unroll
@@ -411,7 +432,10 @@ __device__ __forceinline__ void load_rec_tile_j(const float4* recs, int cidx, in
                                                 int c0, RecTile<TILE>& f) {
   if constexpr (J < TILE / 8) {
     const int idx = __shfl(cidx, min(8 * J + c0, nc - 1), kWave);
-    f.v[J] = reinterpret_cast<const f32x4*>(recs)[(size_t)idx * (kCellRecFloats / 4) + k];
+    // a 32-bit byte offset from the (wavefront-uniform) base: the records
+    // of at most 2^25 cells span less than 4 GiB
+    const unsigned off = (unsigned)idx * (unsigned)CHANGA_CELL_RECORD_BYTES + (unsigned)k * 16u;
+    f.v[J] = *reinterpret_cast<const f32x4*>(reinterpret_cast<const char*>(recs) + off);
     load_rec_tile_j<J + 1, TILE>(recs, cidx, nc, k, c0, f);
   }
 }
@@ -430,8 +454,8 @@ __device__ __forceinline__ void load_rec_tile(const float4* recs, int cidx, int 
 // cell c. Ends with a barrier.
 template <bool PRECISE, int TILE, int PS>
 __device__ __forceinline__ void stage_rec_tile(const RecTile<TILE>& f, int nc, int lane, int oid,
-                                               float L, double cx0, double cy0, double cz0,
-                                               float4* rec4) {
+                                               float L, bool shifted, double cx0, double cy0,
+                                               double cz0, float4* rec4) {
 #pragma clang fp contract(off)
   f32x4* dst = reinterpret_cast<f32x4*>(rec4) + (lane & 7) * PS + (lane >> 3);
   [&]<int... J>(std::integer_sequence<int, J...>) {
@@ -445,8 +469,8 @@ __device__ __forceinline__ void stage_rec_tile(const RecTile<TILE>& f, int nc, i
     const double my = __hiloint2double(__float_as_int(p0.w), __float_as_int(p0.z));
     const double mz = __hiloint2double(__float_as_int(p1.y), __float_as_int(p1.x));
     float x, y, z;
-    cell_centre<PRECISE>(mx, my, mz, (float)mx, (float)my, (float)mz, oid, L, cx0, cy0, cz0, x,
-                         y, z);
+    cell_centre<PRECISE>(mx, my, mz, (float)mx, (float)my, (float)mz, oid, L, shifted, cx0, cy0,
+                         cz0, x, y, z);
     // F_Q4 + 0 = p5.w (xxxx), +1 = p6.x (xyyy), +2 = p6.y (xxxy), +3 = p6.z (yyyy),
     // +6 = p7.y (xxyy)
     const float q4a = -(p5.w + p7.y), q4b = -(p6.z + p7.y), q4c = -(p6.y + p6.x);
@@ -691,10 +715,11 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
     const bool mine = l.lane < nc;
     const int oid = REC ? oid_next : w.oid;
     const bool soft_tile = __ballot(mine && (oid & CHANGA_ILCELL_SOFTENED)) != 0ull;
+    const bool shifted = tile_shifted(mine, oid);
     if constexpr (REC) {
-      stage_rec_tile<PRECISE, TILE, PS>(rt, nc, l.lane, oid, L, cx0, cy0, cz0, rec4);
+      stage_rec_tile<PRECISE, TILE, PS>(rt, nc, l.lane, oid, L, shifted, cx0, cy0, cz0, rec4);
     } else {
-      if (mine) stage_cell_fwd<PRECISE, PS>(a, w, L, cx0, cy0, cz0, rec4 + l.lane);
+      if (mine) stage_cell_fwd<PRECISE, PS>(a, w, L, shifted, cx0, cy0, cz0, rec4 + l.lane);
       __syncthreads();
     }
     // issued after the barrier, whose fence would otherwise wait for them

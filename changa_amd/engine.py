@@ -322,8 +322,10 @@ class Evaluation:
         self.d_moments = torch.zeros((nn, 27), dtype=torch.float32, device=self.device)
         self.d_node64 = torch.zeros((nn, 34), dtype=torch.float64, device=self.device)
         self.d_cm64 = torch.zeros((nn, 3), dtype=torch.float64, device=self.device)
+        # (the loops address at most gpu.CELL_RECORD_MAX records; larger
+        # trees use the wire moments)
         self.d_cellrec = (torch.zeros((nn, 32), dtype=torch.float32, device=self.device)
-                          if self.cell_records else None)
+                          if self.cell_records and nn <= gpu.CELL_RECORD_MAX else None)
         self.d_starts = self._t(self.t_starts)
         self.d_sizes = self._t(self.t_sizes)
         self.stream = torch.cuda.current_stream(self.device)

@@ -161,6 +161,9 @@ LTW_EXPORTS = [
     "changaEwaldRoot", "changaEwaldResident", "changaCellRecords",
 ]
 
+# cells changaCellRecords accepts (CHANGA_CELL_RECORD_MAX)
+CELL_RECORD_MAX = 1 << 25
+
 # doubles of the device Ewald root record of changaEwaldRoot
 EWALD_ROOT_DOUBLES = 26
 

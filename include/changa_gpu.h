@@ -152,9 +152,11 @@ int changaNodeGravity(const ChangaGravityLaunch *a, void *stream);
    128-byte record per cell, built from the wire moments and the FP64 centres
    of mass (cm64, or NULL: the wire centres), with the moments pre-scaled for
    the hexadecapole evaluation. `records` holds CHANGA_CELL_RECORD_BYTES x
-   numCells bytes, 128-byte aligned (hipMalloc gives 256). Rebuild after every
-   moment build. */
+   numCells bytes, 128-byte aligned (hipMalloc gives 256), for at most
+   CHANGA_CELL_RECORD_MAX cells (the loops address them with 32-bit byte
+   offsets). Rebuild after every moment build. */
 #define CHANGA_CELL_RECORD_BYTES 128
+#define CHANGA_CELL_RECORD_MAX (1 << 25)
 int changaCellRecords(const CudaMultipoleMoments *moments, const double *cm64, int numCells,
                       void *records, void *stream);
 int changaPartGravity(const ChangaGravityLaunch *a, void *stream);
