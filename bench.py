@@ -77,6 +77,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--form", choices=["precise", "plain"], default="precise")
+    ap.add_argument("--plan", default=None,
+                    help="pin the batched plan instead of autotuning it: PCTILE,PPTILE,GRAB,ORDER,"
+                         "K,SOURCE (cell tile, particle tile, buckets per claim, order 1 tree / 2 "
+                         "costliest first, targets per lane, cell source 1 records / 2 wire); "
+                         "profiles pin the plan the untraced bench chooses")
     ap.add_argument("--path", choices=["batched", "per-bucket", "fused", "ltw"], default="batched",
                     help="ltw: the GPU_LOCAL_TREE_WALK request route (plain form)")
     ap.add_argument("--config", type=int, choices=[2, 3, 4, 5], default=3)
@@ -166,7 +171,11 @@ def main():
     n_int = sum(counts.values())
     flops = sum(FLOP[k] * v for k, v in counts.items())
     t0 = time.time()
-    opts = ev.batch_plan()
+    if args.plan:
+        pct, ppt, grab, order, kmax, src = (int(x) for x in args.plan.split(","))
+        opts = ev.batch_plan(pct, ppt, grab, order, targets_per_lane=kmax, cell_source=src)
+    else:
+        opts = ev.batch_plan()
     tune_s = time.time() - t0
     stream = torch.cuda.current_stream()
 
