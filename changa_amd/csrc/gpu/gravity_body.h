@@ -1005,18 +1005,32 @@ __device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int sta
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
       float ss = 0.0f;
       bool far = false;
+      int2 e = make_int2(0, kCentralCode << 22);
+      if (valid) e = src(i);
+      // sources of the central image only: no offset arithmetic (adding a
+      // zero offset changes no bit)
+      const bool shifted = tile_shifted(valid, e.y);
       if (valid) {
-        const int2 e = src(i);
         const int p = e.x, oid = e.y;
         const CompactPartData q = a.sources[p];
         if (PRECISE) {
-          v.x = (float)(a.spos64[3 * (size_t)p] + (double)off_x(oid) * (double)L - cx0);
-          v.y = (float)(a.spos64[3 * (size_t)p + 1] + (double)off_y(oid) * (double)L - cy0);
-          v.z = (float)(a.spos64[3 * (size_t)p + 2] + (double)off_z(oid) * (double)L - cz0);
-        } else {
+          if (shifted) {
+            v.x = (float)(a.spos64[3 * (size_t)p] + (double)off_x(oid) * (double)L - cx0);
+            v.y = (float)(a.spos64[3 * (size_t)p + 1] + (double)off_y(oid) * (double)L - cy0);
+            v.z = (float)(a.spos64[3 * (size_t)p + 2] + (double)off_z(oid) * (double)L - cz0);
+          } else {
+            v.x = (float)(a.spos64[3 * (size_t)p] - cx0);
+            v.y = (float)(a.spos64[3 * (size_t)p + 1] - cy0);
+            v.z = (float)(a.spos64[3 * (size_t)p + 2] - cz0);
+          }
+        } else if (shifted) {
           v.x = (float)off_x(oid) * L + q.position.x;
           v.y = (float)off_y(oid) * L + q.position.y;
           v.z = (float)off_z(oid) * L + q.position.z;
+        } else {
+          v.x = q.position.x;
+          v.y = q.position.y;
+          v.z = q.position.z;
         }
         v.w = q.mass;
         ss = q.soft;
