@@ -1515,6 +1515,11 @@ extern "C" int changaGravityAdjoint(const ChangaAdjointLaunch* a, void* stream) 
   if (pc.numBuckets >= kMaxAdjBuckets)
     return fail("changaGravityAdjoint: more than 2^22 buckets", hipErrorInvalidValue, __FILE__,
                 __LINE__);
+  for (const ChangaGravityLaunch* l : {&pc, &pp})
+    if ((l->fperiodY != 0.0f && l->fperiodY != l->fperiod) ||
+        (l->fperiodZ != 0.0f && l->fperiodZ != l->fperiod))
+      return fail("changaGravityAdjoint: the adjoint takes one period on all three axes",
+                  hipErrorInvalidValue, __FILE__, __LINE__);
   // the ILPart particle list is transposed by source run (its source side
   // needs no atomics per pair); per-particle ILCell lists keep the per-bucket
   // source pass

@@ -339,31 +339,40 @@ __device__ __forceinline__ bool tile_shifted(bool mine, int oid) {
   return __ballot(mine && ((oid >> 22) & 0x1ff) != kCentralCode) != 0ull;
 }
 
+// Replica periods per axis (fperiodY / fperiodZ 0: the cube's fperiod).
+struct Period {
+  float x, y, z;
+};
+__device__ __forceinline__ Period period_of(const ChangaGravityLaunch& a) {
+  return {a.fperiod, a.fperiodY > 0.0f ? a.fperiodY : a.fperiod,
+          a.fperiodZ > 0.0f ? a.fperiodZ : a.fperiod};
+}
+
 // Replica-shifted centre of a cell relative to the bucket centre (precision
 // form: FP64 subtraction, then FP32) or absolute (plain form). shifted
 // (wavefront-uniform) false: every cell of the tile is in the central image.
 template <bool PRECISE>
 __device__ __forceinline__ void cell_centre(double mx, double my, double mz, float fx, float fy,
-                                            float fz, int oid, float L, bool shifted, double cx0,
-                                            double cy0, double cz0, float& x, float& y,
+                                            float fz, int oid, const Period& L, bool shifted,
+                                            double cx0, double cy0, double cz0, float& x, float& y,
                                             float& z) {
   // every operation rounded on its own (no contraction into FMAs), so that
   // both staging paths and the records give the same bits
 #pragma clang fp contract(off)
   if (PRECISE) {
     if (shifted) {
-      x = (float)(mx + (double)off_x(oid) * (double)L - cx0);
-      y = (float)(my + (double)off_y(oid) * (double)L - cy0);
-      z = (float)(mz + (double)off_z(oid) * (double)L - cz0);
+      x = (float)(mx + (double)off_x(oid) * (double)L.x - cx0);
+      y = (float)(my + (double)off_y(oid) * (double)L.y - cy0);
+      z = (float)(mz + (double)off_z(oid) * (double)L.z - cz0);
     } else {
       x = (float)(mx - cx0);
       y = (float)(my - cy0);
       z = (float)(mz - cz0);
     }
   } else if (shifted) {
-    x = (float)off_x(oid) * L + fx;
-    y = (float)off_y(oid) * L + fy;
-    z = (float)off_z(oid) * L + fz;
+    x = (float)off_x(oid) * L.x + fx;
+    y = (float)off_y(oid) * L.y + fy;
+    z = (float)off_z(oid) * L.z + fz;
   } else {
     x = fx;
     y = fy;
@@ -375,8 +384,8 @@ __device__ __forceinline__ void cell_centre(double mx, double my, double mz, flo
 // dst[k * STRIDE])
 template <bool PRECISE, int STRIDE>
 __device__ __forceinline__ void stage_cell_fwd(const ChangaGravityLaunch& a, const WireCell& w,
-                                               float L, bool shifted, double cx0, double cy0,
-                                               double cz0, float4* dst) {
+                                               const Period& L, bool shifted, double cx0,
+                                               double cy0, double cz0, float4* dst) {
 #pragma clang fp contract(off)
   float r[kFwdRec];
   const int oid = w.oid;
@@ -454,8 +463,8 @@ __device__ __forceinline__ void load_rec_tile(const float4* recs, int cidx, int 
 // cell c. Ends with a barrier.
 template <bool PRECISE, int TILE, int PS>
 __device__ __forceinline__ void stage_rec_tile(const RecTile<TILE>& f, int nc, int lane, int oid,
-                                               float L, bool shifted, double cx0, double cy0,
-                                               double cz0, float4* rec4) {
+                                               const Period& L, bool shifted, double cx0,
+                                               double cy0, double cz0, float4* rec4) {
 #pragma clang fp contract(off)
   f32x4* dst = reinterpret_cast<f32x4*>(rec4) + (lane & 7) * PS + (lane >> 3);
   [&]<int... J>(std::integer_sequence<int, J...>) {
@@ -681,7 +690,7 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
                                          Acc64 (&acc)[K]) {
   static_assert(TILE <= kWave && TILE % 8 == 0, "one staged cell per lane");
   constexpr int PS = TILE + 1;  // plane stride (float4)
-  const float L = a.fperiod;
+  const Period L = period_of(a);
   const ILCell* cells = static_cast<const ILCell*>(a.list);
   // list entry of this lane in the tile at base (clamped into the list)
   auto entry_at = [&](int base) { return min(base + min(l.lane, TILE - 1), end - 1); };
@@ -792,20 +801,20 @@ template <bool PRECISE, int KIND, int TILE, bool STREAM = true, typename Consume
 __device__ __forceinline__ void pp_tiles(const ChangaGravityLaunch& a, int start, int end,
                                          double cx0, double cy0, double cz0, int lane,
                                          PpLds<TILE>& s, Consume&& consume) {
-  const float L = a.fperiod;
+  const Period L = period_of(a);
 
   // stage source particle p (replica oid) into tile slot i
   auto stage = [&](int i, int p, int oid) {
     CompactPartData q = a.sources[p];
     float4 v;
     if (PRECISE) {
-      v.x = (float)(a.spos64[3 * (size_t)p] + (double)off_x(oid) * (double)L - cx0);
-      v.y = (float)(a.spos64[3 * (size_t)p + 1] + (double)off_y(oid) * (double)L - cy0);
-      v.z = (float)(a.spos64[3 * (size_t)p + 2] + (double)off_z(oid) * (double)L - cz0);
+      v.x = (float)(a.spos64[3 * (size_t)p] + (double)off_x(oid) * (double)L.x - cx0);
+      v.y = (float)(a.spos64[3 * (size_t)p + 1] + (double)off_y(oid) * (double)L.y - cy0);
+      v.z = (float)(a.spos64[3 * (size_t)p + 2] + (double)off_z(oid) * (double)L.z - cz0);
     } else {
-      v.x = (float)off_x(oid) * L + q.position.x;
-      v.y = (float)off_y(oid) * L + q.position.y;
-      v.z = (float)off_z(oid) * L + q.position.z;
+      v.x = (float)off_x(oid) * L.x + q.position.x;
+      v.y = (float)off_y(oid) * L.y + q.position.y;
+      v.z = (float)off_z(oid) * L.z + q.position.z;
     }
     v.w = q.mass;
     s.src4[i] = v;
@@ -992,7 +1001,7 @@ __device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int sta
                                            const Target (&t)[K], const Lanes& l,
                                            PpFwdLds<TILE>& s, Acc64 (&acc)[K]) {
   constexpr int cap = PpFwdLds<TILE>::cap;
-  const float L = a.fperiod;
+  const Period L = period_of(a);
   const int lane = l.lane;
   const ChunkSphere cs = chunk_sphere<PRECISE, K>(t, l);
   const unsigned long long below = (1ull << lane) - 1ull;
@@ -1015,18 +1024,18 @@ __device__ __forceinline__ void pp_sources(const ChangaGravityLaunch& a, int sta
         const CompactPartData q = a.sources[p];
         if (PRECISE) {
           if (shifted) {
-            v.x = (float)(a.spos64[3 * (size_t)p] + (double)off_x(oid) * (double)L - cx0);
-            v.y = (float)(a.spos64[3 * (size_t)p + 1] + (double)off_y(oid) * (double)L - cy0);
-            v.z = (float)(a.spos64[3 * (size_t)p + 2] + (double)off_z(oid) * (double)L - cz0);
+            v.x = (float)(a.spos64[3 * (size_t)p] + (double)off_x(oid) * (double)L.x - cx0);
+            v.y = (float)(a.spos64[3 * (size_t)p + 1] + (double)off_y(oid) * (double)L.y - cy0);
+            v.z = (float)(a.spos64[3 * (size_t)p + 2] + (double)off_z(oid) * (double)L.z - cz0);
           } else {
             v.x = (float)(a.spos64[3 * (size_t)p] - cx0);
             v.y = (float)(a.spos64[3 * (size_t)p + 1] - cy0);
             v.z = (float)(a.spos64[3 * (size_t)p + 2] - cz0);
           }
         } else if (shifted) {
-          v.x = (float)off_x(oid) * L + q.position.x;
-          v.y = (float)off_y(oid) * L + q.position.y;
-          v.z = (float)off_z(oid) * L + q.position.z;
+          v.x = (float)off_x(oid) * L.x + q.position.x;
+          v.y = (float)off_y(oid) * L.y + q.position.y;
+          v.z = (float)off_z(oid) * L.z + q.position.z;
         } else {
           v.x = q.position.x;
           v.y = q.position.y;

@@ -7,7 +7,8 @@
 // Semantics kept from the reference walk:
 //   - every particle is a target, whatever its bucket's rung;
 //   - the tree is walked from rootIdx once per periodic replica (offsets
-//     x fperiod, y fperiodY, z fperiodZ added to the node's centre of mass);
+//     x fperiod, y fperiodY, z fperiodZ added to the node's centre of mass,
+//     and to the particles of an opened bucket);
 //   - a node is opened (CUDA_openCriterionNode, CUDAMoments.cu:197-242) when
 //     it holds at most 6 particles, or when its opening sphere (radius
 //     max(2/sqrt(3) r / theta, r)) meets the bounding box of the target
@@ -58,7 +59,8 @@ struct LtwArgs {
   const CudaMultipoleMoments* nodes;
   const CompactPartData* parts;
   int numNodes, numParts, first, last, root, R;
-  float theta, thetaMono, period;
+  float theta, thetaMono;
+  float px, py, pz;  // replica periods per axis
   ILCell* cells;  // per wave: maxCells entries
   ILPart* runs;   // per wave: maxRuns entries
   int maxCells, maxRuns;
@@ -206,8 +208,8 @@ __device__ bool ltw_walk(const LtwArgs& a, const LtwBucket& b, int lane, unsigne
       code = (int)(e >> kCoopNodeBits);
       v = ltw_node(a.nodes, nd);
       const int x = (code & 7) - 3, y = ((code >> 3) & 7) - 3, z = ((code >> 6) & 7) - 3;
-      kind = ltw_decide(v, b, (float)x * a.period, (float)y * a.period, (float)z * a.period,
-                        a.theta, a.thetaMono);
+      kind = ltw_decide(v, b, (float)x * a.px, (float)y * a.py, (float)z * a.pz, a.theta,
+                        a.thetaMono);
       // children and particle ranges must lie in their arrays
       if (kind == 0 && (v.c0 < -1 || v.c0 >= a.numNodes || v.c1 < -1 || v.c1 >= a.numNodes)) {
         bad = 1;
@@ -408,9 +410,8 @@ int ltw_launch(const CudaMultipoleMoments* nodes, int numNodes, const CompactPar
   if (nReplicas < 0 || nReplicas > 3 || !(theta > 0.0f) || !(thetaMono > 0.0f))
     return fail("local tree walk: theta, thetaMono, nReplicas (0..3)", hipErrorInvalidValue,
                 __FILE__, __LINE__);
-  // the cell and particle loops apply one period to every axis
-  if (nReplicas > 0 && (!(fperiod > 0.0f) || fperiodY != fperiod || fperiodZ != fperiod))
-    return fail("local tree walk: periodic replicas need one period on all three axes",
+  if (nReplicas > 0 && (!(fperiod > 0.0f) || !(fperiodY > 0.0f) || !(fperiodZ > 0.0f)))
+    return fail("local tree walk: periodic replicas need fperiod, fperiodY, fperiodZ > 0",
                 hipErrorInvalidValue, __FILE__, __LINE__);
   const int n = last - first + 1;
   int grid = 0;
@@ -428,6 +429,8 @@ int ltw_launch(const CudaMultipoleMoments* nodes, int numNodes, const CompactPar
   a.pc.vars = vars;
   a.pc.moments = nodes;
   a.pc.fperiod = nReplicas > 0 ? fperiod : 0.0f;
+  a.pc.fperiodY = nReplicas > 0 ? fperiodY : 0.0f;
+  a.pc.fperiodZ = nReplicas > 0 ? fperiodZ : 0.0f;
   a.pc.listKind = 0;
   a.pp = a.pc;
   a.pp.sources = parts;
@@ -442,7 +445,9 @@ int ltw_launch(const CudaMultipoleMoments* nodes, int numNodes, const CompactPar
   a.R = nReplicas;
   a.theta = theta;
   a.thetaMono = thetaMono;
-  a.period = a.pc.fperiod;
+  a.px = a.pc.fperiod;
+  a.py = a.pc.fperiodY;
+  a.pz = a.pc.fperiodZ;
   a.counters = reinterpret_cast<int*>(wb);
   a.err = a.counters + kLtwRanges * kLtwCounterStride;
   a.cells = reinterpret_cast<ILCell*>(wb + L.cells);

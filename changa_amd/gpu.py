@@ -75,7 +75,7 @@ class ChangaGravityLaunch(ctypes.Structure):
                 ("starts", c_void_p), ("sizes", c_void_p), ("numBuckets", c_int),
                 ("listKind", c_int), ("fperiod", c_float), ("variant", c_int),
                 ("tpos64", c_void_p), ("spos64", c_void_p), ("cm64", c_void_p),
-                ("cellRecords", c_void_p)]
+                ("cellRecords", c_void_p), ("fperiodY", c_float), ("fperiodZ", c_float)]
 
 
 class ChangaBatchOptions(ctypes.Structure):

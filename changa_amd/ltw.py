@@ -75,7 +75,8 @@ def serialize_evaluation(ev):
 class LocalWalk:
     """A TreePieceCellListDataTransferLocal request of the local tree walk on
     device copies of `nodes` and `parts` that stay resident, for repeated
-    runs; theta_mono defaults to theta^4 (TreePiece.cpp:5036)."""
+    runs; `period` is one replica period or three (fperiod, fperiodY,
+    fperiodZ); theta_mono defaults to theta^4 (TreePiece.cpp:5036)."""
 
     def __init__(self, nodes, parts, theta, n_replicas=0, period=0.0, first=0, last=None,
                  root=0, device="cuda:0", theta_mono=None):
@@ -102,7 +103,8 @@ class LocalWalk:
         r.theta = theta
         r.thetaMono = theta * theta * theta * theta if theta_mono is None else theta_mono
         r.nReplicas = n_replicas
-        r.fperiod = r.fperiodY = r.fperiodZ = period
+        px, py, pz = (float(x) for x in np.broadcast_to(np.asarray(period, dtype=float), (3,)))
+        r.fperiod, r.fperiodY, r.fperiodZ = px, py, pz
         r.node, r.remote = True, False
         self.request = r
 

@@ -72,8 +72,8 @@ void TransferParticleVarsBack(VariablePartData *hostBuffer, size_t size, void *d
    particle in [firstParticle, lastParticle] of d_localParts walks the
    serialized local tree d_localMoments (sMoments bytes) from rootIdx, per
    periodic replica, with theta / thetaMono, and gets the PC, softened-cell
-   and PP forces of what the walk accepts (+= / max as above). Periodic
-   requests need fperiodY == fperiodZ == fperiod; a walk beyond the
+   and PP forces of what the walk accepts (+= / max as above), with the
+   replica periods fperiod, fperiodY, fperiodZ per axis; a walk beyond the
    per-wavefront scratch (16,384 cells, 8,192 particle runs) or a child or
    bucket index outside its array is reported as an error. */
 void TreePieceCellListDataTransferLocal(CudaRequest *data);
@@ -145,6 +145,8 @@ typedef struct {
   const void *cellRecords;               /* 128-byte cell records of changaCellRecords (built
                                             from these moments and cm64), or NULL: with them a
                                             tile of cells is fetched as whole cache lines */
+  float fperiodY, fperiodZ;              /* periods of the y and z replica offsets (a box
+                                            that is not a cube); 0: fperiod */
 } ChangaGravityLaunch;
 
 int changaNodeGravity(const ChangaGravityLaunch *a, void *stream);

@@ -465,6 +465,15 @@ static void decode_offset(int32_t oid, double period, double off[3]) {
     off[2] = ((int)((code >> 6) & 7) - 3) * period;
 }
 
+/* the replica offset with a period per axis (a box that is not a cube, as
+   gpuLocalTreeWalk's fperiod / fperiodY / fperiodZ, HostCUDA.cu:789-791) */
+static void decode_offset3(int32_t oid, const double period[3], double off[3]) {
+    uint32_t code = ((uint32_t)oid >> 22) & 0x1ff;
+    off[0] = ((int)(code & 7) - 3) * period[0];
+    off[1] = ((int)((code >> 3) & 7) - 3) * period[1];
+    off[2] = ((int)((code >> 6) & 7) - 3) * period[2];
+}
+
 /* partBucketForce for one source (position already offset) */
 static int64_t pp_source(const double sp[3], double sm, double ss, int first, int count,
                          const double *pos, const double *mass, const double *soft,
@@ -496,6 +505,21 @@ void orc_forces_lists(int n_buckets, const int32_t *bucket_node,
                       const int32_t *parts, const int64_t *part_marks,
                       double period, int soft_flags, int n_threads,
                       double *acc, double *pot, double *dtgrav, int64_t counts[3]) {
+    const double p3[3] = {period, period, period};
+    orc_forces_lists3(n_buckets, bucket_node, node_first, node_count, geo, mom, pos, mass, soft,
+                      rung, active_rung, cells, cell_marks, parts, part_marks, p3, soft_flags,
+                      n_threads, acc, pot, dtgrav, counts);
+}
+
+void orc_forces_lists3(int n_buckets, const int32_t *bucket_node,
+                       const int32_t *node_first, const int32_t *node_count,
+                       const double *geo, const double *mom,
+                       const double *pos, const double *mass, const double *soft,
+                       const int32_t *rung, int active_rung,
+                       const int32_t *cells, const int64_t *cell_marks,
+                       const int32_t *parts, const int64_t *part_marks,
+                       const double period[3], int soft_flags, int n_threads,
+                       double *acc, double *pot, double *dtgrav, int64_t counts[3]) {
     int64_t n_pc = 0, n_soft = 0, n_pp = 0;
 #ifdef _OPENMP
     if (n_threads > 0) omp_set_num_threads(n_threads);
@@ -508,7 +532,7 @@ void orc_forces_lists(int n_buckets, const int32_t *bucket_node,
         for (int64_t e = cell_marks[b]; e < cell_marks[b + 1]; ++e) {
             int cn = cells[2 * e];
             double off[3];
-            decode_offset(cells[2 * e + 1], period, off);
+            decode_offset3(cells[2 * e + 1], period, off);
             const double *cg = geo + (size_t)cn * 12;
             int softened = soft_flags ? cells[2 * e + 1] < 0
                                       : orc_open_softening(cg, bg, off[0], off[1], off[2]);
@@ -536,7 +560,7 @@ void orc_forces_lists(int n_buckets, const int32_t *bucket_node,
         for (int64_t e = part_marks[b]; e < part_marks[b + 1]; ++e) {
             int idx = parts[3 * e], num = parts[3 * e + 2];
             double off[3];
-            decode_offset(parts[3 * e + 1], period, off);
+            decode_offset3(parts[3 * e + 1], period, off);
             for (int s = idx; s < idx + num; ++s) {
                 double sp[3] = {pos[3 * s] + off[0], pos[3 * s + 1] + off[1], pos[3 * s + 2] + off[2]};
                 n_pp += pp_source(sp, mass[s], soft[s], first, count, pos, mass, soft, rung,

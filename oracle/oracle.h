@@ -94,6 +94,16 @@ void orc_forces_lists(int n_buckets, const int32_t *bucket_node,
                       const int32_t *parts, const int64_t *part_marks,
                       double period, int soft_flags, int n_threads,
                       double *acc, double *pot, double *dtgrav, int64_t counts[3]);
+/* The same with a period per axis (period[3]: x, y, z). */
+void orc_forces_lists3(int n_buckets, const int32_t *bucket_node,
+                       const int32_t *node_first, const int32_t *node_count,
+                       const double *geo, const double *mom,
+                       const double *pos, const double *mass, const double *soft,
+                       const int32_t *rung, int active_rung,
+                       const int32_t *cells, const int64_t *cell_marks,
+                       const int32_t *parts, const int64_t *part_marks,
+                       const double period[3], int soft_flags, int n_threads,
+                       double *acc, double *pot, double *dtgrav, int64_t counts[3]);
 
 /* openSoftening (gravity.h:251-260) for one (cell, bucket, offset) triple. */
 int orc_open_softening(const double *cell_geo, const double *bucket_geo,

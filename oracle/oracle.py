@@ -46,6 +46,8 @@ def lib():
     L.orc_build_moments.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p]
     L.orc_forces_lists.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i,
                                    _p, _p, _p, _p, _d, _i, _i, _p, _p, _p, _p]
+    L.orc_forces_lists3.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i,
+                                    _p, _p, _p, _p, _p, _i, _i, _p, _p, _p, _p]
     L.orc_open_softening.argtypes = [_p, _p, _d, _d, _d]
     L.orc_open_softening.restype = _i
     L.orc_ewald_table.argtypes = [_p, _d, _d, _d, _p, _i]
@@ -146,9 +148,10 @@ def build_moments(tree, pos, mass, soft):
 
 def forces_lists(tree, geo, mom, pos, mass, soft, lists, period=0.0, rung=None,
                  active_rung=0, n_threads=0, buckets=None, soft_flags=False):
-    """nodeBucketForce/partBucketForce over the interaction lists. `buckets`
-    restricts the evaluation to a subset of bucket indices (the lists are then
-    re-sliced). soft_flags: take the softened-cell decision from bit 31 of
+    """nodeBucketForce/partBucketForce over the interaction lists. `period`:
+    the replica period, or three of them (x, y, z) for a box that is not a
+    cube. `buckets` restricts the evaluation to a subset of bucket indices
+    (the lists are then re-sliced). soft_flags: take the softened-cell decision from bit 31 of
     the list entries instead of openSoftening on `geo` (keeps it fixed when
     positions are perturbed). Returns acc[N,3], pot[N], dtgrav[N], counts[3]."""
     n = pos.shape[0]
@@ -165,6 +168,17 @@ def forces_lists(tree, geo, mom, pos, mass, soft, lists, period=0.0, rung=None,
         cells, cmarks = _reslice(cells, cmarks, buckets, 2)
         parts, pmarks = _reslice(parts, pmarks, buckets, 3)
     rg = None if rung is None else np.ascontiguousarray(rung, dtype=np.int32)
+    per = np.atleast_1d(np.asarray(period, dtype=np.float64))
+    if per.size == 3:
+        per3 = np.ascontiguousarray(per)
+        lib().orc_forces_lists3(len(bnode), _ptr(bnode), _ptr(tree["node_first"]),
+                                _ptr(tree["node_count"]), _ptr(geo), _ptr(mom), _ptr(pos),
+                                _ptr(mass), _ptr(soft), _ptr(rg), active_rung,
+                                _ptr(cells), _ptr(cmarks), _ptr(parts), _ptr(pmarks),
+                                _ptr(per3), int(soft_flags), n_threads, _ptr(acc), _ptr(pot),
+                                _ptr(dtg), _ptr(counts))
+        return acc, pot, dtg, counts
+    period = float(per[0])
     lib().orc_forces_lists(len(bnode), _ptr(bnode), _ptr(tree["node_first"]),
                            _ptr(tree["node_count"]), _ptr(geo), _ptr(mom), _ptr(pos),
                            _ptr(mass), _ptr(soft), _ptr(rg), active_rung,
@@ -318,6 +332,7 @@ def ltw_lists(radius, soft, cm, lo, hi, count, ntype, children, bstart, bsize, b
     check that pins this walk's traversal, actions and replica handling.
 
     Node arrays are per node (cm, lo, hi: [n, 3]; children: [n, 2], -1 none).
+    period: one replica period, or three (fperiod, fperiodY, fperiodZ).
     Returns the lists in the layout of forces_lists (flat int32): cells
     (node, offsetID with the replica code in bits 22-30 and bit 31 for a
     softened cell), parts (bucketStart, offsetID, bucketSize), int64 marks
@@ -348,7 +363,7 @@ def ltw_lists(radius, soft, cm, lo, hi, count, ntype, children, bstart, bsize, b
     fb = np.repeat(np.arange(nb), len(codes))
     fn = np.full(len(fb), root, dtype=np.int64)
     fc = np.tile(codes, nb)
-    per = f(period)
+    per = np.asarray(period, dtype=f)  # one period, or x, y, z
 
     def box_sphere(b, c, r):
         dsq = np.zeros(len(b), dtype=f)

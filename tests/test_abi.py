@@ -38,7 +38,7 @@ def test_wire_struct_sizes():
     assert ctypes.sizeof(gpu.MomcData) == 128
     assert ctypes.sizeof(gpu.EwtData) == 20
     assert ctypes.sizeof(gpu.CudaRequest) > 0
-    assert ctypes.sizeof(gpu.ChangaGravityLaunch) == 8 * 8 + 4 * 4 + 4 * 8
+    assert ctypes.sizeof(gpu.ChangaGravityLaunch) == 8 * 8 + 4 * 4 + 4 * 8 + 2 * 4
 
 
 def test_header_compiles_as_c_and_links(tmp_path):

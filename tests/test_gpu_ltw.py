@@ -11,7 +11,9 @@ of libchanga_gpu_ltw.so on serialized local trees (changa_amd.ltw).
     bit;
   - node types other than bucket / internal act as in CUDA_OptAction (the
     oracle walk on the same records);
-  - invalid records and rectangular periods are reported as errors;
+  - a box that is not a cube (fperiod, fperiodY, fperiodZ) against the
+    oracle with a period per axis;
+  - invalid records and periods are reported as errors;
   - the library's list requests (remote cells, compact particle lists,
     EwaldHost) read the 160-byte node and 24-byte particle records and give
     the main library's results on its own layouts bit for bit.
@@ -38,7 +40,8 @@ CASES = {
 # plummer acc 6.4e-8 / 1.2e-6 / 2.8e-5; softened 2.5e-7 / 4.7e-7 / 5.6e-7;
 # box_r1 1.6e-7 / 3.2e-6 / 1.7e-5; box_r2 2.6e-7 / 1.7e-6 / 4.6e-6; halos
 # 8.7e-7 / 3.2e-5 / 2.4e-4 (close pairs in FP32 absolute coordinates, as in
-# the reference's GPU walk); potentials <= 1.3e-6.
+# the reference's GPU walk); the 1 x 1.5 x 0.75 box 2.5e-7 / 7.4e-6 / 4.9e-5;
+# potentials <= 1.3e-6.
 BARS = {
     "plummer": ((9.5e-8, 1.8e-6, 4.2e-5), (3.4e-8, 1.3e-7, 2.6e-7), (9.3e-7, 7.5e-6, 3.1e-5)),
     "softened": ((3.8e-7, 7.1e-7, 8.4e-7), (8e-8, 2e-7, 2.4e-7), (8.7e-8, 2.2e-6, 3.9e-6)),
@@ -46,6 +49,7 @@ BARS = {
     "box_r2": ((3.8e-7, 2.6e-6, 6.9e-6), (2.8e-8, 1e-7, 1.4e-7), (9.8e-7, 5.3e-6, 1.2e-5)),
     "halos": ((1.3e-6, 4.8e-5, 3.6e-4), (6.5e-8, 6.9e-7, 2e-6), (5e-6, 6e-5, 1.1e-4)),
     "types": ((9.5e-8, 2.2e-6, 1.9e-5), (3.7e-8, 1.3e-7, 2.5e-7), (6e-7, 6e-6, 2.1e-5)),
+    "rectangular": ((3.8e-7, 1.1e-5, 7.3e-5), (4e-8, 1.3e-7, 1.8e-7), (2e-6, 1.8e-5, 5.6e-5)),
 }
 
 
@@ -132,6 +136,30 @@ def test_ltw_long_runs(device):
     _check_list_route(ev, v, _oracle_lists(ev, nodes, 0, 0.0), "deep_chain")
 
 
+def test_ltw_rectangular_box(device):
+    """Periods 1 x 1.5 x 0.75 with one replica shell: the walk offsets nodes
+    and opened buckets per axis, and the cell and particle loops apply the
+    same per-axis offsets (against the oracle on the same lists)."""
+    import torch
+    rng = np.random.default_rng(11)
+    n, per = 8192, np.array([1.0, 1.5, 0.75])
+    w = dict(configs.plummer(n=n), pos=np.ascontiguousarray(rng.uniform(size=(n, 3)) * per),
+             soft=np.full(n, 0.002), theta=0.6, bucket=16)
+    ev = engine.Evaluation(w, precise=False, device=device, cell_records=False)
+    torch.cuda.synchronize()
+    nodes, parts = ltw.serialize_evaluation(ev)
+    v = ltw.walk(nodes, parts, float(w["theta"]), 1, per, device=ev.device)
+    lists = _oracle_lists(ev, nodes, 1, per)
+    geo, mom = O.build_moments(ev.tree, ev.pos, ev.mass, ev.soft)
+    oa, op, od, _ = O.forces_lists(ev.tree, geo, mom, ev.pos, ev.mass, ev.soft, lists,
+                                   period=per, soft_flags=True)
+    ea, ep, ed = rel_err(v[:, 0:3], oa), rel_err(v[:, 3], op), rel_err(v[:, 4], od)
+    print(f"ltw rectangular: acc {summary(ea)} pot {summary(ep)} dtgrav {summary(ed)}",
+          flush=True)
+    for name, e, b in zip(("acc", "pot", "dtgrav"), (ea, ep, ed), BARS["rectangular"]):
+        check_tails(e, *b, label=f"ltw rectangular {name}")
+
+
 def test_ltw_node_types(device):
     """A local tree with a non-local subtree, a non-local bucket and a
     boundary node: the kernel's actions against the oracle walk's."""
@@ -176,11 +204,11 @@ def test_ltw_reports_invalid_records(device):
     r.sMoments, r.sCompactParts = nodes.nbytes, parts.nbytes
     r.firstParticle, r.lastParticle, r.rootIdx = 0, len(parts) - 1, 0
     r.theta, r.thetaMono, r.nReplicas = 0.7, 0.7 ** 4, 1
-    r.fperiod, r.fperiodY, r.fperiodZ = 1.0, 1.0, 2.0
+    r.fperiod, r.fperiodY, r.fperiodZ = 1.0, 1.0, 0.0
     import ctypes
     L.TreePieceCellListDataTransferLocal(ctypes.byref(r))
     torch.cuda.synchronize()
-    with pytest.raises(RuntimeError, match="one period on all three axes"):
+    with pytest.raises(RuntimeError, match="fperiodY, fperiodZ > 0"):
         gpu.check_ltw("TreePieceCellListDataTransferLocal")
 
 

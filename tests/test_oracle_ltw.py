@@ -172,3 +172,44 @@ def test_serialize_layout():
         assert nodes["bucketStart"][bn] == st and nodes["bucketSize"][bn] == sz
         assert (parts["nodeId"][st:st + sz] == bn).all()
     np.testing.assert_array_equal(nodes["lesser_corner"], geo[:, 6:9].astype(np.float32))
+
+
+def test_rectangular_periods_against_the_image_sum():
+    """A box that is not a cube (gpuLocalTreeWalk's fperiod, fperiodY,
+    fperiodZ): the walk's lists with one replica shell and the FP64 list
+    forces on them against the direct sum over the same 27 images, to the
+    expansion's accuracy; the same lists evaluated with a cubic period
+    disagree by orders of magnitude more (the offsets are per axis)."""
+    rng = np.random.default_rng(5)
+    n, per = 600, np.array([1.0, 1.5, 0.75])
+    pos = rng.uniform(size=(n, 3)) * per
+    m = np.full(n, 1.0 / n)
+    s = np.full(n, 1e-7)
+    tr = host.build_tree(pos, 8)
+    p = tr["perm"]
+    pos, m, s = (np.ascontiguousarray(a[p]) for a in (pos, m, s))
+    geo, mom = O.build_moments(tr, pos, m, s)
+    types, ch = _types(tr)
+    theta = 0.4
+    lists = _ltw(tr, geo, types, ch, theta, 1, per, np.float64)
+    acc, pot, _, _ = O.forces_lists(tr, geo, mom, pos, m, s, lists, period=per, soft_flags=True)
+    ref_a = np.zeros((n, 3))
+    ref_p = np.zeros(n)
+    for ix in (-1, 0, 1):
+        for iy in (-1, 0, 1):
+            for iz in (-1, 0, 1):
+                d = pos[None, :, :] + np.array([ix, iy, iz]) * per - pos[:, None, :]
+                r2 = (d * d).sum(axis=2)
+                if (ix, iy, iz) == (0, 0, 0):
+                    np.fill_diagonal(r2, np.inf)  # no self-interaction
+                ir = 1.0 / np.sqrt(r2)
+                ref_a += (d * (m[None, :] * ir ** 3)[:, :, None]).sum(axis=1)
+                ref_p -= (m[None, :] * ir).sum(axis=1)
+    e = np.linalg.norm(acc - ref_a, axis=1) / np.linalg.norm(ref_a, axis=1)
+    ep = np.abs(pot - ref_p) / np.abs(ref_p)
+    cube, _, _, _ = O.forces_lists(tr, geo, mom, pos, m, s, lists, period=1.0, soft_flags=True)
+    e_cube = np.linalg.norm(cube - ref_a, axis=1) / np.linalg.norm(ref_a, axis=1)
+    print(f"rectangular: acc median {np.median(e):.2e} max {e.max():.2e}, pot max {ep.max():.2e};"
+          f" cubic period: acc median {np.median(e_cube):.2e}")
+    assert np.median(e) < 1e-4 and e.max() < 3e-3 and ep.max() < 1e-4
+    assert np.median(e_cube) > 100 * np.median(e)
