@@ -692,8 +692,12 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
   constexpr int PS = TILE + 1;  // plane stride (float4)
   const Period L = period_of(a);
   const ILCell* cells = static_cast<const ILCell*>(a.list);
+  // cells per tile: the most slot rounds that fit TILE, so that the last
+  // round of a tile keeps every slot busy (with 9 slots, 63 cells in 7 full
+  // rounds rather than 64 in 8, the last of them one cell)
+  const int tc = l.nslot <= TILE ? l.nslot * (TILE / l.nslot) : TILE;
   // list entry of this lane in the tile at base (clamped into the list)
-  auto entry_at = [&](int base) { return min(base + min(l.lane, TILE - 1), end - 1); };
+  auto entry_at = [&](int base) { return min(base + min(l.lane, tc - 1), end - 1); };
   // (end - 1 >= start: the loop below runs only for a non-empty list)
   // REC false: the next tile's wire records. REC true: the next tile's
   // record lines, its offsetIDs, and the entries of the tile after it (read
@@ -709,9 +713,9 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
   // evaluation it is meant to overlap
   auto fetch = [&](int base) {
     if constexpr (REC) {
-      load_rec_tile<TILE>(recs, enext.index, max(1, min(TILE, end - base)), l.lane, rt);
+      load_rec_tile<TILE>(recs, enext.index, max(1, min(tc, end - base)), l.lane, rt);
       oid_next = enext.offsetID;
-      enext = load_entry<STREAM>(cells + entry_at(base + TILE));
+      enext = load_entry<STREAM>(cells + entry_at(base + tc));
     } else {
       load_cell_all<STREAM>(a, cells, entry_at(base), w);
     }
@@ -719,8 +723,8 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
   if (start >= end) return;
   if constexpr (REC) enext = load_entry<STREAM>(cells + entry_at(start));
   fetch(start);
-  for (int base = start; base < end; base += TILE) {
-    const int nc = min(TILE, end - base);
+  for (int base = start; base < end; base += tc) {
+    const int nc = min(tc, end - base);
     const bool mine = l.lane < nc;
     const int oid = REC ? oid_next : w.oid;
     const bool soft_tile = __ballot(mine && (oid & CHANGA_ILCELL_SOFTENED)) != 0ull;
@@ -732,7 +736,7 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
       __syncthreads();
     }
     // issued after the barrier, whose fence would otherwise wait for them
-    fetch(base + TILE);
+    fetch(base + tc);
     if (l.live) {
       Acc32 A[K];
 #pragma unroll
