@@ -694,8 +694,10 @@ __device__ __forceinline__ void pc_cells(const ChangaGravityLaunch& a, int start
   const ILCell* cells = static_cast<const ILCell*>(a.list);
   // cells per tile: the most slot rounds that fit TILE, so that the last
   // round of a tile keeps every slot busy (with 9 slots, 63 cells in 7 full
-  // rounds rather than 64 in 8, the last of them one cell)
-  const int tc = l.nslot <= TILE ? l.nslot * (TILE / l.nslot) : TILE;
+  // rounds rather than 64 in 8, the last of them one cell); a tile of fewer
+  // than two rounds (sparse buckets: many slots) keeps TILE cells, its
+  // per-tile work then weighing more than the partial round
+  const int tc = 2 * l.nslot <= TILE ? l.nslot * (TILE / l.nslot) : TILE;
   // list entry of this lane in the tile at base (clamped into the list)
   auto entry_at = [&](int base) { return min(base + min(l.lane, tc - 1), end - 1); };
   // (end - 1 >= start: the loop below runs only for a non-empty list)
