@@ -110,6 +110,23 @@ def parse():
     return ap.parse_args()
 
 
+def parse_plan(text):
+    """--plan PCTILE,PPTILE,GRAB,ORDER,K,SOURCE -> the six ChangaBatchOptions
+    values, each checked against what changaBatchPlanCreate accepts."""
+    try:
+        v = [int(x) for x in text.split(",")]
+    except ValueError:
+        v = []
+    if len(v) != 6:
+        raise SystemExit("--plan: six integers PCTILE,PPTILE,GRAB,ORDER,K,SOURCE")
+    pct, ppt, grab, order, k, src = v
+    if (pct not in (32, 64) or ppt not in (128, 256) or not 1 <= grab <= 64
+            or order not in (1, 2) or k not in (1, 3) or src not in (1, 2)):
+        raise SystemExit("--plan: cell tile 32|64, particle tile 128|256, 1..64 per claim, "
+                         "order 1|2, targets per lane 1|3, cell source 1|2")
+    return pct, ppt, grab, order, k, src
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -172,7 +189,7 @@ def main():
     flops = sum(FLOP[k] * v for k, v in counts.items())
     t0 = time.time()
     if args.plan:
-        pct, ppt, grab, order, kmax, src = (int(x) for x in args.plan.split(","))
+        pct, ppt, grab, order, kmax, src = parse_plan(args.plan)
         opts = ev.batch_plan(pct, ppt, grab, order, targets_per_lane=kmax, cell_source=src)
     else:
         opts = ev.batch_plan()

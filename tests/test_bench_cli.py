@@ -40,3 +40,18 @@ def test_profile_lookups_name_exact_variants():
         other = f"batch_kernel<{pct},{ppt},{k},{'false' if rec == 'true' else 'true'}>"
         assert bench.traffic_from_profile(tag, other + " graph replays") == (None, None)
         assert bench.duration_from_profile(tag, other) == (None, None)
+
+
+def test_plan_option_parsing():
+    """--plan pins the batched plan the profiles were taken with; bad values
+    are refused before anything touches a device."""
+    import pytest
+    sys.path.insert(0, ROOT)
+    import bench
+
+    assert bench.parse_plan("64,256,1,1,3,2") == (64, 256, 1, 1, 3, 2)
+    assert bench.parse_plan("32,256,4,1,1,1") == (32, 256, 4, 1, 1, 1)
+    for bad in ("64,256,1,1,3", "48,256,1,1,3,2", "64,256,1,1,2,2", "64,256,0,1,3,2",
+                "64,256,1,3,3,2", "a,b,c,d,e,f"):
+        with pytest.raises(SystemExit):
+            bench.parse_plan(bad)
