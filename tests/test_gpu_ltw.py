@@ -81,11 +81,13 @@ def _against_oracle(ev, nodes, parts, R, period, label):
     return v, lists
 
 
-def _entries(lists):
+def _entries(lists, buckets=None):
+    """Per bucket (all, or those of `buckets`): the sets of (node, offsetID
+    bits 22-31) cells and (first, replica code, count) runs."""
     c = lists["cells"].reshape(-1, 2)
     p = lists["parts"].reshape(-1, 3)
     out = []
-    for b in range(len(lists["cell_marks"]) - 1):
+    for b in range(len(lists["cell_marks"]) - 1) if buckets is None else buckets:
         cs = c[lists["cell_marks"][b]:lists["cell_marks"][b + 1]].astype(np.int64)
         ps = p[lists["part_marks"][b]:lists["part_marks"][b + 1]].astype(np.int64)
         out.append(({(n, (o & 0xffffffff) >> 22) for n, o in cs},
@@ -365,10 +367,9 @@ def _explain_flips(ev, nodes, parts, v, ref, bad, R, period, rms, label):
                       nodes["greater_corner"], nodes["particleCount"], nodes["type"],
                       nodes["children"], nodes["bucketStart"], nodes["bucketSize"], sub, theta,
                       theta ** 4, R, period, dtype=np.float32)
-    host = _entries(ev.lists)
+    host = _entries(ev.lists, bsel)
     walked = _entries(l32)
-    flips = [len(walked[k][0] ^ host[b][0]) + len(walked[k][1] ^ host[b][1])
-             for k, b in enumerate(bsel)]
+    flips = [len(a[0] ^ b[0]) + len(a[1] ^ b[1]) for a, b in zip(walked, host)]
     print(f"{label}: {int(bad.sum())} particles in {len(bsel)} buckets off the list route; "
           f"entries that differ from the host lists per bucket {flips}", flush=True)
     assert min(flips) > 0
