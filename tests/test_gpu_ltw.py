@@ -18,7 +18,8 @@ of libchanga_gpu_ltw.so on serialized local trees (changa_amd.ltw).
     oracle walk on the same records);
   - a box that is not a cube (fperiod, fperiodY, fperiodZ) against the
     oracle with a period per axis;
-  - invalid records and periods are reported as errors;
+  - invalid records and periods, and a walk beyond the scratch, are reported
+    as errors;
   - the library's list requests (remote cells, compact particle lists,
     EwaldHost) read the 160-byte node and 24-byte particle records and give
     the main library's results on its own layouts bit for bit.
@@ -272,6 +273,22 @@ def _list_route(L, Req, ev, nodes, parts):
     msg = ctypes.c_char_p()
     assert L.changaGpuLastError(ctypes.byref(msg)) == 0, msg.value
     return d_vars.cpu().numpy()
+
+
+def test_ltw_reports_scratch_overflow(device):
+    """A walk that needs more than a wavefront's scratch (8,192 particle runs:
+    at theta = 0.01 every one of ~10^4 buckets is opened) is reported as an
+    error, not truncated, and the library serves the next request."""
+    import torch
+    w = dict(configs.plummer(n=40000), bucket=4)
+    ev = engine.Evaluation(w, precise=False, device=device, cell_records=False)
+    torch.cuda.synchronize()
+    nodes, parts = ltw.serialize_evaluation(ev)
+    assert ev.tree["n_buckets"] > 8192
+    with pytest.raises(RuntimeError, match="exceeded its scratch"):
+        ltw.walk(nodes, parts, 0.01)
+    v = ltw.walk(nodes, parts, float(w["theta"]))
+    assert np.isfinite(v).all() and (np.abs(v[:, 0:3]).sum(axis=1) > 0).all()
 
 
 @pytest.mark.parametrize("case", ["plummer", "box_r1"])
