@@ -144,6 +144,20 @@ def test_ltw_long_runs(device):
     _check_list_route(ev, v, _oracle_lists(ev, nodes, 0, 0.0), "deep_chain")
 
 
+def test_ltw_three_shells(device):
+    """Three replica shells (343 images pushed onto the walk's stack): the
+    walk's lists are the host builder's and its forces the list route's."""
+    import torch
+    w = configs.zeldovich(ng=8, rms=0.05)
+    ev = engine.Evaluation(dict(w, n_replicas=3), precise=False, device=device,
+                           cell_records=False)
+    torch.cuda.synchronize()
+    nodes, parts = ltw.serialize_evaluation(ev)
+    period = float(w["L"])
+    v = ltw.walk(nodes, parts, float(w["theta"]), 3, period, device=ev.device)
+    _check_list_route(ev, v, _oracle_lists(ev, nodes, 3, period), "box_r3")
+
+
 def test_ltw_rectangular_box(device):
     """Periods 1 x 1.5 x 0.75 with one replica shell: the walk offsets nodes
     and opened buckets per axis, and the cell and particle loops apply the

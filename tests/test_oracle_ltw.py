@@ -20,6 +20,7 @@ CASES = {
     "softened": lambda: (configs.plummer(n=2048, eps=0.5), 0),
     "box_r1": lambda: (configs.zeldovich(ng=12, rms=0.05), 1),
     "box_r2": lambda: (configs.zeldovich(ng=10, rms=0.05), 2),
+    "box_r3": lambda: (configs.zeldovich(ng=8, rms=0.05), 3),
     "halos": lambda: (configs.halos(n=16384), 0),
 }
 
