@@ -89,6 +89,8 @@ def parse():
     ap.add_argument("--n", type=int, default=1 << 22, help="particles of the halo workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy legs")
+    ap.add_argument("--pcie", action="store_true",
+                    help="also time the request route on host lists (PCIe-inclusive rate)")
     ap.add_argument("--cpu-frac", type=float, default=None,
                     help="fraction of the buckets the CPU baseline evaluates "
                          "(default 1 for config 2, 1/4 for 3, 1/10 for 5: about 10-30 s of "
@@ -342,6 +344,11 @@ def main():
         out["roofline"]["profile_frac"] = round(dom_flop / (prof_ms * 1e-3) / PEAK_FP32, 4)
 
     log(f"timed: {value:.1f} G interactions/s")
+    if args.pcie:
+        if ev.active is not None:
+            raise SystemExit("--pcie times every bucket's lists: a workload without an active rung")
+        out["pcie_inclusive"] = pcie_leg(ev, w, args, n_int)
+        log("PCIe-inclusive leg done")
     if not args.no_accuracy:
         def step_ltw():
             # the request's results into the evaluation's, then the Ewald pass
@@ -363,6 +370,70 @@ def main():
         out["cpu_baseline"] = cpu_baseline(ev, frac)
         log("cpu baseline done")
     print(json.dumps(out))
+
+
+def pcie_leg(ev, w, args, n_int):
+    """The HostCUDA.h request route on host lists: per step, zero, one
+    TreePieceCellListDataTransferLocal and one
+    TreePiecePartListDataTransferLocalCompact over all buckets, with the
+    lists in pageable host memory (as Compute.cpp:1158 serializes them) that
+    the library uploads on the stream, plus the Ewald pass of a periodic box;
+    tree, moments and particles stay resident, and the forces are the plain
+    form's. Wall-clock time per step (a pageable upload holds the host),
+    and the step's interactions over it: the rate with the lists' PCIe
+    transfers in it."""
+    import ctypes
+
+    import torch
+
+    from changa_amd import gpu
+    L = gpu.lib()
+    stream = torch.cuda.current_stream()
+    nb = int(ev.tree["n_buckets"])
+    starts = np.ascontiguousarray(ev.tree["bucket_start"], dtype=np.int32)
+    sizes = np.ascontiguousarray(ev.tree["bucket_size"], dtype=np.int32)
+    keep, reqs, nbytes = [starts, sizes], [], 0
+    for name, d_list, d_marks in (("TreePieceCellListDataTransferLocal", ev.d_cells, ev.d_cmarks),
+                                  ("TreePiecePartListDataTransferLocalCompact", ev.d_plist,
+                                   ev.d_pmarks)):
+        lst = np.ascontiguousarray(d_list.cpu().numpy())
+        marks = np.ascontiguousarray(d_marks.cpu().numpy().astype(np.int32))
+        nbytes += lst.nbytes + marks.nbytes
+        q = gpu.CudaRequest()
+        q.stream = stream.cuda_stream
+        q.d_localMoments = ev.d_moments.data_ptr()
+        q.d_localParts = ev.d_parts.data_ptr()
+        q.d_localVars = ev.d_vars.data_ptr()
+        q.list = lst.ctypes.data
+        q.bucketMarkers = marks.ctypes.data
+        q.bucketStarts = starts.ctypes.data
+        q.bucketSizes = sizes.ctypes.data
+        q.numInteractions = int(marks[-1])
+        q.numBucketsPlusOne = nb + 1
+        q.fperiod = ev.L
+        keep += [lst, marks]
+        reqs.append((name, getattr(L, name), q))
+
+    def step():
+        ev.zero()
+        for name, fn, q in reqs:
+            gpu.check(fn(ctypes.byref(q)), name)
+        if w["periodic"]:
+            ev.run_ewald()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    return {"value": round(n_int / (ms * 1e-3) / 1e9, 2), "unit": "G interactions/s",
+            "ms_per_step": round(ms, 3), "host_list_bytes": int(nbytes),
+            "route": "TreePieceCellListDataTransferLocal + "
+                     "TreePiecePartListDataTransferLocalCompact on pageable host lists"
+                     + (" + Ewald" if w["periodic"] else "") + ", plain form"}
 
 
 def _events_ms(fn, steps):
