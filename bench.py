@@ -34,6 +34,12 @@ same tree and lists. config.tree_build_ms (including the positions' upload
 and the download of the topology) and config.list_build_ms report the GPU
 builds' times.
 
+--pcie adds the PCIe-inclusive rate of the request route (HostCUDA.h's
+cell and particle list requests on pageable host lists, uploaded per step)
+beside the resident one; --plan fixes the batched kernel's configuration
+instead of autotuning it; --path picks the step behind `value` (batched,
+per-bucket, fused walk and force, or the GPU_LOCAL_TREE_WALK request).
+
 Prints one JSON line. --gpus N with N > 1 exits: this build runs on one GPU.
 """
 import argparse
